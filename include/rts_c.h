@@ -1,0 +1,244 @@
+/*
+ * rts_c.h -- C ABI of the MI355X-native LibRTS spatial index.
+ *
+ * This is the drop-in boundary for rtspatial::SpatialIndex<float, 2>
+ * (reference: include/rtspatial/spatial_index.cuh:62-931). Every entry point
+ * is extern "C", takes plain pointers and sizes, returns an int status
+ * (RTS_OK = 0) and never throws; rts_last_error() gives the message of the
+ * calling thread's last failure. The C++ facade in include/rtspatial/ wraps
+ * these calls with the reference's class and method names.
+ *
+ * Memory: box, id, update and query arrays may live in device memory or in
+ * pinned (hipHostMalloc) host memory, as the reference's ArrayView allows
+ * (test/test_point_queries.h:62-90 passes pinned_vector views). Insert copies
+ * the boxes into index-owned device memory; every other array is borrowed and
+ * must stay valid until the work queued on `stream` has completed.
+ *
+ * Streams: all work is ordered on the caller's stream. Kernel parameters are
+ * passed by value per launch, so queries on different streams may run
+ * concurrently against an index that is not being mutated.
+ */
+#ifndef RTS_C_H
+#define RTS_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same underlying type as hipStream_t; a hipStream_t converts implicitly. */
+typedef struct ihipStream_t* rts_stream_t;
+
+/* Status codes. */
+enum {
+  RTS_OK = 0,
+  RTS_ERR_INVALID = 1,  /* bad argument (null pointer, bad enum, size limit) */
+  RTS_ERR_HIP = 2,      /* a HIP runtime call failed */
+  RTS_ERR_DEVICE = 3,   /* a kernel reported an internal error (bounded wait expired) */
+  RTS_ERR_MISSING = 4   /* no compatible GPU */
+};
+
+/* Point<float,2> (geom/point.cuh:11-42): 8 bytes. */
+typedef struct rts_point {
+  float x, y;
+} rts_point;
+
+/* Envelope<Point<float,2>> (geom/envelope.cuh:13-109): {min, max}, 16 bytes. */
+typedef struct rts_envelope {
+  float x0, y0, x1, y1;
+} rts_envelope;
+
+/* thrust::pair<size_t, envelope_t> as taken by Update (spatial_index.cuh:280-281): 24 bytes. */
+typedef struct rts_update {
+  uint64_t id;
+  rts_envelope box;
+} rts_update;
+
+/* One result of the collecting handler, in its order
+ * (default_handlers/collecting_handlers.h:9-15): geom_id first. */
+typedef struct rts_pair {
+  uint32_t geom_id;
+  uint32_t query_id;
+} rts_pair;
+
+/* Device-side result queue, the layout of dev::Queue<pair<uint32,uint32>>
+ * (utils/queue.h:12-58) with a 64-bit counter: the reference's uint32 counter
+ * (queue.h:112) cannot hold the ~1e10 pairs of a 10M x 1M intersects join.
+ * Results whose slot is >= capacity are dropped, but the counter still counts
+ * them, so after a launch `*counter` is the exact number of results and
+ * `*counter > capacity` means the launch overflowed (the reference leaves
+ * overflow undefined in release builds). */
+typedef struct rts_dev_queue {
+  rts_pair* data;
+  uint64_t capacity;
+  unsigned long long* counter;
+} rts_dev_queue;
+
+/* Predicate (geom/predicates.h:7-10) plus one addition.
+ *   RTS_INTERSECTS: closed overlap            (envelope.cuh:55-69)
+ *   RTS_CONTAINS:   data box contains query   (envelope.cuh:29-53; the
+ *                   reference's kContains for points and envelopes,
+ *                   shaders_contains_envelope_query_2d.cu:22)
+ *   RTS_WITHIN:     data box lies inside the query envelope (the converse of
+ *                   RTS_CONTAINS; no counterpart in the reference). */
+typedef enum rts_predicate {
+  RTS_INTERSECTS = 0,
+  RTS_CONTAINS = 1,
+  RTS_WITHIN = 2
+} rts_predicate;
+
+/* Result handler: how `arg` of a query is interpreted. The reference selects
+ * its handler at shader compile time (CMakeLists.txt:73-104); here it is a
+ * run-time argument.
+ *   RTS_HANDLER_COUNT:   arg is a device `unsigned long long*`; the number of
+ *                        results is added to it (counting_handlers.h:7-25).
+ *   RTS_HANDLER_COLLECT: arg is a device pointer to an rts_dev_queue; results
+ *                        are appended (collecting_handlers.h:9-31). */
+typedef enum rts_handler {
+  RTS_HANDLER_COUNT = 0,
+  RTS_HANDLER_COLLECT = 1
+} rts_handler;
+
+/* Rebuild policy for mutations. */
+typedef enum rts_rebuild_policy {
+  RTS_REBUILD_SAH = 0,     /* refit; rebuild when SAH cost > sah_rebuild_ratio x cost at build */
+  RTS_REBUILD_NEVER = 1,   /* refit only (the reference's behaviour, spatial_index.cuh:208-336) */
+  RTS_REBUILD_ALWAYS = 2   /* rebuild after every mutation */
+} rts_rebuild_policy;
+
+/* Traversal mode. */
+typedef enum rts_traversal {
+  RTS_TRAVERSAL_AUTO = 0,
+  RTS_TRAVERSAL_THREAD = 1, /* one query per lane */
+  RTS_TRAVERSAL_WAVE = 2    /* one query per wavefront, lanes share its nodes and leaves */
+} rts_traversal;
+
+/* Config (spatial_index.cuh:22-37). OptiX-only fields are accepted and
+ * ignored; the sampling fields drive the selectivity probe of the tuner. */
+typedef struct rts_config {
+  uint64_t max_geometries;       /* initial box capacity; grows on demand (10M) */
+  uint64_t max_queries;          /* queries per launch hint (1M) */
+  int32_t preallocate;           /* allocate max_geometries at init */
+  int32_t prefer_fast_build_geom;  /* ignored (OptiX build flag) */
+  int32_t prefer_fast_build_query; /* ignored (OptiX build flag) */
+  int32_t compact;               /* ignored (OptiX compaction) */
+  float geom_sample_rate;        /* 0.2 */
+  float query_sample_rate;       /* 0.2 */
+  float intersect_cost_weight;   /* 0.95 */
+  uint32_t max_geom_samples;     /* 10000 */
+  uint32_t max_query_samples;    /* 1000 */
+  uint32_t max_parallelism;      /* 512 */
+  /* MI355X additions */
+  uint32_t leaf_size;            /* ranges of <= leaf_size boxes are scanned, not descended (8) */
+  uint32_t rebuild_policy;       /* rts_rebuild_policy (RTS_REBUILD_SAH) */
+  float sah_rebuild_ratio;       /* 1.5 */
+  uint32_t traversal;            /* rts_traversal (AUTO) */
+  uint32_t sort_queries;         /* 0 = auto, 1 = Morton-order queries, 2 = keep input order */
+} rts_config;
+
+typedef struct rts_index rts_index;
+
+/* Fills `cfg` with the reference's defaults plus the additions' defaults. */
+void rts_config_default(rts_config* cfg);
+
+/* SpatialIndex::Init (spatial_index.cuh:83-110). cfg may be NULL (defaults).
+ * The index lives on the current HIP device. */
+int rts_index_create(const rts_config* cfg, rts_index** out);
+int rts_index_destroy(rts_index* index);
+
+/* SpatialIndex::Clear (spatial_index.cuh:118-129). Synchronous. */
+int rts_index_clear(rts_index* index);
+
+/* SpatialIndex::Insert (spatial_index.cuh:142-196). Boxes get the implicit ids
+ * size() .. size()+n-1 (spatial_index.cuh:134-136); ids are never reused. */
+int rts_index_insert(rts_index* index, const rts_envelope* boxes, uint64_t n, rts_stream_t stream);
+
+/* SpatialIndex::Delete (spatial_index.cuh:208-265). `ids` are size_t; ids >= size()
+ * are ignored (spatial_index.cuh:219). A deleted box never matches again
+ * unless Update gives it a new envelope. */
+int rts_index_delete(rts_index* index, const uint64_t* ids, uint64_t n, rts_stream_t stream);
+
+/* SpatialIndex::Update (spatial_index.cuh:280-336): box[id] = envelope. Ids >=
+ * size() are ignored. */
+int rts_index_update(rts_index* index, const rts_update* updates, uint64_t n, rts_stream_t stream);
+
+/* SpatialIndex::Query(Predicate, ArrayView<point_t>, void*, stream)
+ * (spatial_index.cuh:352-391). Only RTS_CONTAINS is defined for points; any
+ * other predicate is RTS_ERR_INVALID (the reference aborts, :387-389). */
+int rts_index_query_points(rts_index* index, rts_predicate pred, const rts_point* queries,
+                           uint64_t n, rts_handler handler, void* arg, rts_stream_t stream);
+
+/* SpatialIndex::Query(Predicate, ArrayView<envelope_t>, void*, stream)
+ * (spatial_index.cuh:407-515). RTS_INTERSECTS, RTS_CONTAINS, RTS_WITHIN. */
+int rts_index_query_envelopes(rts_index* index, rts_predicate pred, const rts_envelope* queries,
+                              uint64_t n, rts_handler handler, void* arg, rts_stream_t stream);
+
+/* Number of ids handed out so far (live and deleted). */
+uint64_t rts_index_size(const rts_index* index);
+
+/* Brings the tree up to date now (the index otherwise rebuilds lazily at the
+ * next query after an Insert). No reference counterpart: the reference builds
+ * inside Insert (spatial_index.cuh:161-190). */
+int rts_index_build(rts_index* index, rts_stream_t stream);
+
+/* Waits for `stream` and reports any error a kernel of this index flagged. */
+int rts_index_sync(rts_index* index, rts_stream_t stream);
+
+typedef struct rts_index_stats {
+  uint64_t n_boxes;        /* ids handed out */
+  uint64_t n_tree;         /* leaves in the current tree */
+  uint64_t builds;         /* full LBVH builds */
+  uint64_t refits;         /* refits after Update/Delete */
+  double sah_at_build;     /* SAH-style cost right after the last build */
+  double sah_current;      /* the last measured cost (after the last refit) */
+  int32_t stale;           /* 1 if the next query will rebuild */
+} rts_index_stats;
+
+int rts_index_get_stats(rts_index* index, rts_index_stats* out);
+
+/* Standalone LDS-staged onesweep radix sort of (uint32 key, uint32 value)
+ * pairs on bits [0, key_bits) of the key, stable. `vals_in` NULL means values
+ * 0..n-1. Outputs must not alias inputs. Scratch is allocated on `stream`. */
+int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                   uint32_t* vals_out, uint64_t n, int key_bits, rts_stream_t stream);
+
+/* Per-phase GPU timing: HIP events recorded on the caller's stream around
+ * each phase's kernels. No reference counterpart (the reference's
+ * IntersectsWhatQueryProfiling, spatial_index.cuh:566-683, times with host
+ * stopwatches and stream syncs). */
+enum {
+  RTS_PHASE_BUILD_MORTON = 0,  /* scene bounds + Morton codes */
+  RTS_PHASE_BUILD_SORT = 1,    /* onesweep radix sort of the codes */
+  RTS_PHASE_BUILD_TREE = 2,    /* leaf gather + Karras hierarchy + refit */
+  RTS_PHASE_REFIT = 3,         /* refit after Update/Delete */
+  RTS_PHASE_MUTATE = 4,        /* Update/Delete scatter */
+  RTS_PHASE_QUERY_ORDER = 5,   /* Morton ordering of the queries */
+  RTS_PHASE_TRAVERSE = 6,      /* traversal + result emission */
+  RTS_PHASE_COUNT = 8
+};
+
+typedef struct rts_profile {
+  uint64_t count[RTS_PHASE_COUNT];
+  double ms[RTS_PHASE_COUNT];
+} rts_profile;
+
+/* Turns phase timing on or off (process-wide; off by default). */
+void rts_profile_enable(int on);
+
+/* Waits for the recorded events, adds them to the totals, copies the totals
+ * to `out` (may be NULL) and clears them if `reset`. */
+int rts_profile_read(rts_profile* out, int reset);
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* rts_last_error(void);
+
+/* Library version string. */
+const char* rts_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTS_C_H */

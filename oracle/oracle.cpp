@@ -1,0 +1,571 @@
+// CPU oracle for the LibRTS spatial-index hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
+// product (rtspatial_amd) never links or calls it.
+//
+// It restates, in plain C++ on the host, what the reference computes:
+//   * exact closed-interval predicates      include/rtspatial/geom/envelope.cuh:29-69
+//   * pair order (geom_id, query_id)         include/rtspatial/default_handlers/collecting_handlers.h:9-31
+//   * the reference tests' input generators  test/test_commons.h:22-67,
+//                                            test/test_envelope_queries.h:121-134
+//   * a brute-force join and an STR R-tree join over those predicates
+//   * the reference's ray-slab formulation of intersects
+//                                            include/rtspatial/details/ray_params.h:53-167,
+//                                            src/shaders/shaders_intersects_envelope_query_2d.cu:14-128
+//     (only used to reproduce the reference's own golden count, which carries one
+//      padded-slab false positive; the exact oracle does not use it).
+//
+// Pinning (what checks each function against something other than itself):
+//   predicates + brute force : known answers 4224111 (test_point_queries.h:27,59),
+//                              5 (:89), 2 and 2->1 (test_envelope_queries.h:35,190,203);
+//                              closed-form tiny cases; R-tree == brute force.
+//   ray formulation          : reproduces 6549771 (test_envelope_queries.h:57,90) and
+//                              differs from the exact join by exactly one pair.
+//   R-tree                   : equality with brute force on seeded inputs.
+//   generators               : the known answers above are only reached with
+//                              the exact libstdc++ mt19937 streams.
+//   hash/count summaries     : equality with summaries recomputed from full pair sets.
+//
+// Deleted boxes are modelled by an optional `alive` mask (nullptr = all alive).
+// Predicate codes: 0 = intersects, 1 = contains (data box contains the query,
+// the reference's kContains), 2 = within (data box inside the query envelope).
+// Query kinds: 0 = points (x, y), 1 = envelopes (x0, y0, x1, y1).
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <thread>
+#include <unordered_set>
+#include <vector>
+
+namespace {
+
+struct Box {
+  float x0, y0, x1, y1;
+};
+
+enum { P_INTERSECTS = 0, P_CONTAINS = 1, P_WITHIN = 2 };
+enum { Q_POINT = 0, Q_ENVELOPE = 1 };
+
+// envelope.cuh:29-38 -- Envelope::Contains(point), closed on both ends.
+inline bool box_contains_point(const Box& b, float px, float py) {
+  return px >= b.x0 && px <= b.x1 && py >= b.y0 && py <= b.y1;
+}
+// envelope.cuh:40-53 -- Envelope::Contains(envelope): `a` contains `o`.
+inline bool box_contains_box(const Box& a, const Box& o) {
+  return o.x0 >= a.x0 && o.x1 <= a.x1 && o.y0 >= a.y0 && o.y1 <= a.y1;
+}
+// envelope.cuh:55-69 -- Envelope::Intersects, closed overlap on both axes.
+inline bool box_intersects(const Box& a, const Box& o) {
+  return o.x0 <= a.x1 && o.x1 >= a.x0 && o.y0 <= a.y1 && o.y1 >= a.y0;
+}
+
+// The predicate evaluated for (data box g, query q). Point queries only have
+// `contains` (spatial_index.cuh:352-391 aborts on anything else); the oracle
+// evaluates the same closed form for every predicate code on points.
+inline bool eval(int pred, int kind, const Box& g, const float* q) {
+  if (kind == Q_POINT) return box_contains_point(g, q[0], q[1]);
+  Box qb{q[0], q[1], q[2], q[3]};
+  switch (pred) {
+    case P_INTERSECTS: return box_intersects(g, qb);
+    case P_CONTAINS: return box_contains_box(g, qb);  // reference kContains: g contains q
+    default: return box_contains_box(qb, g);          // within: q contains g
+  }
+}
+
+inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+int resolve_threads(int t) {
+  if (t > 0) return t;
+  unsigned h = std::thread::hardware_concurrency();
+  return h ? (int)h : 1;
+}
+
+// Runs fn(begin, end, thread_index) over [0, n) split into `nt` contiguous chunks.
+void parallel_ranges(uint64_t n, int nt, const std::function<void(uint64_t, uint64_t, int)>& fn) {
+  if (nt <= 1 || n < 2) {
+    fn(0, n, 0);
+    return;
+  }
+  std::vector<std::thread> ths;
+  uint64_t chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; t++) {
+    uint64_t b = std::min<uint64_t>(n, chunk * t), e = std::min<uint64_t>(n, b + chunk);
+    ths.emplace_back(fn, b, e, t);
+  }
+  for (auto& th : ths) th.join();
+}
+
+// Dynamic work distribution: blocks of `grain` queries handed out by an atomic.
+void parallel_dynamic(uint64_t n, int nt, uint64_t grain,
+                      const std::function<void(uint64_t, uint64_t, int)>& fn) {
+  std::atomic<uint64_t> next{0};
+  auto worker = [&](int t) {
+    for (;;) {
+      uint64_t b = next.fetch_add(grain);
+      if (b >= n) break;
+      fn(b, std::min(n, b + grain), t);
+    }
+  };
+  if (nt <= 1) {
+    worker(0);
+    return;
+  }
+  std::vector<std::thread> ths;
+  for (int t = 0; t < nt; t++) ths.emplace_back(worker, t);
+  for (auto& th : ths) th.join();
+}
+
+// ---------------------------------------------------------------- R-tree ----
+// Sort-Tile-Recursive bulk-loaded R-tree with exact FP32 node bounds. Node
+// tests are necessary conditions for the leaf predicate, so pruning never
+// drops a true pair.
+struct RTree {
+  int fanout = 16;
+  std::vector<Box> nodes;              // bounds of every node (leaves first)
+  std::vector<uint32_t> child_begin;   // per node: first child index (or first entry)
+  std::vector<uint32_t> child_count;
+  std::vector<uint8_t> is_leaf;
+  std::vector<uint32_t> entries;       // leaf entry -> box id
+  std::vector<Box> entry_boxes;        // leaf entry -> box (copied for locality)
+  uint32_t root = 0;
+  bool empty = true;
+};
+
+Box unite(const Box& a, const Box& b) {
+  return Box{std::min(a.x0, b.x0), std::min(a.y0, b.y0), std::max(a.x1, b.x1), std::max(a.y1, b.y1)};
+}
+
+const Box kEmptyBox{INFINITY, INFINITY, -INFINITY, -INFINITY};
+
+// STR packing of `items` (indices into `cb`, the item bounds) into groups of
+// `fanout`; returns the groups as ranges over the reordered `items`.
+void str_order(std::vector<uint32_t>& items, const std::vector<Box>& cb, int fanout) {
+  size_t n = items.size();
+  if (n <= (size_t)fanout) return;
+  size_t leaves = (n + fanout - 1) / fanout;
+  size_t slabs = (size_t)std::ceil(std::sqrt((double)leaves));
+  size_t per_slab = slabs ? ((leaves + slabs - 1) / slabs) * fanout : n;
+  auto cx = [&](uint32_t i) { return 0.5f * (cb[i].x0 + cb[i].x1); };
+  auto cy = [&](uint32_t i) { return 0.5f * (cb[i].y0 + cb[i].y1); };
+  std::sort(items.begin(), items.end(), [&](uint32_t a, uint32_t b) {
+    float xa = cx(a), xb = cx(b);
+    return xa < xb || (xa == xb && a < b);
+  });
+  for (size_t s = 0; s < n; s += per_slab) {
+    size_t e = std::min(n, s + per_slab);
+    std::sort(items.begin() + s, items.begin() + e, [&](uint32_t a, uint32_t b) {
+      float ya = cy(a), yb = cy(b);
+      return ya < yb || (ya == yb && a < b);
+    });
+  }
+}
+
+RTree* rtree_build(const Box* boxes, uint64_t n, const uint8_t* alive, int fanout) {
+  auto* t = new RTree();
+  t->fanout = fanout < 2 ? 16 : fanout;
+  std::vector<uint32_t> ids;
+  ids.reserve(n);
+  for (uint64_t i = 0; i < n; i++)
+    if (!alive || alive[i]) ids.push_back((uint32_t)i);
+  if (ids.empty()) return t;
+  t->empty = false;
+  std::vector<Box> cb(boxes, boxes + n);
+  // Dead boxes stay out of the tree; NaN centres sort arbitrarily but are kept
+  // (their predicates are false anyway).
+  str_order(ids, cb, t->fanout);
+  t->entries = ids;
+  t->entry_boxes.resize(ids.size());
+  for (size_t i = 0; i < ids.size(); i++) t->entry_boxes[i] = boxes[ids[i]];
+  // Level 0: leaves over entries.
+  std::vector<uint32_t> level;
+  for (size_t s = 0; s < ids.size(); s += t->fanout) {
+    size_t e = std::min(ids.size(), s + (size_t)t->fanout);
+    Box b = kEmptyBox;
+    for (size_t i = s; i < e; i++) {
+      const Box& x = t->entry_boxes[i];
+      if (!std::isnan(x.x0)) b = unite(b, x);
+    }
+    level.push_back((uint32_t)t->nodes.size());
+    t->nodes.push_back(b);
+    t->child_begin.push_back((uint32_t)s);
+    t->child_count.push_back((uint32_t)(e - s));
+    t->is_leaf.push_back(1);
+  }
+  while (level.size() > 1) {
+    std::vector<Box> lb(t->nodes.size());
+    for (size_t i = 0; i < t->nodes.size(); i++) lb[i] = t->nodes[i];
+    std::vector<uint32_t> order = level;
+    str_order(order, lb, t->fanout);
+    // Children of one parent must be contiguous in `nodes`: re-emit them.
+    std::vector<uint32_t> next;
+    std::vector<uint32_t> copies;
+    for (size_t s = 0; s < order.size(); s += t->fanout) {
+      size_t e = std::min(order.size(), s + (size_t)t->fanout);
+      uint32_t first = (uint32_t)t->nodes.size();
+      for (size_t i = s; i < e; i++) {
+        uint32_t c = order[i];
+        Box cbox = t->nodes[c];
+        uint32_t cbeg = t->child_begin[c], ccnt = t->child_count[c];
+        uint8_t cleaf = t->is_leaf[c];
+        t->nodes.push_back(cbox);
+        t->child_begin.push_back(cbeg);
+        t->child_count.push_back(ccnt);
+        t->is_leaf.push_back(cleaf);
+      }
+      copies.push_back(first);
+      copies.push_back((uint32_t)(e - s));
+    }
+    for (size_t k = 0; k < copies.size(); k += 2) {
+      uint32_t first = copies[k], cnt = copies[k + 1];
+      Box b = kEmptyBox;
+      for (uint32_t i = 0; i < cnt; i++) b = unite(b, t->nodes[first + i]);
+      next.push_back((uint32_t)t->nodes.size());
+      t->nodes.push_back(b);
+      t->child_begin.push_back(first);
+      t->child_count.push_back(cnt);
+      t->is_leaf.push_back(0);
+    }
+    level.swap(next);
+  }
+  t->root = level[0];
+  return t;
+}
+
+// Necessary condition on a node's bounds for any descendant box to satisfy
+// the predicate against q.
+inline bool node_may_match(int pred, int kind, const Box& n, const float* q) {
+  if (kind == Q_POINT) return box_contains_point(n, q[0], q[1]);
+  Box qb{q[0], q[1], q[2], q[3]};
+  switch (pred) {
+    case P_CONTAINS: return box_contains_box(n, qb);  // g contains q and n contains g
+    default: return box_intersects(n, qb);            // intersects, within
+  }
+}
+
+template <typename Emit>
+void rtree_query_one(const RTree* t, int pred, int kind, const float* q, Emit&& emit,
+                     std::vector<uint32_t>& stack) {
+  if (t->empty) return;
+  stack.clear();
+  if (!node_may_match(pred, kind, t->nodes[t->root], q)) return;
+  stack.push_back(t->root);
+  while (!stack.empty()) {
+    uint32_t v = stack.back();
+    stack.pop_back();
+    uint32_t b = t->child_begin[v], c = t->child_count[v];
+    if (t->is_leaf[v]) {
+      for (uint32_t i = 0; i < c; i++) {
+        if (eval(pred, kind, t->entry_boxes[b + i], q)) emit(t->entries[b + i]);
+      }
+    } else {
+      for (uint32_t i = 0; i < c; i++) {
+        if (node_may_match(pred, kind, t->nodes[b + i], q)) stack.push_back(b + i);
+      }
+    }
+  }
+}
+
+// ------------------------------------------- reference ray formulation ----
+// ray_params.h:67-84 (Compute(envelope, diagonal)) and :137-166 (IsHit), written
+// out in IEEE single precision, operation for operation.
+struct Ray {
+  float ox, oy, dx, dy;
+};
+
+inline Ray ray_of(const Box& e, bool anti_diagonal) {
+  float p1x = e.x0, p1y = e.y0, p2x = e.x1, p2y = e.y1;
+  if (anti_diagonal) {
+    p1x = e.x1;
+    p1y = e.y0;
+    p2x = e.x0;
+    p2y = e.y1;
+  }
+  return Ray{p1x, p1y, p2x - p1x, p2y - p1y};
+}
+
+inline bool ray_hits(const Ray& r, const Box& e) {
+  const float eps = FLT_EPSILON;
+  const float gamma3 = (3 * eps) / (1 - 3 * eps);
+  const float pad = 1 + 2 * gamma3;
+  float t0 = 0, t1 = std::nextafter(1.0f, FLT_MAX);
+  const float o[2] = {r.ox, r.oy}, d[2] = {r.dx, r.dy};
+  const float lo[2] = {e.x0, e.y0}, hi[2] = {e.x1, e.y1};
+  for (int i = 0; i < 2; i++) {
+    volatile float inv = 1 / d[i];  // keep the division a separate rounding
+    float tn = (lo[i] - o[i]) * inv;
+    float tf = (hi[i] - o[i]) * inv;
+    if (tn > tf) std::swap(tn, tf);
+    tf *= pad;
+    t0 = tn > t0 ? tn : t0;
+    t1 = tf < t1 ? tf : t1;
+    if (t0 > t1) return false;
+  }
+  return true;
+}
+
+// shaders_intersects_envelope_query_2d.cu:14-41 (forward IS) and :75-128 (backward).
+inline bool ref_ray_pair(const Box& g, const Box& q) {
+  bool fwd = ray_hits(ray_of(q, true), g) && !ray_hits(ray_of(g, false), q);
+  bool valid = g.x0 < g.x1 && g.y0 < g.y1;  // envelope.cuh:103-105
+  bool bwd = valid && ray_hits(ray_of(g, false), q);
+  return fwd || bwd;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ generators ----
+// test/test_commons.h:22-49 -- GenerateUniformBoxes<float>(n, max_w, max_h):
+// std::mt19937(n), min ~ U[0,1), max = min(min + U[0, max_side], 1).
+void orc_ref_uniform_boxes_f32(uint64_t n, float max_w, float max_h, float* out) {
+  std::mt19937 mt((std::mt19937::result_type)n);
+  std::uniform_real_distribution<float> min_x(0.0, 1.0), min_y(0.0, 1.0);
+  std::uniform_real_distribution<float> w_dist(0.0, max_w), h_dist(0.0, max_h);
+  for (uint64_t i = 0; i < n; i++) {
+    float x = min_x(mt), y = min_y(mt), w = w_dist(mt), h = h_dist(mt);
+    out[4 * i + 0] = x;
+    out[4 * i + 1] = y;
+    out[4 * i + 2] = std::min(x + w, 1.0f);
+    out[4 * i + 3] = std::min(y + h, 1.0f);
+  }
+}
+
+// test/test_commons.h:51-67 -- GenerateUniformPoints<float>(n).
+void orc_ref_uniform_points_f32(uint64_t n, float* out) {
+  std::mt19937 mt((std::mt19937::result_type)n);
+  std::uniform_real_distribution<float> gx(0.0, 1.0), gy(0.0, 1.0);
+  for (uint64_t i = 0; i < n; i++) {
+    float x = gx(mt), y = gy(mt);
+    out[2 * i + 0] = x;
+    out[2 * i + 1] = y;
+  }
+}
+
+// test/test_envelope_queries.h:121-134 -- the distinct update ids drawn with
+// std::mt19937(n_updates) and uniform_int_distribution<size_t>(0, n1 - 1).
+void orc_ref_update_ids(uint64_t n1, uint64_t n_updates, uint64_t* out) {
+  std::unordered_set<size_t> ids;
+  std::uniform_int_distribution<size_t> id_dist(0, n1 - 1);
+  std::mt19937 mt((std::mt19937::result_type)n_updates);
+  uint64_t k = 0;
+  while (ids.size() < n_updates) {
+    size_t id = id_dist(mt);
+    if (ids.find(id) == ids.end()) {
+      out[k++] = id;
+      ids.insert(id);
+    }
+  }
+}
+
+// ---------------------------------------------------------- predicates ----
+int orc_eval(int pred, int kind, const float* box, const float* query) {
+  Box g{box[0], box[1], box[2], box[3]};
+  return eval(pred, kind, g, query) ? 1 : 0;
+}
+
+uint64_t orc_splitmix64(uint64_t x) { return splitmix64(x); }
+
+// ---------------------------------------------------------- brute force ----
+// Counts pairs; per_query (nullable) receives each query's count, hsum/hxor
+// (nullable) each query's sum / xor of splitmix64((q << 32) | g).
+uint64_t orc_bf_summary(int pred, int kind, const float* boxes, uint64_t nb, const uint8_t* alive,
+                        const float* queries, uint64_t nq, int threads, uint64_t* per_query,
+                        uint64_t* hsum, uint64_t* hxor) {
+  const Box* b = reinterpret_cast<const Box*>(boxes);
+  int qs = kind == Q_POINT ? 2 : 4;
+  int nt = resolve_threads(threads);
+  std::vector<uint64_t> part(nt, 0);
+  parallel_dynamic(nq, nt, 64, [&](uint64_t qb, uint64_t qe, int t) {
+    uint64_t tot = 0;
+    for (uint64_t q = qb; q < qe; q++) {
+      const float* qq = queries + q * qs;
+      uint64_t c = 0, s = 0, x = 0;
+      for (uint64_t g = 0; g < nb; g++) {
+        if (alive && !alive[g]) continue;
+        if (eval(pred, kind, b[g], qq)) {
+          c++;
+          if (hsum) {
+            uint64_t h = splitmix64((q << 32) | g);
+            s += h;
+            x ^= h;
+          }
+        }
+      }
+      if (per_query) per_query[q] = c;
+      if (hsum) hsum[q] = s;
+      if (hxor) hxor[q] = x;
+      tot += c;
+    }
+    part[t] += tot;
+  });
+  uint64_t total = 0;
+  for (auto v : part) total += v;
+  return total;
+}
+
+// Fills out[2*i] = query_id, out[2*i+1] = box_id sorted by (query, box).
+// Returns the total number of pairs; writes at most `cap` pairs.
+uint64_t orc_bf_pairs(int pred, int kind, const float* boxes, uint64_t nb, const uint8_t* alive,
+                      const float* queries, uint64_t nq, int threads, uint32_t* out, uint64_t cap) {
+  const Box* b = reinterpret_cast<const Box*>(boxes);
+  int qs = kind == Q_POINT ? 2 : 4;
+  int nt = resolve_threads(threads);
+  std::vector<std::vector<uint32_t>> local(nt);
+  parallel_ranges(nq, nt, [&](uint64_t qb, uint64_t qe, int t) {
+    auto& v = local[t];
+    for (uint64_t q = qb; q < qe; q++) {
+      const float* qq = queries + q * qs;
+      for (uint64_t g = 0; g < nb; g++) {
+        if (alive && !alive[g]) continue;
+        if (eval(pred, kind, b[g], qq)) {
+          v.push_back((uint32_t)q);
+          v.push_back((uint32_t)g);
+        }
+      }
+    }
+  });
+  uint64_t k = 0;
+  for (auto& v : local) {
+    for (size_t i = 0; i < v.size(); i += 2, k++) {
+      if (k < cap) {
+        out[2 * k] = v[i];
+        out[2 * k + 1] = v[i + 1];
+      }
+    }
+  }
+  return k;
+}
+
+// -------------------------------------------------------------- R-tree ----
+void* orc_rtree_build(const float* boxes, uint64_t nb, const uint8_t* alive, int fanout) {
+  return rtree_build(reinterpret_cast<const Box*>(boxes), nb, alive, fanout);
+}
+
+void orc_rtree_free(void* t) { delete static_cast<RTree*>(t); }
+
+uint64_t orc_rtree_summary(void* tree, int pred, int kind, const float* queries, uint64_t nq,
+                           int threads, uint64_t* per_query, uint64_t* hsum, uint64_t* hxor) {
+  const RTree* t = static_cast<const RTree*>(tree);
+  int qs = kind == Q_POINT ? 2 : 4;
+  int nt = resolve_threads(threads);
+  std::vector<uint64_t> part(nt, 0);
+  parallel_dynamic(nq, nt, 16, [&](uint64_t qb, uint64_t qe, int th) {
+    std::vector<uint32_t> stack;
+    uint64_t tot = 0;
+    for (uint64_t q = qb; q < qe; q++) {
+      uint64_t c = 0, s = 0, x = 0;
+      rtree_query_one(t, pred, kind, queries + q * qs, [&](uint32_t g) {
+        c++;
+        if (hsum) {
+          uint64_t h = splitmix64((q << 32) | g);
+          s += h;
+          x ^= h;
+        }
+      }, stack);
+      if (per_query) per_query[q] = c;
+      if (hsum) hsum[q] = s;
+      if (hxor) hxor[q] = x;
+      tot += c;
+    }
+    part[th] += tot;
+  });
+  uint64_t total = 0;
+  for (auto v : part) total += v;
+  return total;
+}
+
+// Pairs (query_id, box_id) sorted by (query, box); returns the total count.
+uint64_t orc_rtree_pairs(void* tree, int pred, int kind, const float* queries, uint64_t nq,
+                         int threads, uint32_t* out, uint64_t cap) {
+  const RTree* t = static_cast<const RTree*>(tree);
+  int qs = kind == Q_POINT ? 2 : 4;
+  int nt = resolve_threads(threads);
+  std::vector<std::vector<uint32_t>> local(nt);
+  parallel_ranges(nq, nt, [&](uint64_t qb, uint64_t qe, int th) {
+    std::vector<uint32_t> stack, hits;
+    auto& v = local[th];
+    for (uint64_t q = qb; q < qe; q++) {
+      hits.clear();
+      rtree_query_one(t, pred, kind, queries + q * qs, [&](uint32_t g) { hits.push_back(g); }, stack);
+      std::sort(hits.begin(), hits.end());
+      for (uint32_t g : hits) {
+        v.push_back((uint32_t)q);
+        v.push_back(g);
+      }
+    }
+  });
+  uint64_t k = 0;
+  for (auto& v : local) {
+    for (size_t i = 0; i < v.size(); i += 2, k++) {
+      if (k < cap) {
+        out[2 * k] = v[i];
+        out[2 * k + 1] = v[i + 1];
+      }
+    }
+  }
+  return k;
+}
+
+// ------------------------------------------- reference ray formulation ----
+// Counts the pairs the reference's two-pass ray formulation reports for
+// envelope intersects; writes up to `cap` (query, box) pairs that it reports
+// but the exact predicate rejects (extra) -- and, separately, pairs the exact
+// predicate accepts that it misses (missing).
+uint64_t orc_refray_intersects(const float* boxes, uint64_t nb, const float* queries, uint64_t nq,
+                               int threads, uint32_t* extra, uint64_t extra_cap, uint64_t* n_extra,
+                               uint32_t* missing, uint64_t missing_cap, uint64_t* n_missing) {
+  const Box* b = reinterpret_cast<const Box*>(boxes);
+  const Box* qb = reinterpret_cast<const Box*>(queries);
+  int nt = resolve_threads(threads);
+  std::vector<uint64_t> cnt(nt, 0);
+  std::vector<std::vector<uint32_t>> ex(nt), mi(nt);
+  parallel_ranges(nq, nt, [&](uint64_t s, uint64_t e, int t) {
+    for (uint64_t q = s; q < e; q++) {
+      for (uint64_t g = 0; g < nb; g++) {
+        bool r = ref_ray_pair(b[g], qb[q]);
+        bool x = box_intersects(b[g], qb[q]);
+        if (r) cnt[t]++;
+        if (r && !x) {
+          ex[t].push_back((uint32_t)q);
+          ex[t].push_back((uint32_t)g);
+        }
+        if (!r && x) {
+          mi[t].push_back((uint32_t)q);
+          mi[t].push_back((uint32_t)g);
+        }
+      }
+    }
+  });
+  uint64_t total = 0, ke = 0, km = 0;
+  for (int t = 0; t < nt; t++) {
+    total += cnt[t];
+    for (size_t i = 0; i < ex[t].size(); i += 2, ke++)
+      if (ke < extra_cap) {
+        extra[2 * ke] = ex[t][i];
+        extra[2 * ke + 1] = ex[t][i + 1];
+      }
+    for (size_t i = 0; i < mi[t].size(); i += 2, km++)
+      if (km < missing_cap) {
+        missing[2 * km] = mi[t][i];
+        missing[2 * km + 1] = mi[t][i + 1];
+      }
+  }
+  if (n_extra) *n_extra = ke;
+  if (n_missing) *n_missing = km;
+  return total;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""ctypes binding of librtspatial_amd.so (the C ABI declared in include/rts_c.h).
+
+The library must have been built (python -m rtspatial_amd.build); there is no
+CPU fallback: loading fails loudly when it is missing.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librtspatial_amd.so")
+
+RTS_OK = 0
+RTS_INTERSECTS, RTS_CONTAINS, RTS_WITHIN = 0, 1, 2
+RTS_HANDLER_COUNT, RTS_HANDLER_COLLECT = 0, 1
+RTS_REBUILD_SAH, RTS_REBUILD_NEVER, RTS_REBUILD_ALWAYS = 0, 1, 2
+RTS_TRAVERSAL_AUTO, RTS_TRAVERSAL_THREAD, RTS_TRAVERSAL_WAVE = 0, 1, 2
+
+
+class rts_config(ctypes.Structure):
+    _fields_ = [
+        ("max_geometries", ctypes.c_uint64),
+        ("max_queries", ctypes.c_uint64),
+        ("preallocate", ctypes.c_int32),
+        ("prefer_fast_build_geom", ctypes.c_int32),
+        ("prefer_fast_build_query", ctypes.c_int32),
+        ("compact", ctypes.c_int32),
+        ("geom_sample_rate", ctypes.c_float),
+        ("query_sample_rate", ctypes.c_float),
+        ("intersect_cost_weight", ctypes.c_float),
+        ("max_geom_samples", ctypes.c_uint32),
+        ("max_query_samples", ctypes.c_uint32),
+        ("max_parallelism", ctypes.c_uint32),
+        ("leaf_size", ctypes.c_uint32),
+        ("rebuild_policy", ctypes.c_uint32),
+        ("sah_rebuild_ratio", ctypes.c_float),
+        ("traversal", ctypes.c_uint32),
+        ("sort_queries", ctypes.c_uint32),
+    ]
+
+
+class rts_dev_queue(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("capacity", ctypes.c_uint64),
+        ("counter", ctypes.c_void_p),
+    ]
+
+
+class rts_index_stats(ctypes.Structure):
+    _fields_ = [
+        ("n_boxes", ctypes.c_uint64),
+        ("n_tree", ctypes.c_uint64),
+        ("builds", ctypes.c_uint64),
+        ("refits", ctypes.c_uint64),
+        ("sah_at_build", ctypes.c_double),
+        ("sah_current", ctypes.c_double),
+        ("stale", ctypes.c_int32),
+    ]
+
+
+RTS_PHASES = ["build_morton", "build_sort", "build_tree", "refit", "mutate", "query_order",
+              "traverse"]
+
+
+class rts_profile(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64 * 8), ("ms", ctypes.c_double * 8)]
+
+
+# (name, restype, argtypes) of every exported entry point declared in rts_c.h.
+_VP, _U64, _I32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+SIGNATURES = [
+    ("rts_config_default", None, [ctypes.POINTER(rts_config)]),
+    ("rts_index_create", _I32, [ctypes.POINTER(rts_config), ctypes.POINTER(_VP)]),
+    ("rts_index_destroy", _I32, [_VP]),
+    ("rts_index_clear", _I32, [_VP]),
+    ("rts_index_insert", _I32, [_VP, _VP, _U64, _VP]),
+    ("rts_index_delete", _I32, [_VP, _VP, _U64, _VP]),
+    ("rts_index_update", _I32, [_VP, _VP, _U64, _VP]),
+    ("rts_index_query_points", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_index_query_envelopes", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_index_size", _U64, [_VP]),
+    ("rts_index_build", _I32, [_VP, _VP]),
+    ("rts_index_sync", _I32, [_VP, _VP]),
+    ("rts_index_get_stats", _I32, [_VP, ctypes.POINTER(rts_index_stats)]),
+    ("rts_sort_pairs", _I32, [_VP, _VP, _VP, _VP, _U64, _I32, _VP]),
+    ("rts_profile_enable", None, [_I32]),
+    ("rts_profile_read", _I32, [ctypes.POINTER(rts_profile), _I32]),
+    ("rts_last_error", ctypes.c_char_p, []),
+    ("rts_version", ctypes.c_char_p, []),
+]
+
+_LIB = None
+
+
+class RtsError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m rtspatial_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc != RTS_OK:
+        msg = lib().rts_last_error().decode(errors="replace")
+        raise RtsError(f"{what} failed (status {rc}): {msg}")

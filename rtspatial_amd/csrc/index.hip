@@ -1,0 +1,565 @@
+// Host side of the C ABI: index state, LBVH build orchestration, mutation and
+// query entry points. Mirrors rtspatial::SpatialIndex<float, 2>
+// (reference include/rtspatial/spatial_index.cuh:62-931) with the OptiX
+// GAS/IAS machinery replaced by one LBVH over every inserted box.
+//
+// Build policy: Insert appends boxes by id and marks the tree stale; the next
+// query (or rts_index_build) rebuilds it once, however many Insert calls came
+// before. Update and Delete write the boxes in place and refit; under
+// RTS_REBUILD_SAH a refit whose tree cost exceeds sah_rebuild_ratio times the
+// cost measured right after the last build marks the tree stale instead of
+// letting it degrade (the reference never rebuilds, spatial_index.cuh:208-336).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "rts_internal.h"
+#include "index_state.h"
+
+namespace rts {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace rts
+
+using namespace rts;
+
+#define RTS_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                    \
+      return RTS_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+
+#define RTS_CHECK_ARG(cond, msg)      \
+  do {                                \
+    if (!(cond)) {                    \
+      set_error(msg);                 \
+      return RTS_ERR_INVALID;         \
+    }                                 \
+  } while (0)
+
+namespace {
+
+template <typename T>
+hipError_t dev_alloc(T** p, uint64_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int ensure_id_capacity(rts_index* ix, uint64_t need, hipStream_t stream) {
+  if (need <= ix->cap) return RTS_OK;
+  uint64_t cap = std::max<uint64_t>(need, ix->cap * 2);
+  cap = std::max<uint64_t>(cap, 1024);
+  float4* nb = nullptr;
+  uint32_t* nl = nullptr;
+  RTS_HIP(dev_alloc(&nb, cap));
+  RTS_HIP(dev_alloc(&nl, cap));
+  if (ix->n) {
+    RTS_HIP(hipMemcpyAsync(nb, ix->boxes, ix->n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    RTS_HIP(hipMemcpyAsync(nl, ix->leaf_of_id, ix->n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                           stream));
+  }
+  // The old buffers may still be read by work queued on any stream.
+  RTS_HIP(hipDeviceSynchronize());
+  dev_free(ix->boxes);
+  dev_free(ix->leaf_of_id);
+  ix->boxes = nb;
+  ix->leaf_of_id = nl;
+  ix->cap = cap;
+  return RTS_OK;
+}
+
+int ensure_tree_capacity(rts_index* ix, uint64_t n) {
+  if (n <= ix->tcap) return RTS_OK;
+  RTS_HIP(hipDeviceSynchronize());
+  uint64_t cap = std::max<uint64_t>(n, ix->tcap + ix->tcap / 2);
+  cap = std::max<uint64_t>(cap, 1024);
+  dev_free(ix->leaf_boxes);
+  dev_free(ix->leaf_ids);
+  dev_free(ix->keys);
+  dev_free(ix->keys_alt);
+  dev_free(ix->vals_alt);
+  dev_free(ix->meta);
+  dev_free(ix->child_bounds);
+  dev_free(ix->parent);
+  dev_free(ix->flags);
+  dev_free(ix->sort_scratch);
+  dev_free(ix->partials);
+  RTS_HIP(dev_alloc(&ix->leaf_boxes, cap));
+  RTS_HIP(dev_alloc(&ix->leaf_ids, cap));
+  RTS_HIP(dev_alloc(&ix->keys, cap));
+  RTS_HIP(dev_alloc(&ix->keys_alt, cap));
+  RTS_HIP(dev_alloc(&ix->vals_alt, cap));
+  RTS_HIP(dev_alloc(&ix->meta, cap));
+  RTS_HIP(dev_alloc(&ix->child_bounds, 2 * cap));
+  RTS_HIP(dev_alloc(&ix->parent, 2 * cap));
+  RTS_HIP(dev_alloc(&ix->flags, cap));
+  RTS_HIP(dev_alloc(&ix->sort_scratch, sort_scratch_bytes(cap)));
+  sort_scratch_carve(ix->sort_scratch, cap, &ix->ss);
+  ix->n_partials_cap = scene_bounds_partials((uint32_t)cap);
+  RTS_HIP(dev_alloc(&ix->partials, ix->n_partials_cap));
+  ix->tcap = cap;
+  return RTS_OK;
+}
+
+// Reads back finished tree-cost measurements without blocking.
+void poll_cost(rts_index* ix) {
+  if (ix->cost_build_pending && hipEventQuery(ix->ev_cost_build) == hipSuccess) {
+    ix->sah_at_build = ix->h_cost[1];
+    ix->sah_current = ix->h_cost[1];
+    ix->cost_build_pending = false;
+  }
+  if (ix->cost_refit_pending && hipEventQuery(ix->ev_cost_refit) == hipSuccess) {
+    ix->sah_current = ix->h_cost[3];
+    ix->cost_refit_pending = false;
+    if (ix->cfg.rebuild_policy == RTS_REBUILD_SAH && ix->sah_at_build > 0 &&
+        ix->sah_current > ix->cfg.sah_rebuild_ratio * ix->sah_at_build && ix->refit_epoch == ix->build_epoch) {
+      ix->stale = true;
+    }
+  }
+}
+
+int measure_cost(rts_index* ix, bool after_build, hipStream_t stream) {
+  if (ix->n_tree == 0) return RTS_OK;
+  if (after_build) {
+    if (ix->cost_build_pending) RTS_HIP(hipEventSynchronize(ix->ev_cost_build));
+  } else if (ix->cost_refit_pending || ix->cost_build_pending) {
+    return RTS_OK;  // a measurement is still in flight; skip this one
+  }
+  RTS_HIP(launch_tree_cost(ix->child_bounds, ix->root_box, (uint32_t)ix->n_tree, ix->cost, stream));
+  double* dst = after_build ? ix->h_cost : ix->h_cost + 2;
+  RTS_HIP(hipMemcpyAsync(dst, ix->cost, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (after_build) {
+    RTS_HIP(hipEventRecord(ix->ev_cost_build, stream));
+    ix->cost_build_pending = true;
+  } else {
+    RTS_HIP(hipEventRecord(ix->ev_cost_refit, stream));
+    ix->cost_refit_pending = true;
+    ix->refit_epoch = ix->build_epoch;
+  }
+  return RTS_OK;
+}
+
+int build_tree(rts_index* ix, hipStream_t stream) {
+  const uint64_t N = ix->n;
+  ix->n_tree = 0;
+  if (N == 0) {
+    ix->stale = false;
+    return RTS_OK;
+  }
+  int rc = ensure_tree_capacity(ix, N);
+  if (rc) return rc;
+  const uint32_t n = (uint32_t)N;
+  const uint32_t parts = scene_bounds_partials(n);
+  PhaseToken ph = phase_begin(RTS_PHASE_BUILD_MORTON, stream);
+  RTS_HIP(launch_scene_bounds(ix->boxes, n, ix->partials, parts, ix->scene, stream));
+  RTS_HIP(launch_morton_boxes(ix->boxes, n, ix->scene, ix->keys, stream));
+  phase_end(ph, stream);
+  bool in_b = false;
+  ph = phase_begin(RTS_PHASE_BUILD_SORT, stream);
+  RTS_HIP(radix_sort_pairs(ix->keys, ix->leaf_ids, ix->keys_alt, ix->vals_alt, n, kMortonBits, true,
+                           ix->ss, stream, &in_b));
+  phase_end(ph, stream);
+  if (in_b) {
+    std::swap(ix->keys, ix->keys_alt);
+    std::swap(ix->leaf_ids, ix->vals_alt);
+  }
+  ph = phase_begin(RTS_PHASE_BUILD_TREE, stream);
+  RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids, n, ix->leaf_boxes, ix->leaf_of_id, stream));
+  RTS_HIP(launch_karras(ix->keys, n, ix->meta, ix->parent, ix->flags, stream));
+  RTS_HIP(launch_refit(ix->leaf_boxes, n, ix->parent, ix->child_bounds, ix->root_box, ix->flags,
+                       ix->error, stream));
+  phase_end(ph, stream);
+  ix->n_tree = N;
+  ix->stale = false;
+  ix->builds++;
+  ix->build_epoch++;
+  if (ix->cfg.rebuild_policy == RTS_REBUILD_SAH) return measure_cost(ix, true, stream);
+  return RTS_OK;
+}
+
+int after_mutation(rts_index* ix, hipStream_t stream) {
+  if (ix->stale) return RTS_OK;  // the next query rebuilds anyway
+  if (ix->cfg.rebuild_policy == RTS_REBUILD_ALWAYS) {
+    ix->stale = true;
+    return RTS_OK;
+  }
+  PhaseToken ph = phase_begin(RTS_PHASE_REFIT, stream);
+  RTS_HIP(launch_refit(ix->leaf_boxes, (uint32_t)ix->n_tree, ix->parent, ix->child_bounds,
+                       ix->root_box, ix->flags, ix->error, stream));
+  phase_end(ph, stream);
+  ix->refits++;
+  if (ix->cfg.rebuild_policy == RTS_REBUILD_SAH) return measure_cost(ix, false, stream);
+  return RTS_OK;
+}
+
+int check_device_error(rts_index* ix) {
+  uint32_t err = 0, sort_err = 0;
+  RTS_HIP(hipMemcpy(&err, ix->error, sizeof(err), hipMemcpyDeviceToHost));
+  if (ix->ss.error)
+    RTS_HIP(hipMemcpy(&sort_err, ix->ss.error, sizeof(sort_err), hipMemcpyDeviceToHost));
+  if (sort_err) err |= kErrSortWait;
+  if (err) {
+    std::string m = "device error:";
+    if (err & kErrSortWait) m += " radix-sort look-back exceeded its wait bound;";
+    if (err & kErrRefitWalk) m += " refit walk left the tree;";
+    if (err & kErrTraversal) m += " traversal exceeded its step bound;";
+    set_error(m);
+    return RTS_ERR_DEVICE;
+  }
+  return RTS_OK;
+}
+
+}  // namespace
+
+namespace rts {
+
+TreeView tree_view(const rts_index* ix) {
+  TreeView t;
+  t.leaf_boxes = ix->leaf_boxes;
+  t.leaf_ids = ix->leaf_ids;
+  t.meta = ix->meta;
+  t.child_bounds = ix->child_bounds;
+  t.root_box = ix->root_box;
+  t.n = (uint32_t)ix->n_tree;
+  t.leaf_size = ix->cfg.leaf_size;
+  t.error = ix->error;
+  return t;
+}
+
+int ensure_built(rts_index* ix, hipStream_t stream) {
+  poll_cost(ix);
+  if (ix->stale) return build_tree(ix, stream);
+  return RTS_OK;
+}
+
+}  // namespace rts
+
+extern "C" {
+
+void rts_config_default(rts_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->max_geometries = 1000ull * 1000 * 10;
+  c->max_queries = 1000ull * 1000;
+  c->preallocate = 0;
+  c->geom_sample_rate = 0.2f;
+  c->query_sample_rate = 0.2f;
+  c->intersect_cost_weight = 0.95f;
+  c->max_geom_samples = 10000;
+  c->max_query_samples = 1000;
+  c->max_parallelism = 512;
+  c->leaf_size = 8;
+  c->rebuild_policy = RTS_REBUILD_SAH;
+  c->sah_rebuild_ratio = 1.5f;
+  c->traversal = RTS_TRAVERSAL_AUTO;
+  c->sort_queries = 0;
+}
+
+const char* rts_last_error(void) { return g_last_error.c_str(); }
+
+const char* rts_version(void) { return "rtspatial_amd 0.1 (gfx950)"; }
+
+int rts_index_create(const rts_config* cfg, rts_index** out) {
+  RTS_CHECK_ARG(out != nullptr, "rts_index_create: out is null");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("rts_index_create: no HIP device");
+    return RTS_ERR_MISSING;
+  }
+  rts_index* ix = new rts_index();
+  if (cfg) ix->cfg = *cfg; else rts_config_default(&ix->cfg);
+  if (ix->cfg.leaf_size == 0) ix->cfg.leaf_size = 8;
+  if (ix->cfg.sah_rebuild_ratio <= 1.0f) ix->cfg.sah_rebuild_ratio = 1.5f;
+  RTS_HIP(hipGetDevice(&ix->device));
+  RTS_HIP(hipMalloc(&ix->small, 256));
+  ix->scene = reinterpret_cast<float4*>(ix->small);
+  ix->root_box = reinterpret_cast<float4*>(ix->small + 16);
+  ix->cost = reinterpret_cast<double*>(ix->small + 32);
+  ix->error = reinterpret_cast<uint32_t*>(ix->small + 64);
+  RTS_HIP(hipMemset(ix->small, 0, 256));
+  RTS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ix->h_cost), 4 * sizeof(double)));
+  RTS_HIP(hipEventCreateWithFlags(&ix->ev_cost_build, hipEventDisableTiming));
+  RTS_HIP(hipEventCreateWithFlags(&ix->ev_cost_refit, hipEventDisableTiming));
+  if (ix->cfg.preallocate && ix->cfg.max_geometries) {
+    int rc = ensure_id_capacity(ix, ix->cfg.max_geometries, nullptr);
+    if (!rc) rc = ensure_tree_capacity(ix, ix->cfg.max_geometries);
+    if (rc) {
+      rts_index_destroy(ix);
+      return rc;
+    }
+  }
+  *out = ix;
+  return RTS_OK;
+}
+
+int rts_index_destroy(rts_index* ix) {
+  if (!ix) return RTS_OK;
+  (void)hipDeviceSynchronize();
+  dev_free(ix->boxes);
+  dev_free(ix->leaf_of_id);
+  dev_free(ix->leaf_boxes);
+  dev_free(ix->leaf_ids);
+  dev_free(ix->keys);
+  dev_free(ix->keys_alt);
+  dev_free(ix->vals_alt);
+  dev_free(ix->meta);
+  dev_free(ix->child_bounds);
+  dev_free(ix->parent);
+  dev_free(ix->flags);
+  dev_free(ix->sort_scratch);
+  dev_free(ix->partials);
+  dev_free(ix->small);
+  if (ix->h_cost) (void)hipHostFree(ix->h_cost);
+  if (ix->ev_cost_build) (void)hipEventDestroy(ix->ev_cost_build);
+  if (ix->ev_cost_refit) (void)hipEventDestroy(ix->ev_cost_refit);
+  delete ix;
+  return RTS_OK;
+}
+
+int rts_index_clear(rts_index* ix) {
+  RTS_CHECK_ARG(ix, "rts_index_clear: null index");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  RTS_HIP(hipDeviceSynchronize());
+  ix->n = 0;
+  ix->n_tree = 0;
+  ix->stale = true;
+  ix->cost_build_pending = ix->cost_refit_pending = false;
+  ix->sah_at_build = ix->sah_current = 0;
+  return RTS_OK;
+}
+
+int rts_index_insert(rts_index* ix, const rts_envelope* boxes, uint64_t n, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_insert: null index");
+  if (n == 0) return RTS_OK;  // spatial_index.cuh:144-146
+  RTS_CHECK_ARG(boxes, "rts_index_insert: null boxes");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_CHECK_ARG(ix->n + n <= kMaxBoxes, "rts_index_insert: more than 2^30-1 boxes");
+  RTS_HIP(hipSetDevice(ix->device));
+  int rc = ensure_id_capacity(ix, ix->n + n, stream);
+  if (rc) return rc;
+  RTS_HIP(hipMemcpyAsync(ix->boxes + ix->n, boxes, n * sizeof(float4), hipMemcpyDefault, stream));
+  ix->n += n;
+  ix->stale = true;
+  return RTS_OK;
+}
+
+int rts_index_delete(rts_index* ix, const uint64_t* ids, uint64_t n, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_delete: null index");
+  if (n == 0) return RTS_OK;
+  RTS_CHECK_ARG(ids, "rts_index_delete: null ids");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  poll_cost(ix);
+  if (ix->n == 0) return RTS_OK;
+  PhaseToken ph = phase_begin(RTS_PHASE_MUTATE, stream);
+  RTS_HIP(launch_delete(ids, n, ix->boxes, ix->n, ix->leaf_boxes, ix->leaf_of_id,
+                        ix->stale ? 0 : ix->n_tree, stream));
+  phase_end(ph, stream);
+  return after_mutation(ix, stream);
+}
+
+int rts_index_update(rts_index* ix, const rts_update* ups, uint64_t n, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_update: null index");
+  if (n == 0) return RTS_OK;
+  RTS_CHECK_ARG(ups, "rts_index_update: null updates");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  poll_cost(ix);
+  if (ix->n == 0) return RTS_OK;
+  PhaseToken ph = phase_begin(RTS_PHASE_MUTATE, stream);
+  RTS_HIP(launch_update(ups, n, ix->boxes, ix->n, ix->leaf_boxes, ix->leaf_of_id,
+                        ix->stale ? 0 : ix->n_tree, stream));
+  phase_end(ph, stream);
+  return after_mutation(ix, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Morton-orders the queries for traversal coherence; returns a permutation
+// allocated on `stream` (freed by the caller on the same stream).
+int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, hipStream_t stream,
+                  void** block_out, const uint32_t** order_out) {
+  *block_out = nullptr;
+  *order_out = nullptr;
+  const size_t words = (size_t)nq;
+  const size_t scratch = sort_scratch_bytes(nq);
+  const size_t bytes = 4 * words * sizeof(uint32_t) + scratch + 256;
+  char* base = nullptr;
+  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), bytes, stream));
+  uint32_t* keys = reinterpret_cast<uint32_t*>(base);
+  uint32_t* vals = keys + words;
+  uint32_t* keys_b = vals + words;
+  uint32_t* vals_b = keys_b + words;
+  char* sbase = reinterpret_cast<char*>(vals_b + words);
+  sbase = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(sbase) + 255) & ~uintptr_t(255));
+  SortScratch ss;
+  sort_scratch_carve(sbase, nq, &ss);
+  if (points) {
+    RTS_HIP(launch_morton_points(static_cast<const float2*>(queries), (uint32_t)nq, ix->scene, keys,
+                                 stream));
+  } else {
+    RTS_HIP(launch_morton_boxes(static_cast<const float4*>(queries), (uint32_t)nq, ix->scene, keys,
+                                stream));
+  }
+  bool in_b = false;
+  RTS_HIP(radix_sort_pairs(keys, vals, keys_b, vals_b, (uint32_t)nq, kMortonBits, true, ss, stream,
+                           &in_b));
+  *block_out = base;
+  *order_out = in_b ? vals_b : vals;
+  return RTS_OK;
+}
+
+int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool points,
+              rts_handler handler, void* arg, hipStream_t stream) {
+  RTS_CHECK_ARG(handler == RTS_HANDLER_COUNT || handler == RTS_HANDLER_COLLECT,
+                "query: unknown handler");
+  RTS_CHECK_ARG(arg, "query: null handler argument");
+  RTS_CHECK_ARG(pred >= RTS_INTERSECTS && pred <= RTS_WITHIN, "query: unknown predicate");
+  RTS_CHECK_ARG(!points || pred == RTS_CONTAINS,
+                "query: point queries support only RTS_CONTAINS (spatial_index.cuh:385-389)");
+  if (nq == 0) return RTS_OK;
+  RTS_CHECK_ARG(queries, "query: null queries");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  if (ix->n == 0) return RTS_OK;  // spatial_index.cuh:354-356
+  int rc = ensure_built(ix, stream);
+  if (rc) return rc;
+  const uint32_t mode = ix->cfg.sort_queries;
+  const bool sort = (mode == 1) || (mode == 0 && nq >= 65536);
+  const uint64_t kChunk = 1ull << 30;
+  for (uint64_t off = 0; off < nq; off += kChunk) {
+    const uint64_t m = std::min(kChunk, nq - off);
+    const void* qp = points ? static_cast<const void*>(static_cast<const float2*>(queries) + off)
+                            : static_cast<const void*>(static_cast<const float4*>(queries) + off);
+    void* block = nullptr;
+    const uint32_t* order = nullptr;
+    if (sort) {
+      PhaseToken ph = phase_begin(RTS_PHASE_QUERY_ORDER, stream);
+      rc = order_queries(ix, qp, m, points, stream, &block, &order);
+      phase_end(ph, stream);
+      if (rc) return rc;
+    }
+    QueryLaunch ql;
+    ql.tree = tree_view(ix);
+    ql.queries = qp;
+    ql.n = (uint32_t)m;
+    ql.qid_base = (uint32_t)off;
+    ql.order = order;
+    ql.pred = pred;
+    ql.point_queries = points ? 1 : 0;
+    ql.sink.mode = handler;
+    ql.sink.arg = arg;
+    ql.traversal = ix->cfg.traversal == RTS_TRAVERSAL_AUTO ? RTS_TRAVERSAL_THREAD : ix->cfg.traversal;
+    PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
+    RTS_HIP(launch_query(ql, stream));
+    phase_end(ph, stream);
+    if (block) RTS_HIP(hipFreeAsync(block, stream));
+  }
+  return RTS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rts_index_query_points(rts_index* ix, rts_predicate pred, const rts_point* q, uint64_t n,
+                           rts_handler handler, void* arg, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_query_points: null index");
+  return run_query(ix, pred, q, n, true, handler, arg, stream);
+}
+
+int rts_index_query_envelopes(rts_index* ix, rts_predicate pred, const rts_envelope* q, uint64_t n,
+                              rts_handler handler, void* arg, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_query_envelopes: null index");
+  return run_query(ix, pred, q, n, false, handler, arg, stream);
+}
+
+uint64_t rts_index_size(const rts_index* ix) { return ix ? ix->n : 0; }
+
+int rts_index_build(rts_index* ix, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_build: null index");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  return ensure_built(ix, stream);
+}
+
+int rts_index_sync(rts_index* ix, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_sync: null index");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  RTS_HIP(hipStreamSynchronize(stream));
+  poll_cost(ix);
+  return check_device_error(ix);
+}
+
+int rts_index_get_stats(rts_index* ix, rts_index_stats* out) {
+  RTS_CHECK_ARG(ix && out, "rts_index_get_stats: null argument");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  poll_cost(ix);
+  out->n_boxes = ix->n;
+  out->n_tree = ix->n_tree;
+  out->builds = ix->builds;
+  out->refits = ix->refits;
+  out->sah_at_build = ix->sah_at_build;
+  out->sah_current = ix->sah_current;
+  out->stale = ix->stale ? 1 : 0;
+  return RTS_OK;
+}
+
+int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                   uint32_t* vals_out, uint64_t n, int key_bits, rts_stream_t stream) {
+  RTS_CHECK_ARG(keys_out && vals_out && (keys_in || n == 0), "rts_sort_pairs: null pointer");
+  RTS_CHECK_ARG(key_bits >= 1 && key_bits <= 32, "rts_sort_pairs: key_bits must be in [1, 32]");
+  RTS_CHECK_ARG(n < (1ull << 30), "rts_sort_pairs: n must be below 2^30");
+  if (n == 0) return RTS_OK;
+  const size_t scratch = sort_scratch_bytes(n);
+  char* base = nullptr;
+  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), 2 * n * sizeof(uint32_t) + scratch + 256,
+                         stream));
+  uint32_t* kb = reinterpret_cast<uint32_t*>(base);
+  uint32_t* vb = kb + n;
+  char* sbase = reinterpret_cast<char*>(
+      (reinterpret_cast<uintptr_t>(vb + n) + 255) & ~uintptr_t(255));
+  SortScratch ss;
+  sort_scratch_carve(sbase, n, &ss);
+  RTS_HIP(hipMemcpyAsync(keys_out, keys_in, n * sizeof(uint32_t), hipMemcpyDefault, stream));
+  if (vals_in)
+    RTS_HIP(hipMemcpyAsync(vals_out, vals_in, n * sizeof(uint32_t), hipMemcpyDefault, stream));
+  bool in_b = false;
+  RTS_HIP(radix_sort_pairs(keys_out, vals_out, kb, vb, (uint32_t)n, key_bits, vals_in == nullptr, ss,
+                           stream, &in_b));
+  if (in_b) {
+    RTS_HIP(hipMemcpyAsync(keys_out, kb, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    RTS_HIP(hipMemcpyAsync(vals_out, vb, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+  }
+  uint32_t err = 0;
+  RTS_HIP(hipMemcpyAsync(&err, ss.error, sizeof(err), hipMemcpyDeviceToHost, stream));
+  RTS_HIP(hipFreeAsync(base, stream));
+  RTS_HIP(hipStreamSynchronize(stream));
+  if (err) {
+    set_error("rts_sort_pairs: look-back exceeded its wait bound");
+    return RTS_ERR_DEVICE;
+  }
+  return RTS_OK;
+}
+
+}  // extern "C"

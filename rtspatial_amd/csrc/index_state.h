@@ -1,0 +1,66 @@
+// The state behind an rts_index handle (opaque to C ABI callers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+
+#include "rts_internal.h"
+
+struct rts_index {
+  rts_config cfg{};
+  int device = 0;
+  std::mutex mu;  // serializes host-side state changes of one index
+
+  // Boxes by id (the reference's envelopes_, spatial_index.cuh:919) and the
+  // sorted position of each id in the current tree.
+  uint64_t n = 0;    // ids handed out
+  uint64_t cap = 0;  // capacity of the by-id arrays
+  float4* boxes = nullptr;
+  uint32_t* leaf_of_id = nullptr;
+
+  // Tree (valid for n_tree leaves unless stale).
+  uint64_t tcap = 0;
+  uint64_t n_tree = 0;
+  bool stale = true;
+  float4* leaf_boxes = nullptr;
+  uint32_t* leaf_ids = nullptr;
+  uint32_t* keys = nullptr;
+  uint32_t* keys_alt = nullptr;
+  uint32_t* vals_alt = nullptr;
+  uint4* meta = nullptr;
+  float4* child_bounds = nullptr;
+  uint32_t* parent = nullptr;
+  uint32_t* flags = nullptr;
+  char* sort_scratch = nullptr;
+  rts::SortScratch ss{};
+  float4* partials = nullptr;
+  uint32_t n_partials_cap = 0;
+
+  // Device scalars: scene bounds, root box, tree cost (sum, ratio).
+  char* small = nullptr;
+  float4* scene = nullptr;
+  float4* root_box = nullptr;
+  double* cost = nullptr;
+  uint32_t* error = nullptr;  // device error bits (kErr*)
+
+  // Tree-cost bookkeeping for the SAH rebuild policy.
+  double* h_cost = nullptr;  // pinned: [0..1] after build, [2..3] after refit
+  hipEvent_t ev_cost_build = nullptr;
+  hipEvent_t ev_cost_refit = nullptr;
+  bool cost_build_pending = false;
+  bool cost_refit_pending = false;
+  uint64_t build_epoch = 0;
+  uint64_t refit_epoch = 0;
+  double sah_at_build = 0;
+  double sah_current = 0;
+
+  uint64_t builds = 0;
+  uint64_t refits = 0;
+};
+
+namespace rts {
+TreeView tree_view(const rts_index* ix);
+int ensure_built(rts_index* ix, hipStream_t stream);
+}  // namespace rts

@@ -1,0 +1,249 @@
+// LBVH traversal kernels: point and envelope queries with exact FP32
+// predicates, replacing the reference's OptiX raygen/intersection programs
+// (src/shaders/shaders_contains_point_query_2d.cu:14-49,
+//  shaders_contains_envelope_query_2d.cu:14-51,
+//  shaders_intersects_envelope_query_2d.cu:14-128).
+//
+// Node tests are necessary conditions of the leaf predicate on exact FP32
+// bounds, so pruning never drops a true pair; every reported pair passed the
+// exact leaf predicate (geom/envelope.cuh:29-69). A child whose bounds lie
+// inside the query (intersects / within) is not descended: its leaf range is
+// scanned directly, since every live box under it matches.
+//
+// Output: results of a wavefront are staged in LDS and flushed with one
+// device-scope atomic per ~1000 pairs and fully coalesced 8-byte stores, in
+// place of the reference's one atomic per result (utils/queue.h:22-27).
+#include "rts_internal.h"
+#include "wave.h"
+
+namespace rts {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kStageCap = 1024;  // pairs staged per wavefront before a flush
+
+template <int PRED, bool POINTS>
+struct Pred;
+
+// Point queries: Envelope::Contains(point) (envelope.cuh:29-38).
+template <int P>
+struct Pred<P, true> {
+  __device__ static bool node(const float4 n, const float4 q) { return contains_point(n, q.x, q.y); }
+  __device__ static bool inside(const float4, const float4) { return false; }
+  __device__ static bool leaf(const float4 b, const float4 q) { return contains_point(b, q.x, q.y); }
+};
+// Envelope intersects (envelope.cuh:55-69).
+template <>
+struct Pred<RTS_INTERSECTS, false> {
+  __device__ static bool node(const float4 n, const float4 q) { return intersects(n, q); }
+  __device__ static bool inside(const float4 n, const float4 q) { return contains_box(q, n); }
+  __device__ static bool leaf(const float4 b, const float4 q) { return intersects(b, q); }
+};
+// Data box contains the query (envelope.cuh:40-53, the reference's kContains).
+template <>
+struct Pred<RTS_CONTAINS, false> {
+  __device__ static bool node(const float4 n, const float4 q) { return contains_box(n, q); }
+  __device__ static bool inside(const float4, const float4) { return false; }
+  __device__ static bool leaf(const float4 b, const float4 q) { return contains_box(b, q); }
+};
+// Data box inside the query: the node must meet the query.
+template <>
+struct Pred<RTS_WITHIN, false> {
+  __device__ static bool node(const float4 n, const float4 q) { return intersects(n, q); }
+  __device__ static bool inside(const float4 n, const float4 q) { return contains_box(q, n); }
+  __device__ static bool leaf(const float4 b, const float4 q) { return contains_box(q, b); }
+};
+
+__device__ __forceinline__ void lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Flushes `fill` staged pairs of this wave to the queue.
+__device__ __forceinline__ void flush_stage(const rts_pair* stage, uint32_t fill,
+                                            unsigned long long* counter, rts_pair* out,
+                                            uint64_t cap) {
+  const uint32_t lane = lane_id();
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(counter, (unsigned long long)fill);
+  base = readlane_u64(base, 0);
+  lds_fence();
+  for (uint32_t j = lane; j < fill; j += kWave) {
+    const uint64_t slot = base + j;
+    if (slot < cap) out[slot] = stage[j];
+  }
+  lds_fence();
+}
+
+// One query per lane. The wave iterates until all of its lanes are done; in
+// each iteration a lane either expands one node or tests one leaf box.
+template <int PRED, bool POINTS, bool COLLECT>
+__global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) {
+  using P = Pred<PRED, POINTS>;
+  __shared__ rts_pair s_stage[COLLECT ? kWaves : 1][COLLECT ? kStageCap : 1];
+
+  unsigned long long* counter;
+  rts_pair* out = nullptr;
+  uint64_t cap = 0;
+  if (COLLECT) {
+    const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
+    out = dq->data;
+    cap = dq->capacity;
+    counter = dq->counter;
+  } else {
+    counter = static_cast<unsigned long long*>(const_cast<void*>(q.sink.arg));
+  }
+
+  const TreeView t = q.tree;
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  bool working = i < q.n && t.n > 0;
+  const uint32_t qi = working ? (q.order ? q.order[i] : i) : 0;
+  const uint32_t qid = qi + q.qid_base;
+
+  float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (working) {
+    if (POINTS) {
+      const float2 p = static_cast<const float2*>(q.queries)[qi];
+      qb = make_float4(p.x, p.y, p.x, p.y);
+    } else {
+      qb = static_cast<const float4*>(q.queries)[qi];
+    }
+  }
+
+  uint32_t stack[kStackDepth];
+  int sp = 0;
+  uint32_t cur = 1, end = 0;  // leaf range being scanned: [cur, end]
+  if (working) {
+    const float4 rb = t.root_box[0];
+    if (!P::node(rb, qb)) {
+      working = false;
+    } else if (t.n <= t.leaf_size || t.n == 1 || P::inside(rb, qb)) {
+      cur = 0;
+      end = t.n - 1;
+    } else {
+      stack[sp++] = 0;
+    }
+  }
+
+  uint64_t my_count = 0;
+  uint32_t fill = 0;  // wave-uniform
+  rts_pair* stage = s_stage[COLLECT ? wave : 0];
+  // A correct tree is walked in at most 2n-1 node expansions plus n leaf
+  // tests per query; the bound only stops a walk of a corrupt tree.
+  const uint64_t max_steps = 4ull * t.n + 64;
+  uint64_t steps = 0;
+
+  while (ballot(working)) {
+    bool hit = false;
+    uint32_t prim = 0;
+    if (working && ++steps > max_steps) {
+      atomicOr(t.error, kErrTraversal);
+      working = false;
+    }
+    if (working) {
+      bool have = false;
+      if (cur <= end) {
+        prim = cur++;
+        have = true;
+      } else if (sp > 0) {
+        const uint32_t e = stack[--sp];
+        if (e & kLeafFlag) {
+          prim = e & ~kLeafFlag;
+          have = true;
+        } else if (e & kRangeFlag) {
+          if ((e & kIndexMask) < t.n - 1) {
+            const uint4 m = t.meta[e & kIndexMask];
+            prim = m.z;
+            cur = m.z + 1;
+            end = m.w < t.n ? m.w : t.n - 1;
+            have = true;
+          }
+        } else if (e < t.n - 1) {
+          const uint4 m = t.meta[e];
+          const float4 lb = t.child_bounds[2 * e];
+          const float4 rbx = t.child_bounds[2 * e + 1];
+          const uint32_t gamma = m.x & ~kLeafFlag;
+          // right child first so that the left one is expanded next
+          if (P::node(rbx, qb)) {
+            if (m.y & kLeafFlag) {
+              if (sp < kStackDepth) stack[sp++] = m.y;
+            } else {
+              const uint32_t size = m.w - gamma;
+              if (sp < kStackDepth)
+                stack[sp++] = (size <= t.leaf_size || P::inside(rbx, qb)) ? (m.y | kRangeFlag) : m.y;
+            }
+          }
+          if (P::node(lb, qb)) {
+            if (m.x & kLeafFlag) {
+              if (sp < kStackDepth) stack[sp++] = m.x;
+            } else {
+              const uint32_t size = gamma - m.z + 1;
+              if (sp < kStackDepth)
+                stack[sp++] = (size <= t.leaf_size || P::inside(lb, qb)) ? (m.x | kRangeFlag) : m.x;
+            }
+          }
+        }
+      } else {
+        working = false;
+      }
+      if (have && prim < t.n) hit = P::leaf(t.leaf_boxes[prim], qb);
+    }
+
+    if (COLLECT) {
+      const uint64_t mask = ballot(hit);
+      if (mask) {
+        if (hit) {
+          rts_pair pr;
+          pr.geom_id = t.leaf_ids[prim];
+          pr.query_id = qid;
+          stage[fill + mask_rank(mask)] = pr;
+        }
+        fill += (uint32_t)__popcll(mask);
+        if (fill > kStageCap - kWave) {
+          lds_fence();
+          flush_stage(stage, fill, counter, out, cap);
+          fill = 0;
+        }
+      }
+    } else {
+      my_count += hit ? 1u : 0u;
+    }
+  }
+
+  if (COLLECT) {
+    if (fill) {
+      lds_fence();
+      flush_stage(stage, fill, counter, out, cap);
+    }
+  } else {
+    const uint64_t total = wave_sum_u64(my_count);
+    if (lane_id() == 0 && total) atomicAdd(counter, (unsigned long long)total);
+  }
+}
+
+template <int PRED, bool POINTS>
+hipError_t launch_pred(const QueryLaunch& q, hipStream_t stream) {
+  const dim3 grid((q.n + kThreads - 1) / kThreads), block(kThreads);
+  if (q.sink.mode == RTS_HANDLER_COLLECT)
+    hipLaunchKernelGGL((k_query_thread<PRED, POINTS, true>), grid, block, 0, stream, q);
+  else
+    hipLaunchKernelGGL((k_query_thread<PRED, POINTS, false>), grid, block, 0, stream, q);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_query(const QueryLaunch& q, hipStream_t stream) {
+  if (q.n == 0) return hipSuccess;
+  if (q.point_queries) return launch_pred<RTS_CONTAINS, true>(q, stream);
+  switch (q.pred) {
+    case RTS_INTERSECTS: return launch_pred<RTS_INTERSECTS, false>(q, stream);
+    case RTS_CONTAINS: return launch_pred<RTS_CONTAINS, false>(q, stream);
+    case RTS_WITHIN: return launch_pred<RTS_WITHIN, false>(q, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace rts

@@ -1,0 +1,140 @@
+// Internal declarations shared by the HIP translation units of
+// librtspatial_amd.so. Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "rts_c.h"
+
+namespace rts {
+
+// ------------------------------------------------------------ constants ----
+constexpr int kWave = 64;                 // CDNA wavefront width
+constexpr uint32_t kLeafFlag = 0x80000000u;   // child ref: leaf (sorted position)
+constexpr uint32_t kRangeFlag = 0x40000000u;  // stack entry: scan the range of an internal node
+constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
+constexpr uint64_t kMaxBoxes = (1ull << 30) - 1;  // ids and sorted positions fit in 30 bits
+constexpr int kMortonBits = 30;           // 15 bits per axis
+constexpr int kStackDepth = 72;           // >= max Karras depth (64) + 2
+
+// --------------------------------------------------------------- layout ----
+// Karras LBVH over n sorted leaves: internal nodes 0..n-2 (root 0).
+//   meta[i]          = {left ref, right ref, first leaf, last leaf}
+//                      ref = internal index, or sorted leaf position | kLeafFlag
+//   child_bounds[2i] = bounds of the left child, [2i+1] of the right child
+//   parent[j]        = (parent << 1) | side, for internal j in [0, n-1) and
+//                      leaf k at [n-1+k]
+// Leaves are the boxes in Morton order: leaf_boxes[k], with id leaf_ids[k].
+struct TreeView {
+  const float4* leaf_boxes;
+  const uint32_t* leaf_ids;
+  const uint4* meta;
+  const float4* child_bounds;
+  const float4* root_box;   // device scalar
+  uint32_t n;               // leaves
+  uint32_t leaf_size;       // ranges with <= leaf_size leaves are scanned directly
+  uint32_t* error;          // device word: bit 1 refit walk, bit 2 traversal bound
+};
+
+// Device error bits (the index's error word).
+constexpr uint32_t kErrSortWait = 1u;
+constexpr uint32_t kErrRefitWalk = 2u;
+constexpr uint32_t kErrTraversal = 4u;
+
+// Result sink resolved from (handler, arg).
+struct SinkArgs {
+  int mode;          // rts_handler
+  const void* arg;   // counter or rts_dev_queue device pointer
+};
+
+// ----------------------------------------------------------- predicates ----
+// Exact closed-interval tests in FP32 (reference geom/envelope.cuh:29-69).
+// Boxes are float4 (x0, y0, x1, y1). Every comparison with a NaN is false, so
+// a deleted box (NaN coordinates) never matches.
+__device__ __forceinline__ bool contains_point(const float4 b, float px, float py) {
+  return px >= b.x && px <= b.z && py >= b.y && py <= b.w;
+}
+__device__ __forceinline__ bool contains_box(const float4 a, const float4 o) {
+  return o.x >= a.x && o.z <= a.z && o.y >= a.y && o.w <= a.w;
+}
+__device__ __forceinline__ bool intersects(const float4 a, const float4 o) {
+  return o.x <= a.z && o.z >= a.x && o.y <= a.w && o.w >= a.y;
+}
+
+// ----------------------------------------------------- host-side kernels ----
+// Thin launch wrappers (defined in the .hip files). All are asynchronous on
+// `stream` and return the hipError_t of the launch.
+
+// sort.hip
+struct SortScratch {
+  uint32_t* hist;      // 4 x 256
+  uint32_t* offsets;   // 4 x 256
+  uint32_t* status;    // 4 x tiles x 256
+  uint32_t* tickets;   // 4
+  uint32_t* error;     // 1
+  uint32_t tiles_cap;  // tiles the status array holds per pass
+};
+size_t sort_scratch_bytes(uint64_t n);
+void sort_scratch_carve(void* base, uint64_t n, SortScratch* s);
+// Sorts (keys, vals) on bits [0, key_bits). keys_a/vals_a hold the input
+// (with identity_vals the values 0..n-1 are generated and vals_a is only a
+// buffer); keys_b/vals_b are ping-pong buffers. `*in_b` tells whether the
+// result ended in the b buffers.
+hipError_t radix_sort_pairs(uint32_t* keys_a, uint32_t* vals_a, uint32_t* keys_b,
+                            uint32_t* vals_b, uint32_t n, int key_bits, bool identity_vals,
+                            const SortScratch& s, hipStream_t stream, bool* in_b);
+
+// build.hip
+hipError_t launch_scene_bounds(const float4* boxes, uint32_t n, float4* partials,
+                               uint32_t n_partials, float4* scene, hipStream_t stream);
+uint32_t scene_bounds_partials(uint32_t n);
+hipError_t launch_morton_boxes(const float4* boxes, uint32_t n, const float4* scene,
+                               uint32_t* keys, hipStream_t stream);
+hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* scene,
+                                uint32_t* keys, hipStream_t stream);
+hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids, uint32_t n,
+                                float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream);
+hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, uint4* meta,
+                         uint32_t* parent, uint32_t* flags, hipStream_t stream);
+hipError_t launch_refit(const float4* leaf_boxes, uint32_t n, const uint32_t* parent,
+                        float4* child_bounds, float4* root_box, uint32_t* flags, uint32_t* error,
+                        hipStream_t stream);
+hipError_t launch_tree_cost(const float4* child_bounds, const float4* root_box, uint32_t n,
+                            double* out, hipStream_t stream);
+hipError_t launch_delete(const uint64_t* ids, uint64_t n_ids, float4* boxes, uint64_t n_boxes,
+                         float4* leaf_boxes, const uint32_t* leaf_of_id, uint64_t n_tree,
+                         hipStream_t stream);
+hipError_t launch_update(const rts_update* ups, uint64_t n_ups, float4* boxes, uint64_t n_boxes,
+                         float4* leaf_boxes, const uint32_t* leaf_of_id, uint64_t n_tree,
+                         hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t stream);
+
+// query.hip
+struct QueryLaunch {
+  TreeView tree;
+  const void* queries;   // float2 points or float4 envelopes
+  uint32_t n;
+  uint32_t qid_base;     // added to the query index when reporting
+  const uint32_t* order; // optional permutation: lane i works on queries[order[i]]
+  int pred;              // rts_predicate
+  int point_queries;     // 1 = float2 points
+  SinkArgs sink;
+  int traversal;         // rts_traversal (resolved: THREAD or WAVE)
+};
+hipError_t launch_query(const QueryLaunch& q, hipStream_t stream);
+
+// index.hip
+void set_error(const std::string& msg);
+
+// profile.hip
+struct PhaseToken {
+  int phase;
+  hipEvent_t begin;
+};
+PhaseToken phase_begin(int phase, hipStream_t stream);
+void phase_end(const PhaseToken& t, hipStream_t stream);
+
+}  // namespace rts

@@ -1,0 +1,293 @@
+"""Python mirror of rtspatial::SpatialIndex<float, 2> over the C ABI.
+
+Reference: include/rtspatial/spatial_index.cuh:20-931 (Config, SpatialIndex),
+utils/queue.h:60-113 (Queue), utils/shared_value.h:11-58 (SharedValue),
+geom/predicates.h:7-10 (Predicate). Method names and argument meanings follow
+the reference; tensors are torch tensors on the GPU (or pinned host tensors,
+as the reference's ArrayView of a pinned_vector).
+
+  points    float32 [n, 2]        Point<float,2>
+  envelopes float32 [n, 4]        Envelope: (min x, min y, max x, max y)
+  ids       int64   [n]           size_t ids (Delete)
+  updates   (ids int64 [n], envelopes float32 [n, 4])   thrust::pair<size_t, envelope_t>
+  results   int32   [k, 2]        (geom_id, query_id), collecting_handlers.h:9-31
+"""
+import ctypes
+import dataclasses
+import enum
+
+import torch
+
+from . import _native as N
+
+
+class Predicate(enum.IntEnum):
+    """geom/predicates.h:7-10, plus kWithin (data box inside the query)."""
+    kIntersects = N.RTS_INTERSECTS
+    kContains = N.RTS_CONTAINS
+    kWithin = N.RTS_WITHIN
+
+
+@dataclasses.dataclass
+class Config:
+    """Config (spatial_index.cuh:22-37). ptx_root, prefer_fast_build_* and
+    compact configure OptiX in the reference and are accepted and ignored."""
+    ptx_root: str = ""
+    max_geometries: int = 10_000_000
+    max_queries: int = 1_000_000
+    preallocate: bool = False
+    prefer_fast_build_geom: bool = False
+    prefer_fast_build_query: bool = False
+    compact: bool = False
+    geom_sample_rate: float = 0.2
+    query_sample_rate: float = 0.2
+    intersect_cost_weight: float = 0.95
+    max_geom_samples: int = 10000
+    max_query_samples: int = 1000
+    max_parallelism: int = 512
+    # MI355X additions
+    leaf_size: int = 8
+    rebuild_policy: int = N.RTS_REBUILD_SAH
+    sah_rebuild_ratio: float = 1.5
+    traversal: int = N.RTS_TRAVERSAL_AUTO
+    sort_queries: int = 0
+
+    def to_c(self):
+        c = N.rts_config()
+        N.lib().rts_config_default(ctypes.byref(c))
+        for f, _ in N.rts_config._fields_:
+            v = getattr(self, f)
+            setattr(c, f, int(v) if isinstance(v, bool) else v)
+        return c
+
+
+def _stream_handle(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    if isinstance(stream, torch.cuda.Stream):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+def _check_array(t, cols, dtype, what):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what}: expected a torch tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{what}: expected {dtype}, got {t.dtype}")
+    if cols is not None and (t.dim() != 2 or t.shape[1] != cols):
+        raise ValueError(f"{what}: expected shape [n, {cols}], got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: tensor must be contiguous")
+    if not (t.is_cuda or t.is_pinned()):
+        raise ValueError(f"{what}: tensor must be on the GPU or in pinned host memory")
+    return t
+
+
+class SharedValue:
+    """A device-resident unsigned 64-bit counter (utils/shared_value.h:11-58),
+    the `arg` of the counting handler (counting_handlers.h:7-25)."""
+
+    def __init__(self, device=None):
+        self._t = torch.zeros(1, dtype=torch.int64, device=device or "cuda")
+
+    def data(self):
+        return self._t.data_ptr()
+
+    def set(self, value, stream=None):
+        with torch.cuda.stream(stream) if isinstance(stream, torch.cuda.Stream) else _null():
+            self._t.fill_(int(value))
+
+    def get(self, stream=None):
+        if isinstance(stream, torch.cuda.Stream):
+            stream.synchronize()
+        else:
+            torch.cuda.synchronize()
+        return int(self._t.item()) & 0xFFFFFFFFFFFFFFFF
+
+    @property
+    def tensor(self):
+        return self._t
+
+
+class Queue:
+    """Result queue of (geom_id, query_id) pairs (utils/queue.h:60-113) with
+    a 64-bit counter. DeviceObject() is the device pointer the collecting
+    handler takes as `arg`."""
+
+    def __init__(self, capacity=0, device=None):
+        self._device = torch.device(device or "cuda")
+        self._counter = torch.zeros(1, dtype=torch.int64, device=self._device)
+        self._desc = None
+        self._data = None
+        self.Init(capacity)
+
+    def Init(self, capacity):
+        capacity = max(int(capacity), 1)
+        self._data = torch.empty((capacity, 2), dtype=torch.int32, device=self._device)
+        host = N.rts_dev_queue(self._data.data_ptr(), capacity, self._counter.data_ptr())
+        raw = torch.frombuffer(bytearray(bytes(host)), dtype=torch.uint8)
+        self._desc = raw.to(self._device)
+        self._counter.zero_()
+
+    def Clear(self, stream=None):
+        with torch.cuda.stream(stream) if isinstance(stream, torch.cuda.Stream) else _null():
+            self._counter.zero_()
+
+    def size(self, stream=None):
+        """Number of results reported (may exceed capacity on overflow)."""
+        if isinstance(stream, torch.cuda.Stream):
+            stream.synchronize()
+        else:
+            torch.cuda.synchronize()
+        return int(self._counter.item())
+
+    def capacity(self):
+        return self._data.shape[0]
+
+    def overflowed(self, stream=None):
+        return self.size(stream) > self.capacity()
+
+    def DeviceObject(self):
+        return self._desc.data_ptr()
+
+    def pairs(self, stream=None):
+        """The stored (geom_id, query_id) pairs as an int32 [k, 2] tensor."""
+        k = min(self.size(stream), self.capacity())
+        return self._data[:k]
+
+    @property
+    def counter(self):
+        return self._counter
+
+
+class _null:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+
+class SpatialIndex:
+    """rtspatial::SpatialIndex<float, 2> (spatial_index.cuh:62-931)."""
+
+    def __init__(self):
+        self._h = None
+
+    # -- lifecycle ---------------------------------------------------------
+    def Init(self, config=None):
+        """spatial_index.cuh:83-110."""
+        self._destroy()
+        cfg = (config or Config()).to_c()
+        h = ctypes.c_void_p()
+        N.check(N.lib().rts_index_create(ctypes.byref(cfg), ctypes.byref(h)), "Init")
+        self._h = h
+        self.config = config or Config()
+
+    def Clear(self):
+        """spatial_index.cuh:118-129."""
+        N.check(N.lib().rts_index_clear(self._handle()), "Clear")
+
+    def _destroy(self):
+        if self._h is not None:
+            N.lib().rts_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
+
+    def _handle(self):
+        if self._h is None:
+            raise RuntimeError("SpatialIndex.Init() has not been called")
+        return self._h
+
+    # -- mutation ------------------------------------------------------------
+    def Insert(self, envelopes, stream=None):
+        """spatial_index.cuh:142-196: appends boxes with ids size()..size()+n-1."""
+        _check_array(envelopes, 4, torch.float32, "Insert")
+        N.check(N.lib().rts_index_insert(self._handle(), envelopes.data_ptr(), envelopes.shape[0],
+                                         _stream_handle(stream)), "Insert")
+
+    def Delete(self, ids, stream=None):
+        """spatial_index.cuh:208-265. ids >= size() are ignored."""
+        _check_array(ids, None, torch.int64, "Delete")
+        N.check(N.lib().rts_index_delete(self._handle(), ids.data_ptr(), ids.numel(),
+                                         _stream_handle(stream)), "Delete")
+
+    def Update(self, updates, stream=None):
+        """spatial_index.cuh:280-336. `updates` is a packed int64 [n, 3] tensor
+        from pack_updates(), or an (ids, envelopes) tuple."""
+        if isinstance(updates, tuple):
+            updates = pack_updates(*updates)
+        _check_array(updates, 3, torch.int64, "Update")
+        N.check(N.lib().rts_index_update(self._handle(), updates.data_ptr(), updates.shape[0],
+                                         _stream_handle(stream)), "Update")
+
+    # -- queries -------------------------------------------------------------
+    def Query(self, predicate, queries, arg, stream=None):
+        """spatial_index.cuh:352-391 (points, [n, 2]) and :407-515 (envelopes,
+        [n, 4]). `arg` is a SharedValue (counting handler) or a Queue
+        (collecting handler)."""
+        _check_array(queries, None, torch.float32, "Query")
+        if queries.dim() != 2 or queries.shape[1] not in (2, 4):
+            raise ValueError("Query: queries must be [n, 2] points or [n, 4] envelopes")
+        if isinstance(arg, SharedValue):
+            handler, ptr = N.RTS_HANDLER_COUNT, arg.data()
+        elif isinstance(arg, Queue):
+            handler, ptr = N.RTS_HANDLER_COLLECT, arg.DeviceObject()
+        else:
+            raise TypeError("Query: arg must be a SharedValue or a Queue")
+        fn = (N.lib().rts_index_query_points if queries.shape[1] == 2
+              else N.lib().rts_index_query_envelopes)
+        N.check(fn(self._handle(), int(predicate), queries.data_ptr(), queries.shape[0], handler,
+                   ptr, _stream_handle(stream)), "Query")
+
+    # -- extras ----------------------------------------------------------------
+    def size(self):
+        return int(N.lib().rts_index_size(self._handle()))
+
+    def Build(self, stream=None):
+        N.check(N.lib().rts_index_build(self._handle(), _stream_handle(stream)), "Build")
+
+    def Sync(self, stream=None):
+        N.check(N.lib().rts_index_sync(self._handle(), _stream_handle(stream)), "Sync")
+
+    def stats(self):
+        s = N.rts_index_stats()
+        N.check(N.lib().rts_index_get_stats(self._handle(), ctypes.byref(s)), "stats")
+        return {f: getattr(s, f) for f, _ in N.rts_index_stats._fields_}
+
+
+def pack_updates(ids, envelopes):
+    """(ids int64 [n], envelopes float32 [n, 4]) -> int64 [n, 3], the 24-byte
+    rts_update records (thrust::pair<size_t, envelope_t>)."""
+    ids = ids.reshape(-1, 1).to(torch.int64)
+    env = envelopes.to(torch.float32).contiguous().view(torch.int64).reshape(-1, 2)
+    return torch.cat([ids, env], dim=1).contiguous()
+
+
+def sort_pairs(keys, vals=None, key_bits=32, stream=None):
+    """Stable onesweep radix sort of uint32 keys (int32 tensors) with uint32
+    values (identity when vals is None). Returns (sorted_keys, sorted_vals)."""
+    n = keys.numel()
+    ko = torch.empty_like(keys)
+    vo = torch.empty(n, dtype=torch.int32, device=keys.device)
+    N.check(N.lib().rts_sort_pairs(keys.data_ptr(), vals.data_ptr() if vals is not None else None,
+                                   ko.data_ptr(), vo.data_ptr(), n, key_bits,
+                                   _stream_handle(stream)), "sort_pairs")
+    return ko, vo
+
+
+def profile_enable(on=True):
+    """Per-phase GPU timing (HIP events on the caller's stream)."""
+    N.lib().rts_profile_enable(1 if on else 0)
+
+
+def profile_read(reset=True):
+    """{phase: (launch count, total ms)} since the last reset."""
+    p = N.rts_profile()
+    N.check(N.lib().rts_profile_read(ctypes.byref(p), 1 if reset else 0), "profile_read")
+    return {name: (int(p.count[i]), float(p.ms[i])) for i, name in enumerate(N.RTS_PHASES)}

@@ -1,0 +1,85 @@
+"""CPU tests of the C-ABI library: it loads, exports every entry point that
+include/rts_c.h declares, and its host-only calls behave (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(rts_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("rts_index_create", "rts_index_insert", "rts_index_delete", "rts_index_update",
+              "rts_index_query_points", "rts_index_query_envelopes", "rts_sort_pairs"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from rtspatial_amd import _native as N
+    L = N.lib()
+    for n in declared_functions():
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
+    bound = {s[0] for s in N.SIGNATURES}
+    assert set(declared_functions()) <= bound, set(declared_functions()) - bound
+
+
+def test_config_defaults_match_reference():
+    # spatial_index.cuh:22-37
+    from rtspatial_amd import _native as N
+    c = N.rts_config()
+    N.lib().rts_config_default(ctypes.byref(c))
+    assert c.max_geometries == 10_000_000 and c.max_queries == 1_000_000
+    assert abs(c.geom_sample_rate - 0.2) < 1e-7 and abs(c.intersect_cost_weight - 0.95) < 1e-7
+    assert c.max_geom_samples == 10000 and c.max_query_samples == 1000 and c.max_parallelism == 512
+    assert c.leaf_size > 0
+
+
+def test_struct_layouts():
+    from rtspatial_amd import _native as N
+    assert ctypes.sizeof(N.rts_dev_queue) == 24
+    assert ctypes.sizeof(N.rts_config) % 8 == 0
+
+
+def test_version_and_error_string():
+    from rtspatial_amd import _native as N
+    assert b"gfx950" in N.lib().rts_version()
+    assert isinstance(N.lib().rts_last_error(), bytes)
+
+
+def test_null_arguments_rejected_without_gpu():
+    from rtspatial_amd import _native as N
+    L = N.lib()
+    assert L.rts_index_create(None, None) == 1
+    assert L.rts_index_insert(None, None, 1, None) == 1
+    assert L.rts_sort_pairs(None, None, None, None, 4, 8, None) == 1
+    assert L.rts_index_query_points(None, 1, None, 1, 0, None, None) == 1
+
+
+def test_create_without_gpu_reports_missing():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from rtspatial_amd import _native as N
+    h = ctypes.c_void_p()
+    assert N.lib().rts_index_create(None, ctypes.byref(h)) == 4
+    assert h.value is None
+
+
+def test_python_api_surface():
+    import rtspatial_amd as R
+    for m in ("Init", "Clear", "Insert", "Delete", "Update", "Query"):
+        assert hasattr(R.SpatialIndex, m)
+    assert int(R.Predicate.kIntersects) == 0 and int(R.Predicate.kContains) == 1

@@ -1,0 +1,285 @@
+"""GPU parity tests: the HIP LBVH path, called through the C ABI, against the
+oracle (exact brute force / R-tree) on the same seeded inputs, and against the
+reference tests' known answers (test/test_point_queries.h,
+test/test_envelope_queries.h). Integer/index results must match bit-exactly."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import rtspatial_amd as R  # noqa: E402
+from gpu_util import canon, collect, count, dev, log, make_index, pinned  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+P = R.Predicate
+
+
+@pytest.fixture(scope="module")
+def ka():
+    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def ref_inputs(oracle):
+    return {
+        "boxes": oracle.ref_uniform_boxes(100000, 0.5, 0.5),
+        "points": oracle.ref_uniform_points(1000),
+        "queries": oracle.ref_uniform_boxes(1000, 0.1, 0.1),
+    }
+
+
+# ------------------------------------------------------- reference known answers
+def test_tiny_point_case_pinned_host(ka):
+    # test/test_point_queries.h:62-90: pinned host inputs, count 5.
+    log("tiny point case")
+    c = ka["tiny_points"]
+    boxes = pinned(np.array(c["boxes"], np.float32))
+    pts = pinned(np.array(c["points"], np.float32))
+    ix = R.SpatialIndex()
+    ix.Init(R.Config())
+    ix.Insert(boxes)
+    cnt = R.SharedValue()
+    cnt.set(0)
+    ix.Query(P.kContains, pts, cnt)
+    assert cnt.get() == 5
+    pairs, n = collect(ix, P.kContains, dev(np.array(c["points"], np.float32)))
+    assert n == 5 and pairs.tolist() == c["pairs_query_box"]
+
+
+def test_tiny_intersects_then_delete(ka):
+    # test/test_envelope_queries.h:11-36 and :166-245: 2, then 1 after Delete({1}).
+    log("tiny intersects + delete")
+    c = ka["tiny_intersects_delete"]
+    for compact in (False, True):
+        ix = R.SpatialIndex()
+        ix.Init(R.Config(compact=compact))
+        ix.Insert(pinned(np.array(c["boxes"], np.float32)))
+        q = pinned(np.array(c["queries"], np.float32))
+        assert count(ix, P.kIntersects, q) == 2
+        ix.Delete(dev(np.array([1], np.int64)))
+        torch.cuda.synchronize()
+        assert count(ix, P.kIntersects, q) == 1
+        pairs, _ = collect(ix, P.kIntersects, q)
+        assert pairs.tolist() == [[0, 0]]
+
+
+@pytest.mark.parametrize("batches", [1, 10])
+def test_point_contains_known_answer(ref_inputs, batches):
+    # test/test_point_queries.h:9-60: 4224111, single and 10-batch insert.
+    log(f"point known answer, batches={batches}")
+    ix = make_index(ref_inputs["boxes"], batches=batches)
+    assert count(ix, P.kContains, ref_inputs["points"]) == 4224111
+
+
+@pytest.mark.parametrize("batches", [1, 10])
+def test_intersects_known_answer_exact(ref_inputs, ka, batches):
+    # test/test_envelope_queries.h:38-91 asserts 6549771, which includes one
+    # 1-ulp false positive of the reference's ray formulation; the exact set
+    # has 6549770 pairs and excludes (query 362, box 76881).
+    log(f"intersects known answer, batches={batches}")
+    ix = make_index(ref_inputs["boxes"], batches=batches)
+    assert count(ix, P.kIntersects, ref_inputs["queries"]) == 6549770
+    pairs, n = collect(ix, P.kIntersects, ref_inputs["queries"])
+    assert n == 6549770
+    extra = ka["intersects_100k_1k"]["extra_pairs_query_box"][0]
+    sel = pairs[pairs[:, 0] == extra[0]]
+    assert extra[1] not in sel[:, 1]
+
+
+def test_update_refit_equals_rebuild(ref_inputs, ka, oracle):
+    # test/test_envelope_queries.h:93-164: update 5% of ids after a 10-batch
+    # insert; the refitted index must answer like a freshly built one.
+    log("update 5% refit == rebuild")
+    ids = oracle.ref_update_ids(100000, 5000).astype(np.int64)
+    new = oracle.ref_uniform_boxes(5000, 0.5, 0.5)
+    b2 = ref_inputs["boxes"].copy()
+    b2[ids] = new
+    fresh = make_index(b2)
+    want = count(fresh, P.kIntersects, ref_inputs["queries"])
+    assert want == ka["intersects_after_update_5pct"]["exact"]
+    for policy in (R._native.RTS_REBUILD_NEVER, R._native.RTS_REBUILD_SAH, R._native.RTS_REBUILD_ALWAYS):
+        ix = make_index(ref_inputs["boxes"], batches=10, rebuild_policy=policy)
+        ix.Build()
+        ix.Update((dev(ids), dev(new)))
+        ix.Sync()
+        got, _ = collect(ix, P.kIntersects, ref_inputs["queries"])
+        exp, _ = collect(fresh, P.kIntersects, ref_inputs["queries"])
+        assert np.array_equal(got, exp), f"policy {policy}"
+
+
+# -------------------------------------------------------------- pair-set parity
+@pytest.mark.parametrize("leaf_size", [1, 4, 8, 32])
+def test_c1_point_pairs_exact(oracle, leaf_size):
+    # Config 1 at full size: 100k boxes x 100k points, sorted pair set vs brute force.
+    log(f"c1 point pairs leaf_size={leaf_size}")
+    from rtspatial_amd import workloads as W
+    b, q = W.config1()
+    want = oracle.bf_pairs(oracle.CONTAINS, b, q)
+    ix = make_index(b, leaf_size=leaf_size)
+    got, n = collect(ix, P.kContains, q)
+    assert n == want.shape[0]
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("pred", [P.kIntersects, P.kContains, P.kWithin])
+def test_c2_style_envelope_pairs_exact(oracle, pred):
+    # Config 2 shapes at 200k x 100k: every predicate, sorted pair set vs R-tree.
+    log(f"c2-style envelope pairs pred={pred.name}")
+    from rtspatial_amd import workloads as W
+    b, q = W.config2(200_000, 100_000)
+    if pred == P.kContains:
+        # data boxes that contain a query: make the queries small
+        q = W.uniform_boxes(np.random.default_rng(5), 100_000, 1e-4)
+    want = oracle.RTree(b).pairs(int(pred), q)
+    ix = make_index(b)
+    got, n = collect(ix, pred, q)
+    assert n == want.shape[0] and n > 0
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("name", ["uniform", "gaussian"])
+def test_wkt_fixture_pairs(name):
+    # The reference's sample datasets (fixtures in tests/golden/wkt_1000.npz).
+    log(f"wkt fixtures {name}")
+    d = np.load(os.path.join(GOLDEN, "wkt_1000.npz"))
+    boxes = d[f"{name}_boxes"]
+    ix = make_index(boxes, leaf_size=2)
+    for qk in ("uniform", "gaussian"):
+        got, _ = collect(ix, P.kContains, d[f"{qk}_points"])
+        assert np.array_equal(got, d[f"pairs_{name}_boxes_{qk}_points_contains"])
+        for pn, pred in (("intersects", P.kIntersects), ("contains", P.kContains), ("within", P.kWithin)):
+            got, _ = collect(ix, pred, d[f"{qk}_boxes"])
+            assert np.array_equal(got, d[f"pairs_{name}_boxes_{qk}_boxes_{pn}"]), (qk, pn)
+
+
+# ------------------------------------------------------------------ edge cases
+def test_empty_index_and_empty_queries():
+    log("empty cases")
+    ix = R.SpatialIndex()
+    ix.Init(R.Config())
+    q = dev(np.float32([[0, 0, 1, 1]]))
+    assert count(ix, P.kIntersects, q) == 0
+    ix.Insert(dev(np.float32([[0, 0, 1, 1]])))
+    assert count(ix, P.kIntersects, dev(np.zeros((0, 4), np.float32))) == 0
+    assert count(ix, P.kIntersects, q) == 1
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 4095, 4096, 4097])
+def test_small_sizes_and_tile_edges(oracle, n):
+    log(f"small size n={n}")
+    rng = np.random.default_rng(n)
+    from rtspatial_amd import workloads as W
+    b = W.uniform_boxes(rng, n, 0.2)
+    q = W.uniform_boxes(rng, 257, 0.3)
+    ix = make_index(b, leaf_size=1)
+    for pred in (P.kIntersects, P.kContains, P.kWithin):
+        got, _ = collect(ix, pred, q)
+        assert np.array_equal(got, oracle.bf_pairs(int(pred), b, q))
+
+
+def test_duplicates_degenerate_and_boundaries(oracle):
+    # Identical boxes (equal Morton codes), zero-width boxes, boxes touching
+    # the query only on an edge or a corner, points on edges.
+    log("duplicates / degenerate / boundaries")
+    grid = (np.arange(0, 9, dtype=np.float32) / 8)
+    xs, ys = np.meshgrid(grid[:-1], grid[:-1])
+    cells = np.stack([xs.ravel(), ys.ravel(), xs.ravel() + np.float32(0.125), ys.ravel() + np.float32(0.125)], 1)
+    dup = np.repeat(np.float32([[0.25, 0.25, 0.5, 0.5]]), 3000, axis=0)
+    degen = np.float32([[0.5, 0.5, 0.5, 0.5], [0.25, 0, 0.25, 1], [0, 0.75, 1, 0.75]])
+    b = np.concatenate([cells, dup, degen]).astype(np.float32)
+    q = np.concatenate([cells, np.float32([[0.5, 0.5, 0.5, 0.5], [0.25, 0.25, 0.5, 0.5], [0, 0, 1, 1]])])
+    pts = np.stack(np.meshgrid(grid, grid), -1).reshape(-1, 2).astype(np.float32)
+    for leaf in (1, 8):
+        ix = make_index(b, leaf_size=leaf)
+        for pred in (P.kIntersects, P.kContains, P.kWithin):
+            got, _ = collect(ix, pred, q)
+            assert np.array_equal(got, oracle.bf_pairs(int(pred), b, q)), (leaf, pred)
+        got, _ = collect(ix, P.kContains, pts)
+        assert np.array_equal(got, oracle.bf_pairs(oracle.CONTAINS, b, pts))
+
+
+def test_delete_update_sequence_matches_oracle(oracle):
+    # Deletes (including out-of-range ids, which are ignored), updates,
+    # re-updates of deleted ids, inserts after deletes.
+    log("delete/update sequence")
+    from rtspatial_amd import workloads as W
+    rng = np.random.default_rng(3)
+    b = W.uniform_boxes(rng, 50_000, 0.01)
+    q = W.uniform_boxes(rng, 2000, 0.05)
+    alive = np.ones(b.shape[0], np.uint8)
+    ix = make_index(b)
+    for step in range(4):
+        dels = rng.choice(b.shape[0], 3000, replace=False).astype(np.int64)
+        ix.Delete(dev(np.concatenate([dels, [b.shape[0] + 5, 1 << 40]]).astype(np.int64)))
+        alive[dels] = 0
+        ups = rng.choice(b.shape[0], 2000, replace=False).astype(np.int64)
+        nb = W.uniform_boxes(rng, 2000, 0.02)
+        ix.Update((dev(ups), dev(nb)))
+        b[ups] = nb
+        alive[ups] = 1
+        if step == 2:
+            extra = W.uniform_boxes(rng, 5000, 0.01)
+            ix.Insert(dev(extra))
+            b = np.concatenate([b, extra])
+            alive = np.concatenate([alive, np.ones(5000, np.uint8)])
+        for pred in (P.kIntersects, P.kWithin):
+            got, _ = collect(ix, pred, q)
+            assert np.array_equal(got, oracle.bf_pairs(int(pred), b, q, alive=alive)), (step, pred)
+    ix.Sync()
+
+
+def test_queue_overflow_reports_exact_count(oracle):
+    log("queue overflow")
+    from rtspatial_amd import workloads as W
+    b, q = W.config2(100_000, 20_000)
+    want = oracle.bf_count(oracle.WITHIN, b, q)
+    ix = make_index(b)
+    queue = R.Queue(1000)
+    ix.Query(P.kWithin, dev(q), queue)
+    assert queue.size() == want and queue.overflowed()
+    stored = canon(queue.pairs())
+    full = oracle.bf_pairs(oracle.WITHIN, b, q)
+    s = set(map(tuple, full.tolist()))
+    assert stored.shape[0] == 1000 and all(tuple(p) in s for p in stored.tolist())
+
+
+def test_query_sorting_modes_agree(oracle):
+    log("query ordering modes")
+    from rtspatial_amd import workloads as W
+    b, q = W.config2(200_000, 100_000)
+    want = oracle.RTree(b).pairs(oracle.INTERSECTS, q)
+    for mode in (1, 2):
+        ix = make_index(b, sort_queries=mode)
+        got, _ = collect(ix, P.kIntersects, q)
+        assert np.array_equal(got, want), mode
+
+
+def test_point_predicate_other_than_contains_rejected():
+    ix = make_index(np.float32([[0, 0, 1, 1]]))
+    with pytest.raises(R.RtsError):
+        ix.Query(P.kIntersects, dev(np.float32([[0.5, 0.5]])), R.SharedValue())
+
+
+# ---------------------------------------------------------------------- sort
+@pytest.mark.parametrize("n,bits", [(1, 30), (1000, 8), (4097, 16), (1_000_003, 30), (3_000_000, 32)])
+def test_radix_sort_matches_stable_argsort(n, bits):
+    log(f"sort n={n} bits={bits}")
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(np.uint32)
+    if n > 10:
+        keys[: n // 3] = keys[0]  # many equal keys: stability matters
+    vals = rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    k, v = R.sort_pairs(dev(keys.view(np.int32)), dev(vals.view(np.int32)), key_bits=bits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k.cpu().numpy().view(np.uint32), keys[order])
+    assert np.array_equal(v.cpu().numpy().view(np.uint32), vals[order])
+    k2, v2 = R.sort_pairs(dev(keys.view(np.int32)), None, key_bits=bits)
+    assert np.array_equal(v2.cpu().numpy().view(np.uint32), order.astype(np.uint32))

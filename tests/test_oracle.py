@@ -1,0 +1,225 @@
+"""CPU tests of the oracle: it must reproduce the reference tests' known
+answers and agree with independent checks before it may judge the GPU path."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def ka():
+    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def ref100k(oracle):
+    return oracle.ref_uniform_boxes(100000, 0.5, 0.5)
+
+
+# ---------------------------------------------------------------- generators --
+def test_generator_streams_are_libstdcxx_mt19937(oracle):
+    # Shape and range of the std::mt19937(n) streams of test/test_commons.h:22-67;
+    # that they are the reference's exact streams is pinned by the known
+    # answers below, which no other stream reproduces.
+    p = oracle.ref_uniform_points(3)
+    assert p.dtype == np.float32 and p.shape == (3, 2)
+    assert np.all((p >= 0) & (p < 1))
+    b = oracle.ref_uniform_boxes(1000, 0.1, 0.1)
+    assert np.all(b[:, 2] >= b[:, 0]) and np.all(b[:, 3] >= b[:, 1])
+    assert np.all(b[:, 2] <= 1.0) and np.all(b[:, 2] - b[:, 0] <= np.float32(0.1) + 1e-7)
+
+
+def test_update_ids_distinct_and_deterministic(oracle, ka):
+    ids = oracle.ref_update_ids(100000, 5000)
+    assert len(set(ids.tolist())) == 5000 and ids.max() < 100000
+    assert ids[:5].tolist() == ka["intersects_after_update_5pct"]["update_ids_first5"]
+
+
+# ------------------------------------------------------------- known answers --
+def test_point_contains_known_answer(oracle, ref100k):
+    # test/test_point_queries.h:9-28,30-60 assert 4224111.
+    pts = oracle.ref_uniform_points(1000)
+    assert oracle.bf_count(oracle.CONTAINS, ref100k, pts) == 4224111
+
+
+def test_intersects_known_answer_and_reference_extra_pair(oracle, ref100k):
+    # test/test_envelope_queries.h:38-58 asserts 6549771. The reference's ray
+    # formulation (ray_params.h:137-166) reproduces it exactly; the exact
+    # closed predicate gives one pair fewer: (query 362, box 76881), where the
+    # query's min y is 1 ulp above the box's max y.
+    q = oracle.ref_uniform_boxes(1000, 0.1, 0.1)
+    exact = oracle.bf_count(oracle.INTERSECTS, ref100k, q)
+    ref, extra, missing = oracle.refray_intersects(ref100k, q)
+    assert ref == 6549771
+    assert exact == 6549770
+    assert extra.tolist() == [[362, 76881]] and missing.size == 0
+    g, qq = ref100k[76881], q[362]
+    assert qq[1] == np.nextafter(g[3], np.float32(2)), "query min y is one ulp above box max y"
+    assert not oracle.evaluate(oracle.INTERSECTS, g, qq)
+
+
+def test_tiny_point_case(oracle, ka):
+    c = ka["tiny_points"]
+    boxes = np.array(c["boxes"], np.float32)
+    pts = np.array(c["points"], np.float32)
+    assert oracle.bf_count(oracle.CONTAINS, boxes, pts) == c["reference"] == 5
+    assert oracle.bf_pairs(oracle.CONTAINS, boxes, pts).tolist() == c["pairs_query_box"]
+
+
+def test_tiny_intersects_and_delete(oracle, ka):
+    c = ka["tiny_intersects_delete"]
+    boxes = np.array(c["boxes"], np.float32)
+    q = np.array(c["queries"], np.float32)
+    assert oracle.bf_count(oracle.INTERSECTS, boxes, q) == 2
+    alive = np.array([1, 0], np.uint8)
+    assert oracle.bf_count(oracle.INTERSECTS, boxes, q, alive=alive) == 1
+
+
+def test_update_count_matches_rebuilt_array(oracle, ref100k, ka):
+    # test/test_envelope_queries.h:93-164: the updated index must answer like a
+    # fresh index over the updated array; the oracle evaluates the latter.
+    ids = oracle.ref_update_ids(100000, 5000).astype(np.int64)
+    new = oracle.ref_uniform_boxes(5000, 0.5, 0.5)
+    b = ref100k.copy()
+    b[ids] = new
+    q = oracle.ref_uniform_boxes(1000, 0.1, 0.1)
+    assert oracle.bf_count(oracle.INTERSECTS, b, q) == ka["intersects_after_update_5pct"]["exact"]
+
+
+# ----------------------------------------------------- closed-form predicates --
+@pytest.mark.parametrize("box,pt,want", [
+    ([0, 0, 1, 1], [0, 0], True),          # corner: closed
+    ([0, 0, 1, 1], [1, 1], True),
+    ([0, 0, 1, 1], [1, 0.5], True),        # edge
+    ([0, 0, 1, 1], [np.nextafter(np.float32(1), np.float32(2)), 0.5], False),
+    ([0.5, 0.5, 0.5, 0.5], [0.5, 0.5], True),   # degenerate box = point
+    ([np.nan] * 4, [0.5, 0.5], False),           # deleted box
+])
+def test_point_predicate_closed_form(oracle, box, pt, want):
+    assert oracle.evaluate(oracle.CONTAINS, np.float32(box), np.float32(pt)) == want
+
+
+@pytest.mark.parametrize("g,q,inter,cont,within", [
+    ([0, 0, 1, 1], [1, 1, 2, 2], True, False, False),       # touching corners
+    ([0, 0, 1, 1], [0.2, 0.2, 0.4, 0.4], True, True, False),
+    ([0.2, 0.2, 0.4, 0.4], [0, 0, 1, 1], True, False, True),
+    ([0, 0, 1, 1], [0, 0, 1, 1], True, True, True),          # equal boxes
+    ([0, 0, 1, 1], [1.5, 0, 2, 1], False, False, False),
+    ([0, 0, 1, 1], [0, 1.0000001, 1, 2], False, False, False),
+    ([0, 0, 0, 1], [0, 0, 0, 1], True, True, True),          # zero width
+    ([np.nan] * 4, [0, 0, 1, 1], False, False, False),
+])
+def test_envelope_predicates_closed_form(oracle, g, q, inter, cont, within):
+    g, q = np.float32(g), np.float32(q)
+    assert oracle.evaluate(oracle.INTERSECTS, g, q) == inter
+    assert oracle.evaluate(oracle.CONTAINS, g, q) == cont
+    assert oracle.evaluate(oracle.WITHIN, g, q) == within
+
+
+def test_predicates_match_numpy_restatement(oracle):
+    # A second, vectorised statement of the closed forms on random inputs
+    # with many ties (coordinates on a coarse grid).
+    rng = np.random.default_rng(7)
+    g = np.sort(rng.integers(0, 8, (400, 2, 2)), axis=1).astype(np.float32) / 8
+    q = np.sort(rng.integers(0, 8, (50, 2, 2)), axis=1).astype(np.float32) / 8
+    g = g.transpose(0, 2, 1).reshape(-1, 4)[:, [0, 2, 1, 3]]
+    q = q.transpose(0, 2, 1).reshape(-1, 4)[:, [0, 2, 1, 3]]
+    G, Q = g[None, :, :], q[:, None, :]
+    inter = (Q[..., 0] <= G[..., 2]) & (Q[..., 2] >= G[..., 0]) & (Q[..., 1] <= G[..., 3]) & (Q[..., 3] >= G[..., 1])
+    cont = (Q[..., 0] >= G[..., 0]) & (Q[..., 2] <= G[..., 2]) & (Q[..., 1] >= G[..., 1]) & (Q[..., 3] <= G[..., 3])
+    within = (G[..., 0] >= Q[..., 0]) & (G[..., 2] <= Q[..., 2]) & (G[..., 1] >= Q[..., 1]) & (G[..., 3] <= Q[..., 3])
+    for pred, m in ((oracle.INTERSECTS, inter), (oracle.CONTAINS, cont), (oracle.WITHIN, within)):
+        want = np.argwhere(m).astype(np.uint32)
+        assert np.array_equal(oracle.bf_pairs(pred, g, q), want)
+    p = (rng.integers(0, 9, (60, 2)) / 8).astype(np.float32)
+    P = p[:, None, :]
+    pin = (P[..., 0] >= G[..., 0]) & (P[..., 0] <= G[..., 2]) & (P[..., 1] >= G[..., 1]) & (P[..., 1] <= G[..., 3])
+    assert np.array_equal(oracle.bf_pairs(oracle.CONTAINS, g, p), np.argwhere(pin).astype(np.uint32))
+
+
+def test_planted_strict_inequality_is_caught(oracle):
+    # A plausible mistake -- an open interval instead of a closed one -- must
+    # change the answer on the tiny case the reference asserts.
+    boxes = np.array([[0, 0, 1, 1], [0.5, -0.5, 1.5, 0.5]], np.float32)
+    q = np.array([[0.75, 0.25, 1.25, 0.75]], np.float32)
+    touching = np.array([[1, 0.5, 2, 2]], np.float32)
+    assert oracle.bf_count(oracle.INTERSECTS, boxes, q) == 2
+    assert oracle.bf_count(oracle.INTERSECTS, boxes, touching) == 2   # closed
+    strict = sum(bool(t[0] < b[2] and t[2] > b[0] and t[1] < b[3] and t[3] > b[1])
+                 for b in boxes for t in touching)
+    assert strict != 2
+
+
+# ------------------------------------------------------------------- R-tree --
+@pytest.mark.parametrize("pred", [0, 1, 2])
+def test_rtree_matches_brute_force(oracle, pred):
+    rng = np.random.default_rng(11 + pred)
+    b = np.concatenate([
+        oracle.ref_uniform_boxes(20000, 0.05, 0.05),
+        np.repeat(np.float32([[0.3, 0.3, 0.31, 0.31]]), 500, axis=0),   # duplicates
+        np.float32([[0.5, 0.5, 0.5, 0.5], [0, 0, 1, 1]]),
+    ])
+    alive = (rng.random(b.shape[0]) > 0.1).astype(np.uint8)
+    q = oracle.ref_uniform_boxes(500, 0.1, 0.1)
+    t = oracle.RTree(b, alive=alive)
+    assert np.array_equal(t.pairs(pred, q), oracle.bf_pairs(pred, b, q, alive=alive))
+    pts = oracle.ref_uniform_points(700)
+    assert np.array_equal(t.pairs(oracle.CONTAINS, pts), oracle.bf_pairs(oracle.CONTAINS, b, pts, alive=alive))
+
+
+def test_rtree_known_answers(oracle, ref100k):
+    t = oracle.RTree(ref100k)
+    assert t.count(oracle.CONTAINS, oracle.ref_uniform_points(1000)) == 4224111
+    assert t.count(oracle.INTERSECTS, oracle.ref_uniform_boxes(1000, 0.1, 0.1)) == 6549770
+
+
+def test_rtree_fanout_and_empty(oracle):
+    b = oracle.ref_uniform_boxes(3000, 0.05, 0.05)
+    q = oracle.ref_uniform_boxes(100, 0.1, 0.1)
+    want = oracle.bf_pairs(oracle.INTERSECTS, b, q)
+    for fan in (2, 4, 16, 64):
+        assert np.array_equal(oracle.RTree(b, fanout=fan).pairs(oracle.INTERSECTS, q), want)
+    empty = oracle.RTree(b, alive=np.zeros(3000, np.uint8))
+    assert empty.count(oracle.INTERSECTS, q) == 0
+
+
+# --------------------------------------------------------------- summaries --
+def test_splitmix64_reference_value(oracle):
+    # First output of SplitMix64 (Steele, Lea, Flood 2014) from state 0.
+    assert oracle.splitmix64(0) == 0xE220A8397B1DCDAF
+    assert oracle._splitmix_np(np.array([0], np.uint64))[0] == 0xE220A8397B1DCDAF
+
+
+def test_summaries_agree(oracle):
+    b = oracle.ref_uniform_boxes(5000, 0.05, 0.05)
+    q = oracle.ref_uniform_boxes(300, 0.1, 0.1)
+    total, cnt, hs, hx = oracle.bf_summary(oracle.INTERSECTS, b, q)
+    pairs = oracle.bf_pairs(oracle.INTERSECTS, b, q)
+    c2, s2, x2 = oracle.summarize_pairs(pairs, q.shape[0])
+    assert total == pairs.shape[0]
+    assert np.array_equal(cnt, c2) and np.array_equal(hs, s2) and np.array_equal(hx, x2)
+    t2, cnt3, hs3, hx3 = oracle.RTree(b).summary(oracle.INTERSECTS, q)
+    assert t2 == total and np.array_equal(cnt3, cnt) and np.array_equal(hs3, hs) and np.array_equal(hx3, hx)
+
+
+# ----------------------------------------------------------------- fixtures --
+def test_wkt_fixtures_consistent(oracle):
+    d = np.load(os.path.join(GOLDEN, "wkt_1000.npz"))
+    for bk in ("uniform", "gaussian"):
+        boxes = d[f"{bk}_boxes"]
+        assert boxes.shape == (1000, 4) and np.all(boxes[:, 2] >= boxes[:, 0])
+        for qk in ("uniform", "gaussian"):
+            got = oracle.bf_pairs(oracle.CONTAINS, boxes, d[f"{qk}_points"])
+            assert np.array_equal(got, d[f"pairs_{bk}_boxes_{qk}_points_contains"])
+            for name, pred in (("intersects", 0), ("contains", 1), ("within", 2)):
+                got = oracle.bf_pairs(pred, boxes, d[f"{qk}_boxes"])
+                assert np.array_equal(got, d[f"pairs_{bk}_boxes_{qk}_boxes_{name}"])
+    # within is contains with the roles of the two sets swapped
+    a = d["pairs_uniform_boxes_gaussian_boxes_within"]
+    b = d["pairs_gaussian_boxes_uniform_boxes_contains"]
+    key = lambda p: np.sort(np.ascontiguousarray(p).view(np.uint64).ravel())
+    assert np.array_equal(key(a[:, ::-1]), key(b))
