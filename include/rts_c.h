@@ -215,7 +215,8 @@ enum {
   RTS_PHASE_REFIT = 3,         /* refit after Update/Delete */
   RTS_PHASE_MUTATE = 4,        /* Update/Delete scatter */
   RTS_PHASE_QUERY_ORDER = 5,   /* Morton ordering of the queries */
-  RTS_PHASE_TRAVERSE = 6,      /* traversal + result emission */
+  RTS_PHASE_TRAVERSE = 6,      /* the traversal kernel that emits the results */
+  RTS_PHASE_QUERY_COUNT = 7,   /* count pass + scan + queue reservation (two-pass queries) */
   RTS_PHASE_COUNT = 8
 };
 

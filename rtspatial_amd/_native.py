@@ -59,7 +59,7 @@ class rts_index_stats(ctypes.Structure):
 
 
 RTS_PHASES = ["build_morton", "build_sort", "build_tree", "refit", "mutate", "query_order",
-              "traverse"]
+              "traverse", "query_count"]
 
 
 class rts_profile(ctypes.Structure):

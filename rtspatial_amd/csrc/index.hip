@@ -394,25 +394,63 @@ int rts_index_update(rts_index* ix, const rts_update* ups, uint64_t n, rts_strea
 
 namespace {
 
-// Morton-orders the queries for traversal coherence; returns a permutation
-// allocated on `stream` (freed by the caller on the same stream).
-int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, hipStream_t stream,
-                  void** block_out, const uint32_t** order_out) {
-  *block_out = nullptr;
-  *order_out = nullptr;
+// Per-call device scratch, carved from one stream-ordered allocation.
+struct QueryScratch {
+  char* block = nullptr;
+  const uint32_t* order = nullptr;  // Morton permutation (optional)
+  uint32_t* counts = nullptr;       // wave two-pass: per-query counts
+  uint64_t* offsets = nullptr;      //                exclusive scan of counts
+  uint64_t* total = nullptr;        //                [0] total, [1] queue base
+  void* scan_tmp = nullptr;
+};
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+int alloc_scratch(uint64_t nq, bool sort, bool two_pass, hipStream_t stream, QueryScratch* qs,
+                  size_t* order_off, size_t* sort_off) {
+  size_t off = 0;
+  size_t o_keys = 0, o_sort = 0, o_counts = 0, o_offsets = 0, o_total = 0, o_scan = 0;
+  if (sort) {
+    o_keys = off;
+    off = align_up(off + 4 * nq * sizeof(uint32_t));
+    o_sort = off;
+    off = align_up(off + sort_scratch_bytes(nq));
+  }
+  if (two_pass) {
+    o_counts = off;
+    off = align_up(off + nq * sizeof(uint32_t));
+    o_offsets = off;
+    off = align_up(off + nq * sizeof(uint64_t));
+    o_total = off;
+    off = align_up(off + 2 * sizeof(uint64_t));
+    o_scan = off;
+    off = align_up(off + scan_scratch_bytes(nq));
+  }
+  if (off == 0) return RTS_OK;
+  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&qs->block), off, stream));
+  char* b = qs->block;
+  if (two_pass) {
+    qs->counts = reinterpret_cast<uint32_t*>(b + o_counts);
+    qs->offsets = reinterpret_cast<uint64_t*>(b + o_offsets);
+    qs->total = reinterpret_cast<uint64_t*>(b + o_total);
+    qs->scan_tmp = b + o_scan;
+  }
+  *order_off = o_keys;
+  *sort_off = o_sort;
+  return RTS_OK;
+}
+
+// Morton-orders the queries for traversal coherence; the permutation lands in
+// the scratch block.
+int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, char* keys_base,
+                  char* sort_base, hipStream_t stream, const uint32_t** order_out) {
   const size_t words = (size_t)nq;
-  const size_t scratch = sort_scratch_bytes(nq);
-  const size_t bytes = 4 * words * sizeof(uint32_t) + scratch + 256;
-  char* base = nullptr;
-  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), bytes, stream));
-  uint32_t* keys = reinterpret_cast<uint32_t*>(base);
+  uint32_t* keys = reinterpret_cast<uint32_t*>(keys_base);
   uint32_t* vals = keys + words;
   uint32_t* keys_b = vals + words;
   uint32_t* vals_b = keys_b + words;
-  char* sbase = reinterpret_cast<char*>(vals_b + words);
-  sbase = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(sbase) + 255) & ~uintptr_t(255));
   SortScratch ss;
-  sort_scratch_carve(sbase, nq, &ss);
+  sort_scratch_carve(sort_base, nq, &ss);
   if (points) {
     RTS_HIP(launch_morton_points(static_cast<const float2*>(queries), (uint32_t)nq, ix->scene, keys,
                                  stream));
@@ -423,9 +461,17 @@ int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, 
   bool in_b = false;
   RTS_HIP(radix_sort_pairs(keys, vals, keys_b, vals_b, (uint32_t)nq, kMortonBits, true, ss, stream,
                            &in_b));
-  *block_out = base;
   *order_out = in_b ? vals_b : vals;
   return RTS_OK;
+}
+
+// AUTO: one query per wavefront where queries have many results (envelope
+// intersects / within), one per lane otherwise (points, envelope contains).
+int resolve_traversal(const rts_index* ix, int pred, bool points) {
+  if (ix->cfg.traversal == RTS_TRAVERSAL_THREAD || ix->cfg.traversal == RTS_TRAVERSAL_WAVE)
+    return (int)ix->cfg.traversal;
+  if (points || pred == RTS_CONTAINS) return RTS_TRAVERSAL_THREAD;
+  return RTS_TRAVERSAL_WAVE;
 }
 
 int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool points,
@@ -445,16 +491,20 @@ int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool po
   if (rc) return rc;
   const uint32_t mode = ix->cfg.sort_queries;
   const bool sort = (mode == 1) || (mode == 0 && nq >= 65536);
+  const int trav = resolve_traversal(ix, pred, points);
+  const bool two_pass = trav == RTS_TRAVERSAL_WAVE && handler == RTS_HANDLER_COLLECT;
   const uint64_t kChunk = 1ull << 30;
   for (uint64_t off = 0; off < nq; off += kChunk) {
     const uint64_t m = std::min(kChunk, nq - off);
     const void* qp = points ? static_cast<const void*>(static_cast<const float2*>(queries) + off)
                             : static_cast<const void*>(static_cast<const float4*>(queries) + off);
-    void* block = nullptr;
-    const uint32_t* order = nullptr;
+    QueryScratch qs;
+    size_t o_keys = 0, o_sort = 0;
+    rc = alloc_scratch(m, sort, two_pass, stream, &qs, &o_keys, &o_sort);
+    if (rc) return rc;
     if (sort) {
       PhaseToken ph = phase_begin(RTS_PHASE_QUERY_ORDER, stream);
-      rc = order_queries(ix, qp, m, points, stream, &block, &order);
+      rc = order_queries(ix, qp, m, points, qs.block + o_keys, qs.block + o_sort, stream, &qs.order);
       phase_end(ph, stream);
       if (rc) return rc;
     }
@@ -463,16 +513,33 @@ int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool po
     ql.queries = qp;
     ql.n = (uint32_t)m;
     ql.qid_base = (uint32_t)off;
-    ql.order = order;
+    ql.order = qs.order;
     ql.pred = pred;
     ql.point_queries = points ? 1 : 0;
     ql.sink.mode = handler;
     ql.sink.arg = arg;
-    ql.traversal = ix->cfg.traversal == RTS_TRAVERSAL_AUTO ? RTS_TRAVERSAL_THREAD : ix->cfg.traversal;
-    PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
-    RTS_HIP(launch_query(ql, stream));
-    phase_end(ph, stream);
-    if (block) RTS_HIP(hipFreeAsync(block, stream));
+    ql.traversal = trav;
+    if (trav == RTS_TRAVERSAL_THREAD) {
+      PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
+      RTS_HIP(launch_query(ql, stream));
+      phase_end(ph, stream);
+    } else if (!two_pass) {
+      WaveArgs wa{nullptr, nullptr, nullptr};
+      PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
+      RTS_HIP(launch_query_wave(ql, wa, kPhaseCountSink, stream));
+      phase_end(ph, stream);
+    } else {
+      WaveArgs wa{qs.counts, qs.offsets, qs.total + 1};
+      PhaseToken ph = phase_begin(RTS_PHASE_QUERY_COUNT, stream);
+      RTS_HIP(launch_query_wave(ql, wa, kPhaseCount, stream));
+      RTS_HIP(launch_exclusive_scan(qs.counts, m, qs.offsets, qs.total, qs.scan_tmp, stream));
+      RTS_HIP(launch_reserve(arg, qs.total, qs.total + 1, stream));
+      phase_end(ph, stream);
+      ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
+      RTS_HIP(launch_query_wave(ql, wa, kPhaseWrite, stream));
+      phase_end(ph, stream);
+    }
+    if (qs.block) RTS_HIP(hipFreeAsync(qs.block, stream));
   }
   return RTS_OK;
 }

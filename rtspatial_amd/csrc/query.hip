@@ -223,6 +223,220 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
   }
 }
 
+// ---------------------------------------------------------------- wave ----
+// One query per wavefront. The wave expands up to 64 stack entries per step
+// (one per lane) from a stack in LDS; children that are leaves, small
+// subtrees (<= leaf_size leaves) or lie inside the query become leaf ranges
+// in an LDS list. Once the list holds enough leaves the wave scans it 64
+// leaves at a time -- consecutive lanes read consecutive leaves -- and writes
+// the matches of this query contiguously.
+//
+// Modes: kWaveCount writes the query's result count to counts[query];
+// kWaveCountSink adds it to the counting handler's counter; kWaveWrite
+// writes the pairs at queue slot base + offsets[query] (offsets from the
+// scan of the counts), so the queue holds each query's pairs contiguously,
+// in query order.
+constexpr int kWaveStack = 1024;    // LDS stack entries per wave
+constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
+constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
+enum { kWaveCount = 0, kWaveCountSink = 1, kWaveWrite = 2 };
+
+template <int PRED, bool POINTS, int MODE>
+__global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, const WaveArgs a) {
+  using P = Pred<PRED, POINTS>;
+  __shared__ uint32_t s_stack[kWaves][kWaveStack];
+  __shared__ uint32_t s_rfirst[kWaves][kWaveRanges];
+  __shared__ uint32_t s_rpre[kWaves][kWaveRanges + 1];
+
+  const TreeView t = q.tree;
+  const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+  const uint32_t lin = blockIdx.x * kWaves + w;
+  if (lin >= q.n || t.n == 0) return;  // wave-uniform
+  const uint32_t qi = q.order ? q.order[lin] : lin;
+  const uint32_t qid = qi + q.qid_base;
+  float4 qb;
+  if (POINTS) {
+    const float2 p = static_cast<const float2*>(q.queries)[qi];
+    qb = make_float4(p.x, p.y, p.x, p.y);
+  } else {
+    qb = static_cast<const float4*>(q.queries)[qi];
+  }
+
+  rts_pair* out = nullptr;
+  uint64_t cap = 0, obase = 0;
+  if (MODE == kWaveWrite) {
+    const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
+    out = dq->data;
+    cap = dq->capacity;
+    obase = a.base[0] + a.offsets[qi];
+  }
+
+  uint32_t* stack = s_stack[w];
+  uint32_t* rfirst = s_rfirst[w];
+  uint32_t* rpre = s_rpre[w];
+  uint32_t sp = 0, m = 0, total = 0;  // wave-uniform
+  uint64_t emitted = 0;
+
+  const float4 rb = t.root_box[0];
+  if (P::node(rb, qb)) {
+    if (t.n <= t.leaf_size || t.n == 1 || P::inside(rb, qb)) {
+      if (lane == 0) {
+        rfirst[0] = 0;
+        rpre[0] = 0;
+        rpre[1] = t.n;
+      }
+      m = 1;
+      total = t.n;
+    } else {
+      if (lane == 0) stack[0] = 0;
+      sp = 1;
+    }
+  }
+  lds_fence();
+
+  const uint64_t max_iters = 4ull * t.n + 64;
+  uint64_t iters = 0;
+  while (sp > 0 || m > 0) {
+    if (++iters > max_iters) {
+      if (lane == 0) atomicOr(t.error, kErrTraversal);
+      break;
+    }
+    if (sp > 0 && total < kFlushLeaves && m + 2 * kWave <= kWaveRanges) {
+      // ---- expand up to 64 stack entries, one per lane
+      const uint32_t c = sp < (uint32_t)kWave ? sp : (uint32_t)kWave;
+      const bool act = lane < c;
+      uint32_t e = act ? stack[sp - 1 - lane] : 0u;
+      lds_fence();
+      sp -= c;
+      // Near a full stack, subtrees are scanned as ranges instead of pushed.
+      const bool degrade = sp + 2 * kWave > (uint32_t)kWaveStack;
+      uint32_t push0 = 0, push1 = 0, np = 0;
+      uint32_t rf0 = 0, rn0 = 0, rf1 = 0, rn1 = 0, nr = 0;
+      if (act && e < t.n - 1) {
+        const uint4 mt = t.meta[e];
+        const float4 lb = t.child_bounds[2 * e];
+        const float4 rbx = t.child_bounds[2 * e + 1];
+        const uint32_t gamma = mt.x & ~kLeafFlag;
+        if (P::node(lb, qb)) {
+          const uint32_t f = mt.z, cnt = gamma - mt.z + 1;
+          if ((mt.x & kLeafFlag) || cnt <= t.leaf_size || degrade || P::inside(lb, qb)) {
+            rf0 = f;
+            rn0 = cnt;
+            nr = 1;
+          } else {
+            push0 = mt.x;
+            np = 1;
+          }
+        }
+        if (P::node(rbx, qb)) {
+          const uint32_t f = gamma + 1, cnt = mt.w - gamma;
+          if ((mt.y & kLeafFlag) || cnt <= t.leaf_size || degrade || P::inside(rbx, qb)) {
+            if (nr == 0) {
+              rf0 = f;
+              rn0 = cnt;
+            } else {
+              rf1 = f;
+              rn1 = cnt;
+            }
+            nr++;
+          } else {
+            if (np == 0) push0 = mt.y; else push1 = mt.y;
+            np++;
+          }
+        }
+      }
+      const uint32_t pinc = wave_inclusive_sum(np);
+      const uint32_t ptot = readlane_u32(pinc, kWave - 1);
+      const uint32_t pex = pinc - np;
+      if (np > 0) stack[sp + pex] = push0;
+      if (np > 1) stack[sp + pex + 1] = push1;
+      sp += ptot;
+      const uint32_t rinc = wave_inclusive_sum(nr);
+      const uint32_t rtot = readlane_u32(rinc, kWave - 1);
+      const uint32_t rex = rinc - nr;
+      const uint32_t sz = rn0 + rn1;
+      const uint32_t sinc = wave_inclusive_sum(sz);
+      const uint32_t stot = readlane_u32(sinc, kWave - 1);
+      const uint32_t pre = total + (sinc - sz);
+      if (nr > 0) {
+        rfirst[m + rex] = rf0;
+        rpre[m + rex] = pre;
+      }
+      if (nr > 1) {
+        rfirst[m + rex + 1] = rf1;
+        rpre[m + rex + 1] = pre + rn0;
+      }
+      m += rtot;
+      total += stot;
+      if (lane == 0) rpre[m] = total;
+      lds_fence();
+    } else {
+      // ---- scan the queued leaf ranges, 64 leaves per step
+      uint32_t e0 = 0;
+      for (uint32_t base = 0; base < total; base += kWave) {
+        const uint32_t j = base + lane;
+        bool hit = false;
+        uint32_t leaf = 0, lo = m - 1;
+        if (j < total) {
+          if (rpre[e0 + 1] >= base + kWave) {
+            lo = e0;  // the whole step lies in one range
+          } else {
+            lo = e0;
+            uint32_t hi = m - 1;
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi + 1) >> 1;
+              if (rpre[mid] <= j) lo = mid; else hi = mid - 1;
+            }
+          }
+          leaf = rfirst[lo] + (j - rpre[lo]);
+          if (leaf < t.n) hit = P::leaf(t.leaf_boxes[leaf], qb);
+        }
+        e0 = readlane_u32(lo, kWave - 1);
+        const uint64_t mask = ballot(hit);
+        if (MODE == kWaveWrite && hit) {
+          const uint64_t slot = obase + emitted + mask_rank(mask);
+          if (slot < cap) {
+            rts_pair pr;
+            pr.geom_id = t.leaf_ids[leaf];
+            pr.query_id = qid;
+            out[slot] = pr;
+          }
+        }
+        emitted += (uint64_t)__popcll(mask);
+      }
+      m = 0;
+      total = 0;
+      lds_fence();
+    }
+  }
+
+  if (lane == 0) {
+    if (MODE == kWaveCount) {
+      a.counts[qi] = (uint32_t)emitted;
+    } else if (MODE == kWaveCountSink) {
+      if (emitted)
+        atomicAdd(static_cast<unsigned long long*>(const_cast<void*>(q.sink.arg)),
+                  (unsigned long long)emitted);
+    }
+  }
+}
+
+template <int PRED, bool POINTS>
+hipError_t launch_wave(const QueryLaunch& q, const WaveArgs& a, int phase, hipStream_t stream) {
+  const dim3 grid((q.n + kWaves - 1) / kWaves), block(kThreads);
+  switch (phase) {
+    case kWaveCount:
+      hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveCount>), grid, block, 0, stream, q, a);
+      break;
+    case kWaveCountSink:
+      hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveCountSink>), grid, block, 0, stream, q, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveWrite>), grid, block, 0, stream, q, a);
+  }
+  return hipGetLastError();
+}
+
 template <int PRED, bool POINTS>
 hipError_t launch_pred(const QueryLaunch& q, hipStream_t stream) {
   const dim3 grid((q.n + kThreads - 1) / kThreads), block(kThreads);
@@ -242,6 +456,18 @@ hipError_t launch_query(const QueryLaunch& q, hipStream_t stream) {
     case RTS_INTERSECTS: return launch_pred<RTS_INTERSECTS, false>(q, stream);
     case RTS_CONTAINS: return launch_pred<RTS_CONTAINS, false>(q, stream);
     case RTS_WITHIN: return launch_pred<RTS_WITHIN, false>(q, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_query_wave(const QueryLaunch& q, const WaveArgs& a, int phase,
+                             hipStream_t stream) {
+  if (q.n == 0) return hipSuccess;
+  if (q.point_queries) return launch_wave<RTS_CONTAINS, true>(q, a, phase, stream);
+  switch (q.pred) {
+    case RTS_INTERSECTS: return launch_wave<RTS_INTERSECTS, false>(q, a, phase, stream);
+    case RTS_CONTAINS: return launch_wave<RTS_CONTAINS, false>(q, a, phase, stream);
+    case RTS_WITHIN: return launch_wave<RTS_WITHIN, false>(q, a, phase, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -125,6 +125,26 @@ struct QueryLaunch {
   int traversal;         // rts_traversal (resolved: THREAD or WAVE)
 };
 hipError_t launch_query(const QueryLaunch& q, hipStream_t stream);
+// Wave-per-query traversal. phase 0: per-query counts -> a.counts;
+// 1: counts added to the counting handler's counter; 2: pairs written at
+// queue slot a.base[0] + a.offsets[query].
+struct WaveArgs {
+  uint32_t* counts;
+  const uint64_t* offsets;
+  const uint64_t* base;
+};
+enum { kPhaseCount = 0, kPhaseCountSink = 1, kPhaseWrite = 2 };
+hipError_t launch_query_wave(const QueryLaunch& q, const WaveArgs& a, int phase,
+                             hipStream_t stream);
+
+// scan.hip
+size_t scan_scratch_bytes(uint64_t n);
+// offsets[i] = counts[0] + ... + counts[i-1]; *total = the sum of all counts.
+hipError_t launch_exclusive_scan(const uint32_t* counts, uint64_t n, uint64_t* offsets,
+                                 uint64_t* total, void* scratch, hipStream_t stream);
+// *base = atomicAdd(queue counter, *total).
+hipError_t launch_reserve(const void* dev_queue, const uint64_t* total, uint64_t* base,
+                          hipStream_t stream);
 
 // index.hip
 void set_error(const std::string& msg);

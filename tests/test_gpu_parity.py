@@ -19,6 +19,8 @@ from gpu_util import canon, collect, count, dev, log, make_index, pinned  # noqa
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 P = R.Predicate
+THREAD, WAVE = R._native.RTS_TRAVERSAL_THREAD, R._native.RTS_TRAVERSAL_WAVE
+MODES = [THREAD, WAVE]
 
 
 @pytest.fixture(scope="module")
@@ -116,33 +118,69 @@ def test_update_refit_equals_rebuild(ref_inputs, ka, oracle):
 
 
 # -------------------------------------------------------------- pair-set parity
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("leaf_size", [1, 4, 8, 32])
-def test_c1_point_pairs_exact(oracle, leaf_size):
+def test_c1_point_pairs_exact(oracle, leaf_size, mode):
     # Config 1 at full size: 100k boxes x 100k points, sorted pair set vs brute force.
-    log(f"c1 point pairs leaf_size={leaf_size}")
+    log(f"c1 point pairs leaf_size={leaf_size} traversal={mode}")
     from rtspatial_amd import workloads as W
     b, q = W.config1()
     want = oracle.bf_pairs(oracle.CONTAINS, b, q)
-    ix = make_index(b, leaf_size=leaf_size)
+    ix = make_index(b, leaf_size=leaf_size, traversal=mode)
     got, n = collect(ix, P.kContains, q)
     assert n == want.shape[0]
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("pred", [P.kIntersects, P.kContains, P.kWithin])
-def test_c2_style_envelope_pairs_exact(oracle, pred):
+def test_c2_style_envelope_pairs_exact(oracle, pred, mode):
     # Config 2 shapes at 200k x 100k: every predicate, sorted pair set vs R-tree.
-    log(f"c2-style envelope pairs pred={pred.name}")
+    log(f"c2-style envelope pairs pred={pred.name} traversal={mode}")
     from rtspatial_amd import workloads as W
     b, q = W.config2(200_000, 100_000)
     if pred == P.kContains:
         # data boxes that contain a query: make the queries small
         q = W.uniform_boxes(np.random.default_rng(5), 100_000, 1e-4)
     want = oracle.RTree(b).pairs(int(pred), q)
-    ix = make_index(b)
+    ix = make_index(b, traversal=mode)
     got, n = collect(ix, pred, q)
     assert n == want.shape[0] and n > 0
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("leaf_size", [1, 8, 64])
+def test_c3_style_skewed_intersects_exact(oracle, leaf_size):
+    # Config 3's distributions at 1M boxes x 20k queries (wave traversal, the
+    # mode AUTO picks): sorted pair set vs R-tree.
+    log(f"c3-style skewed intersects leaf_size={leaf_size}")
+    from rtspatial_amd import workloads as W
+    b, q = W.config3(1_000_000, 20_000)
+    want = oracle.RTree(b).pairs(oracle.INTERSECTS, q)
+    ix = make_index(b, leaf_size=leaf_size)
+    got, n = collect(ix, P.kIntersects, q)
+    assert n == want.shape[0] and n > 1_000_000
+    assert np.array_equal(got, want)
+
+
+def test_wave_output_is_grouped_in_query_order_and_appends(oracle):
+    # The two-pass wave path writes each query's pairs contiguously, queries
+    # in input order; a second Query on the same queue appends after the first.
+    log("wave output grouping + append")
+    from rtspatial_amd import workloads as W
+    b, q = W.config2(100_000, 30_000)
+    ix = make_index(b, traversal=WAVE)
+    n1 = count(ix, P.kIntersects, q)
+    queue = R.Queue(2 * n1 + 10)
+    ix.Query(P.kIntersects, dev(q), queue)
+    ix.Query(P.kIntersects, dev(q[:1000]), queue)
+    n1000 = oracle.bf_count(oracle.INTERSECTS, b, q[:1000])
+    assert queue.size() == n1 + n1000
+    raw = queue.pairs().cpu().numpy()
+    first, second = raw[:n1], raw[n1:]
+    assert np.all(np.diff(first[:, 1]) >= 0)
+    assert np.all(np.diff(second[:, 1]) >= 0)
+    assert np.array_equal(canon(torch.from_numpy(second)), oracle.bf_pairs(oracle.INTERSECTS, b, q[:1000]))
 
 
 @pytest.mark.parametrize("name", ["uniform", "gaussian"])
@@ -172,20 +210,22 @@ def test_empty_index_and_empty_queries():
     assert count(ix, P.kIntersects, q) == 1
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 4095, 4096, 4097])
-def test_small_sizes_and_tile_edges(oracle, n):
-    log(f"small size n={n}")
+def test_small_sizes_and_tile_edges(oracle, n, mode):
+    log(f"small size n={n} traversal={mode}")
     rng = np.random.default_rng(n)
     from rtspatial_amd import workloads as W
     b = W.uniform_boxes(rng, n, 0.2)
     q = W.uniform_boxes(rng, 257, 0.3)
-    ix = make_index(b, leaf_size=1)
+    ix = make_index(b, leaf_size=1, traversal=mode)
     for pred in (P.kIntersects, P.kContains, P.kWithin):
         got, _ = collect(ix, pred, q)
         assert np.array_equal(got, oracle.bf_pairs(int(pred), b, q))
 
 
-def test_duplicates_degenerate_and_boundaries(oracle):
+@pytest.mark.parametrize("mode", MODES)
+def test_duplicates_degenerate_and_boundaries(oracle, mode):
     # Identical boxes (equal Morton codes), zero-width boxes, boxes touching
     # the query only on an edge or a corner, points on edges.
     log("duplicates / degenerate / boundaries")
@@ -198,7 +238,7 @@ def test_duplicates_degenerate_and_boundaries(oracle):
     q = np.concatenate([cells, np.float32([[0.5, 0.5, 0.5, 0.5], [0.25, 0.25, 0.5, 0.5], [0, 0, 1, 1]])])
     pts = np.stack(np.meshgrid(grid, grid), -1).reshape(-1, 2).astype(np.float32)
     for leaf in (1, 8):
-        ix = make_index(b, leaf_size=leaf)
+        ix = make_index(b, leaf_size=leaf, traversal=mode)
         for pred in (P.kIntersects, P.kContains, P.kWithin):
             got, _ = collect(ix, pred, q)
             assert np.array_equal(got, oracle.bf_pairs(int(pred), b, q)), (leaf, pred)
@@ -206,16 +246,17 @@ def test_duplicates_degenerate_and_boundaries(oracle):
         assert np.array_equal(got, oracle.bf_pairs(oracle.CONTAINS, b, pts))
 
 
-def test_delete_update_sequence_matches_oracle(oracle):
+@pytest.mark.parametrize("mode", MODES)
+def test_delete_update_sequence_matches_oracle(oracle, mode):
     # Deletes (including out-of-range ids, which are ignored), updates,
     # re-updates of deleted ids, inserts after deletes.
-    log("delete/update sequence")
+    log(f"delete/update sequence traversal={mode}")
     from rtspatial_amd import workloads as W
     rng = np.random.default_rng(3)
     b = W.uniform_boxes(rng, 50_000, 0.01)
     q = W.uniform_boxes(rng, 2000, 0.05)
     alive = np.ones(b.shape[0], np.uint8)
-    ix = make_index(b)
+    ix = make_index(b, traversal=mode)
     for step in range(4):
         dels = rng.choice(b.shape[0], 3000, replace=False).astype(np.int64)
         ix.Delete(dev(np.concatenate([dels, [b.shape[0] + 5, 1 << 40]]).astype(np.int64)))
@@ -236,12 +277,13 @@ def test_delete_update_sequence_matches_oracle(oracle):
     ix.Sync()
 
 
-def test_queue_overflow_reports_exact_count(oracle):
-    log("queue overflow")
+@pytest.mark.parametrize("mode", MODES)
+def test_queue_overflow_reports_exact_count(oracle, mode):
+    log(f"queue overflow traversal={mode}")
     from rtspatial_amd import workloads as W
     b, q = W.config2(100_000, 20_000)
     want = oracle.bf_count(oracle.WITHIN, b, q)
-    ix = make_index(b)
+    ix = make_index(b, traversal=mode)
     queue = R.Queue(1000)
     ix.Query(P.kWithin, dev(q), queue)
     assert queue.size() == want and queue.overflowed()
