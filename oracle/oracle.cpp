@@ -157,8 +157,15 @@ void str_order(std::vector<uint32_t>& items, const std::vector<Box>& cb, int fan
   size_t leaves = (n + fanout - 1) / fanout;
   size_t slabs = (size_t)std::ceil(std::sqrt((double)leaves));
   size_t per_slab = slabs ? ((leaves + slabs - 1) / slabs) * fanout : n;
-  auto cx = [&](uint32_t i) { return 0.5f * (cb[i].x0 + cb[i].x1); };
-  auto cy = [&](uint32_t i) { return 0.5f * (cb[i].y0 + cb[i].y1); };
+  // NaN centres (never-matching boxes) sort last so the order stays strict-weak.
+  auto cx = [&](uint32_t i) {
+    float v = 0.5f * (cb[i].x0 + cb[i].x1);
+    return std::isnan(v) ? INFINITY : v;
+  };
+  auto cy = [&](uint32_t i) {
+    float v = 0.5f * (cb[i].y0 + cb[i].y1);
+    return std::isnan(v) ? INFINITY : v;
+  };
   std::sort(items.begin(), items.end(), [&](uint32_t a, uint32_t b) {
     float xa = cx(a), xb = cx(b);
     return xa < xb || (xa == xb && a < b);
@@ -195,7 +202,10 @@ RTree* rtree_build(const Box* boxes, uint64_t n, const uint8_t* alive, int fanou
     Box b = kEmptyBox;
     for (size_t i = s; i < e; i++) {
       const Box& x = t->entry_boxes[i];
-      if (!std::isnan(x.x0)) b = unite(b, x);
+      if (std::isnan(x.x0) || std::isnan(x.y0) || std::isnan(x.x1) || std::isnan(x.y1)) continue;
+      // Cover the box's extent even when its corners are swapped.
+      Box nx{std::min(x.x0, x.x1), std::min(x.y0, x.y1), std::max(x.x0, x.x1), std::max(x.y0, x.y1)};
+      b = unite(b, nx);
     }
     level.push_back((uint32_t)t->nodes.size());
     t->nodes.push_back(b);

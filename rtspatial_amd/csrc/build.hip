@@ -186,16 +186,40 @@ __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict_
 // 30-bit keys with position tie-breaks); the level bound only guards against
 // a corrupt parent array, and reports it through `error`.
 constexpr int kMaxLevels = 128;
+constexpr int kRefitPerThread = 8;
+constexpr uint32_t kRefitChunk = kThreads * kRefitPerThread;  // leaves per workgroup
 
+// Workgroup b walks up from leaves [b*2048, (b+1)*2048). A node whose leaf
+// range lies inside that chunk has both children finished by this workgroup,
+// so its arrival counter only needs workgroup-scope ordering (no L2
+// write-back or invalidation); only nodes that straddle chunks -- about one in
+// a hundred -- synchronize at device scope. Each child slot also receives the
+// number of live (not deleted) leaves below it.
 __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ leaf_boxes, uint32_t n,
                                                     const uint32_t* __restrict__ parent,
-                                                    float4* child_bounds, float4* __restrict__ root_box,
+                                                    const uint4* __restrict__ meta,
+                                                    float4* child_bounds, uint32_t* child_live,
+                                                    float4* __restrict__ root_box,
+                                                    uint32_t* __restrict__ root_live,
                                                     uint32_t* flags, uint32_t* error) {
-  for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < n; k += gridDim.x * kThreads) {
+  const uint32_t c0 = blockIdx.x * kRefitChunk;
+  const uint32_t c1 = (c0 + kRefitChunk < n ? c0 + kRefitChunk : n) - 1;  // inclusive
+  for (int r = 0; r < kRefitPerThread; r++) {
+    const uint32_t k = c0 + r * kThreads + threadIdx.x;
+    if (k >= n) break;
     float4 b = leaf_boxes[k];
-    if (box_dead(b)) b = empty_box();
+    uint32_t live = 1;
+    if (box_dead(b)) {
+      b = empty_box();
+      live = 0;
+    } else {
+      // Bounds cover the box's extent even if its corners are swapped, so a
+      // node inside a query implies every live box below it matches.
+      b = make_float4(fminf(b.x, b.z), fminf(b.y, b.w), fmaxf(b.x, b.z), fmaxf(b.y, b.w));
+    }
     if (n == 1) {
       root_box[0] = b;
+      root_live[0] = live;
       continue;
     }
     uint32_t code = parent[(n - 1) + k];
@@ -206,14 +230,20 @@ __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ l
         break;
       }
       child_bounds[2 * p + side] = b;
+      child_live[2 * p + side] = live;
+      const uint4 mp = meta[p];
+      const bool local = mp.z >= c0 && mp.w <= c1;
       // Release our child's bounds; acquire the sibling's if we arrive second.
-      uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL,
-                                            __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t old =
+          local ? __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP)
+                : __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
       if (old == 0) break;
       const float4 o = child_bounds[2 * p + (side ^ 1u)];
+      live += child_live[2 * p + (side ^ 1u)];
       b = unite(b, o);
       if (p == 0) {
         root_box[0] = b;
+        root_live[0] = live;
         break;
       }
       code = parent[p];
@@ -337,15 +367,16 @@ hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, uint4* meta, u
   return hipGetLastError();
 }
 
-hipError_t launch_refit(const float4* leaf_boxes, uint32_t n, const uint32_t* parent,
-                        float4* child_bounds, float4* root_box, uint32_t* flags, uint32_t* error,
-                        hipStream_t stream) {
+hipError_t launch_refit(const RefitArgs& a, hipStream_t stream) {
+  const uint32_t n = a.n;
   if (n == 0) return hipSuccess;
   if (n > 1)
-    hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, flags,
+    hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, a.flags,
                        n - 1);
-  hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(kThreads), 0, stream, leaf_boxes, n, parent,
-                     child_bounds, root_box, flags, error);
+  const uint32_t blocks = (n + kRefitChunk - 1) / kRefitChunk;
+  hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(kThreads), 0, stream, a.leaf_boxes, n, a.parent,
+                     a.meta, a.child_bounds, a.child_live, a.root_box, a.root_live, a.flags,
+                     a.error);
   return hipGetLastError();
 }
 

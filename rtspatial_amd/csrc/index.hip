@@ -95,6 +95,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   dev_free(ix->vals_alt);
   dev_free(ix->meta);
   dev_free(ix->child_bounds);
+  dev_free(ix->child_live);
   dev_free(ix->parent);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);
@@ -106,6 +107,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   RTS_HIP(dev_alloc(&ix->vals_alt, cap));
   RTS_HIP(dev_alloc(&ix->meta, cap));
   RTS_HIP(dev_alloc(&ix->child_bounds, 2 * cap));
+  RTS_HIP(dev_alloc(&ix->child_live, 2 * cap));
   RTS_HIP(dev_alloc(&ix->parent, 2 * cap));
   RTS_HIP(dev_alloc(&ix->flags, cap));
   RTS_HIP(dev_alloc(&ix->sort_scratch, sort_scratch_bytes(cap)));
@@ -154,6 +156,21 @@ int measure_cost(rts_index* ix, bool after_build, hipStream_t stream) {
   return RTS_OK;
 }
 
+RefitArgs refit_args(rts_index* ix, uint32_t n) {
+  RefitArgs a;
+  a.leaf_boxes = ix->leaf_boxes;
+  a.n = n;
+  a.parent = ix->parent;
+  a.meta = ix->meta;
+  a.child_bounds = ix->child_bounds;
+  a.child_live = ix->child_live;
+  a.root_box = ix->root_box;
+  a.root_live = ix->root_live;
+  a.flags = ix->flags;
+  a.error = ix->error;
+  return a;
+}
+
 int build_tree(rts_index* ix, hipStream_t stream) {
   const uint64_t N = ix->n;
   ix->n_tree = 0;
@@ -181,8 +198,7 @@ int build_tree(rts_index* ix, hipStream_t stream) {
   ph = phase_begin(RTS_PHASE_BUILD_TREE, stream);
   RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids, n, ix->leaf_boxes, ix->leaf_of_id, stream));
   RTS_HIP(launch_karras(ix->keys, n, ix->meta, ix->parent, ix->flags, stream));
-  RTS_HIP(launch_refit(ix->leaf_boxes, n, ix->parent, ix->child_bounds, ix->root_box, ix->flags,
-                       ix->error, stream));
+  RTS_HIP(launch_refit(refit_args(ix, n), stream));
   phase_end(ph, stream);
   ix->n_tree = N;
   ix->stale = false;
@@ -199,8 +215,7 @@ int after_mutation(rts_index* ix, hipStream_t stream) {
     return RTS_OK;
   }
   PhaseToken ph = phase_begin(RTS_PHASE_REFIT, stream);
-  RTS_HIP(launch_refit(ix->leaf_boxes, (uint32_t)ix->n_tree, ix->parent, ix->child_bounds,
-                       ix->root_box, ix->flags, ix->error, stream));
+  RTS_HIP(launch_refit(refit_args(ix, (uint32_t)ix->n_tree), stream));
   phase_end(ph, stream);
   ix->refits++;
   if (ix->cfg.rebuild_policy == RTS_REBUILD_SAH) return measure_cost(ix, false, stream);
@@ -234,7 +249,9 @@ TreeView tree_view(const rts_index* ix) {
   t.leaf_ids = ix->leaf_ids;
   t.meta = ix->meta;
   t.child_bounds = ix->child_bounds;
+  t.child_live = ix->child_live;
   t.root_box = ix->root_box;
+  t.root_live = ix->root_live;
   t.n = (uint32_t)ix->n_tree;
   t.leaf_size = ix->cfg.leaf_size;
   t.error = ix->error;
@@ -292,6 +309,7 @@ int rts_index_create(const rts_config* cfg, rts_index** out) {
   ix->root_box = reinterpret_cast<float4*>(ix->small + 16);
   ix->cost = reinterpret_cast<double*>(ix->small + 32);
   ix->error = reinterpret_cast<uint32_t*>(ix->small + 64);
+  ix->root_live = reinterpret_cast<uint32_t*>(ix->small + 80);
   RTS_HIP(hipMemset(ix->small, 0, 256));
   RTS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ix->h_cost), 4 * sizeof(double)));
   RTS_HIP(hipEventCreateWithFlags(&ix->ev_cost_build, hipEventDisableTiming));
@@ -320,6 +338,7 @@ int rts_index_destroy(rts_index* ix) {
   dev_free(ix->vals_alt);
   dev_free(ix->meta);
   dev_free(ix->child_bounds);
+  dev_free(ix->child_live);
   dev_free(ix->parent);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);

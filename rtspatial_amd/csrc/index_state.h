@@ -31,6 +31,7 @@ struct rts_index {
   uint32_t* vals_alt = nullptr;
   uint4* meta = nullptr;
   float4* child_bounds = nullptr;
+  uint32_t* child_live = nullptr;
   uint32_t* parent = nullptr;
   uint32_t* flags = nullptr;
   char* sort_scratch = nullptr;
@@ -42,6 +43,7 @@ struct rts_index {
   char* small = nullptr;
   float4* scene = nullptr;
   float4* root_box = nullptr;
+  uint32_t* root_live = nullptr;
   double* cost = nullptr;
   uint32_t* error = nullptr;  // device error bits (kErr*)
 

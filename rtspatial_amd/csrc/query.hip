@@ -241,6 +241,18 @@ constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
 constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
 enum { kWaveCount = 0, kWaveCountSink = 1, kWaveWrite = 2 };
 
+// Workgroups are dealt to the 8 XCDs round-robin; remapping the launch index
+// gives each XCD (and its L2) one contiguous slice of the query order.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_slice_block(uint32_t b, uint32_t grid) {
+  const uint32_t per = grid / kXcds;  // grid is a multiple of kXcds
+  return (b % kXcds) * per + b / kXcds;
+}
+
+// Range entries: first leaf | kAllMatch when every leaf of the range is a
+// live box inside the query (no test needed).
+constexpr uint32_t kAllMatch = 0x80000000u;
+
 template <int PRED, bool POINTS, int MODE>
 __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, const WaveArgs a) {
   using P = Pred<PRED, POINTS>;
@@ -250,7 +262,7 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
 
   const TreeView t = q.tree;
   const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-  const uint32_t lin = blockIdx.x * kWaves + w;
+  const uint32_t lin = xcd_slice_block(blockIdx.x, gridDim.x) * kWaves + w;
   if (lin >= q.n || t.n == 0) return;  // wave-uniform
   const uint32_t qi = q.order ? q.order[lin] : lin;
   const uint32_t qid = qi + q.qid_base;
@@ -270,18 +282,23 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
     cap = dq->capacity;
     obase = a.base[0] + a.offsets[qi];
   }
+  constexpr bool kCounting = MODE != kWaveWrite;
 
   uint32_t* stack = s_stack[w];
   uint32_t* rfirst = s_rfirst[w];
   uint32_t* rpre = s_rpre[w];
   uint32_t sp = 0, m = 0, total = 0;  // wave-uniform
-  uint64_t emitted = 0;
+  uint64_t emitted = 0;               // wave-uniform: results of scanned ranges
+  uint64_t inside_live = 0;           // per lane: counting mode, subtrees inside the query
 
   const float4 rb = t.root_box[0];
   if (P::node(rb, qb)) {
-    if (t.n <= t.leaf_size || t.n == 1 || P::inside(rb, qb)) {
+    const bool inside = P::inside(rb, qb);
+    if (kCounting && inside) {
+      if (lane == 0) inside_live = t.root_live[0];
+    } else if (t.n <= t.leaf_size || t.n == 1 || inside) {
       if (lane == 0) {
-        rfirst[0] = 0;
+        rfirst[0] = (inside && t.root_live[0] == t.n) ? kAllMatch : 0u;
         rpre[0] = 0;
         rpre[1] = t.n;
       }
@@ -314,33 +331,30 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
       uint32_t rf0 = 0, rn0 = 0, rf1 = 0, rn1 = 0, nr = 0;
       if (act && e < t.n - 1) {
         const uint4 mt = t.meta[e];
-        const float4 lb = t.child_bounds[2 * e];
-        const float4 rbx = t.child_bounds[2 * e + 1];
+        const float4 cb[2] = {t.child_bounds[2 * e], t.child_bounds[2 * e + 1]};
         const uint32_t gamma = mt.x & ~kLeafFlag;
-        if (P::node(lb, qb)) {
-          const uint32_t f = mt.z, cnt = gamma - mt.z + 1;
-          if ((mt.x & kLeafFlag) || cnt <= t.leaf_size || degrade || P::inside(lb, qb)) {
-            rf0 = f;
-            rn0 = cnt;
-            nr = 1;
-          } else {
-            push0 = mt.x;
-            np = 1;
-          }
-        }
-        if (P::node(rbx, qb)) {
-          const uint32_t f = gamma + 1, cnt = mt.w - gamma;
-          if ((mt.y & kLeafFlag) || cnt <= t.leaf_size || degrade || P::inside(rbx, qb)) {
+        const uint32_t ref[2] = {mt.x, mt.y};
+        const uint32_t first[2] = {mt.z, gamma + 1};
+        const uint32_t cnt[2] = {gamma - mt.z + 1, mt.w - gamma};
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          if (!P::node(cb[side], qb)) continue;
+          const bool inside = P::inside(cb[side], qb);
+          if (kCounting && inside) {
+            inside_live += t.child_live[2 * e + side];
+          } else if ((ref[side] & kLeafFlag) || cnt[side] <= t.leaf_size || degrade || inside) {
+            uint32_t f = first[side];
+            if (inside && t.child_live[2 * e + side] == cnt[side]) f |= kAllMatch;
             if (nr == 0) {
               rf0 = f;
-              rn0 = cnt;
+              rn0 = cnt[side];
             } else {
               rf1 = f;
-              rn1 = cnt;
+              rn1 = cnt[side];
             }
             nr++;
           } else {
-            if (np == 0) push0 = mt.y; else push1 = mt.y;
+            if (np == 0) push0 = ref[side]; else push1 = ref[side];
             np++;
           }
         }
@@ -378,24 +392,27 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
         bool hit = false;
         uint32_t leaf = 0, lo = m - 1;
         if (j < total) {
-          if (rpre[e0 + 1] >= base + kWave) {
-            lo = e0;  // the whole step lies in one range
-          } else {
-            lo = e0;
+          lo = e0;
+          if (rpre[e0 + 1] < base + kWave) {  // the step spans more than one range
             uint32_t hi = m - 1;
             while (lo < hi) {
               const uint32_t mid = (lo + hi + 1) >> 1;
               if (rpre[mid] <= j) lo = mid; else hi = mid - 1;
             }
           }
-          leaf = rfirst[lo] + (j - rpre[lo]);
-          if (leaf < t.n) hit = P::leaf(t.leaf_boxes[leaf], qb);
+          const uint32_t f = rfirst[lo];
+          leaf = (f & ~kAllMatch) + (j - rpre[lo]);
+          if (f & kAllMatch) {
+            hit = true;
+          } else if (leaf < t.n) {
+            hit = P::leaf(t.leaf_boxes[leaf], qb);
+          }
         }
         e0 = readlane_u32(lo, kWave - 1);
         const uint64_t mask = ballot(hit);
         if (MODE == kWaveWrite && hit) {
           const uint64_t slot = obase + emitted + mask_rank(mask);
-          if (slot < cap) {
+          if (slot < cap && leaf < t.n) {
             rts_pair pr;
             pr.geom_id = t.leaf_ids[leaf];
             pr.query_id = qid;
@@ -410,20 +427,24 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
     }
   }
 
-  if (lane == 0) {
-    if (MODE == kWaveCount) {
-      a.counts[qi] = (uint32_t)emitted;
-    } else if (MODE == kWaveCountSink) {
-      if (emitted)
+  if (kCounting) {
+    const uint64_t all = emitted + wave_sum_u64(inside_live);
+    if (lane == 0) {
+      if (MODE == kWaveCount) {
+        a.counts[qi] = (uint32_t)all;
+      } else if (all) {
         atomicAdd(static_cast<unsigned long long*>(const_cast<void*>(q.sink.arg)),
-                  (unsigned long long)emitted);
+                  (unsigned long long)all);
+      }
     }
   }
 }
 
 template <int PRED, bool POINTS>
 hipError_t launch_wave(const QueryLaunch& q, const WaveArgs& a, int phase, hipStream_t stream) {
-  const dim3 grid((q.n + kWaves - 1) / kWaves), block(kThreads);
+  uint32_t blocks = (q.n + kWaves - 1) / kWaves;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;  // whole rounds over the XCDs
+  const dim3 grid(blocks), block(kThreads);
   switch (phase) {
     case kWaveCount:
       hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveCount>), grid, block, 0, stream, q, a);

@@ -33,7 +33,9 @@ struct TreeView {
   const uint32_t* leaf_ids;
   const uint4* meta;
   const float4* child_bounds;
+  const uint32_t* child_live;  // live leaves below child slot [2i + side]
   const float4* root_box;   // device scalar
+  const uint32_t* root_live;  // device scalar: live leaves in the tree
   uint32_t n;               // leaves
   uint32_t leaf_size;       // ranges with <= leaf_size leaves are scanned directly
   uint32_t* error;          // device word: bit 1 refit walk, bit 2 traversal bound
@@ -99,9 +101,19 @@ hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids,
                                 float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream);
 hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, uint4* meta,
                          uint32_t* parent, uint32_t* flags, hipStream_t stream);
-hipError_t launch_refit(const float4* leaf_boxes, uint32_t n, const uint32_t* parent,
-                        float4* child_bounds, float4* root_box, uint32_t* flags, uint32_t* error,
-                        hipStream_t stream);
+struct RefitArgs {
+  const float4* leaf_boxes;
+  uint32_t n;
+  const uint32_t* parent;
+  const uint4* meta;
+  float4* child_bounds;
+  uint32_t* child_live;
+  float4* root_box;
+  uint32_t* root_live;
+  uint32_t* flags;
+  uint32_t* error;
+};
+hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
 hipError_t launch_tree_cost(const float4* child_bounds, const float4* root_box, uint32_t n,
                             double* out, hipStream_t stream);
 hipError_t launch_delete(const uint64_t* ids, uint64_t n_ids, float4* boxes, uint64_t n_boxes,

@@ -247,6 +247,29 @@ def test_duplicates_degenerate_and_boundaries(oracle, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_inverted_boxes_match_exact_predicates(oracle, mode):
+    # Boxes whose min corner exceeds the max corner on one or both axes still
+    # go through the exact closed predicates; bounds and the inside-subtree
+    # shortcut must not change their results.
+    log(f"inverted boxes traversal={mode}")
+    from rtspatial_amd import workloads as W
+    rng = np.random.default_rng(17)
+    b = W.uniform_boxes(rng, 30_000, 0.02)
+    flip = rng.random(b.shape[0]) < 0.3
+    b[flip] = b[flip][:, [2, 1, 0, 3]]
+    flip2 = rng.random(b.shape[0]) < 0.2
+    b[flip2] = b[flip2][:, [0, 3, 2, 1]]
+    q = np.concatenate([W.uniform_boxes(rng, 500, 0.3), np.float32([[0, 0, 1, 1]])])
+    pts = W.uniform_points(rng, 2000)
+    ix = make_index(b, leaf_size=4, traversal=mode)
+    for pred in (P.kIntersects, P.kContains, P.kWithin):
+        got, _ = collect(ix, pred, q)
+        assert np.array_equal(got, oracle.bf_pairs(int(pred), b, q)), pred
+    got, _ = collect(ix, P.kContains, pts)
+    assert np.array_equal(got, oracle.bf_pairs(oracle.CONTAINS, b, pts))
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_delete_update_sequence_matches_oracle(oracle, mode):
     # Deletes (including out-of-range ids, which are ignored), updates,
     # re-updates of deleted ids, inserts after deletes.
@@ -271,9 +294,11 @@ def test_delete_update_sequence_matches_oracle(oracle, mode):
             ix.Insert(dev(extra))
             b = np.concatenate([b, extra])
             alive = np.concatenate([alive, np.ones(5000, np.uint8)])
+        qq = np.concatenate([q, np.float32([[-1, -1, 2, 2], [0.2, 0.2, 0.7, 0.7]])])
         for pred in (P.kIntersects, P.kWithin):
-            got, _ = collect(ix, pred, q)
-            assert np.array_equal(got, oracle.bf_pairs(int(pred), b, q, alive=alive)), (step, pred)
+            got, _ = collect(ix, pred, qq)
+            assert np.array_equal(got, oracle.bf_pairs(int(pred), b, qq, alive=alive)), (step, pred)
+            assert count(ix, pred, qq) == got.shape[0]
     ix.Sync()
 
 

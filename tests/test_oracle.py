@@ -171,6 +171,17 @@ def test_rtree_matches_brute_force(oracle, pred):
     assert np.array_equal(t.pairs(oracle.CONTAINS, pts), oracle.bf_pairs(oracle.CONTAINS, b, pts, alive=alive))
 
 
+def test_rtree_handles_inverted_boxes(oracle):
+    rng = np.random.default_rng(5)
+    b = oracle.ref_uniform_boxes(5000, 0.05, 0.05)
+    flip = rng.random(5000) < 0.4
+    b[flip] = b[flip][:, [2, 3, 0, 1]]
+    q = oracle.ref_uniform_boxes(300, 0.2, 0.2)
+    t = oracle.RTree(b)
+    for pred in (0, 1, 2):
+        assert np.array_equal(t.pairs(pred, q), oracle.bf_pairs(pred, b, q))
+
+
 def test_rtree_known_answers(oracle, ref100k):
     t = oracle.RTree(ref100k)
     assert t.count(oracle.CONTAINS, oracle.ref_uniform_points(1000)) == 4224111
