@@ -147,7 +147,7 @@ __device__ __forceinline__ int prefix_len(const uint32_t* __restrict__ keys, int
 }
 
 __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict__ keys, uint32_t n,
-                                                     uint4* __restrict__ meta,
+                                                     NodeRec* __restrict__ nodes,
                                                      uint32_t* __restrict__ parent,
                                                      uint32_t* __restrict__ flags) {
   const int64_t N = n;
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict_
     const uint32_t left = (first == gamma) ? ((uint32_t)gamma | kLeafFlag) : (uint32_t)gamma;
     const uint32_t right =
         (last == gamma + 1) ? ((uint32_t)(gamma + 1) | kLeafFlag) : (uint32_t)(gamma + 1);
-    meta[i] = make_uint4(left, right, (uint32_t)first, (uint32_t)last);
+    nodes[i].meta = make_uint4(left, right, (uint32_t)first, (uint32_t)last);
     const uint32_t lslot = (left & kLeafFlag) ? (uint32_t)(N - 1) + (left & ~kLeafFlag) : left;
     const uint32_t rslot = (right & kLeafFlag) ? (uint32_t)(N - 1) + (right & ~kLeafFlag) : right;
     parent[lslot] = ((uint32_t)i << 1) | 0u;
@@ -197,9 +197,7 @@ constexpr uint32_t kRefitChunk = kThreads * kRefitPerThread;  // leaves per work
 // number of live (not deleted) leaves below it.
 __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ leaf_boxes, uint32_t n,
                                                     const uint32_t* __restrict__ parent,
-                                                    const uint4* __restrict__ meta,
-                                                    float4* child_bounds, uint32_t* child_live,
-                                                    float4* __restrict__ root_box,
+                                                    NodeRec* nodes, float4* __restrict__ root_box,
                                                     uint32_t* __restrict__ root_live,
                                                     uint32_t* flags, uint32_t* error) {
   const uint32_t c0 = blockIdx.x * kRefitChunk;
@@ -229,17 +227,18 @@ __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ l
         atomicOr(error, 2u);
         break;
       }
-      child_bounds[2 * p + side] = b;
-      child_live[2 * p + side] = live;
-      const uint4 mp = meta[p];
+      NodeRec* np = nodes + p;
+      np->box[side] = b;
+      if (side) np->live.y = live; else np->live.x = live;
+      const uint4 mp = np->meta;
       const bool local = mp.z >= c0 && mp.w <= c1;
       // Release our child's bounds; acquire the sibling's if we arrive second.
       const uint32_t old =
           local ? __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP)
                 : __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
       if (old == 0) break;
-      const float4 o = child_bounds[2 * p + (side ^ 1u)];
-      live += child_live[2 * p + (side ^ 1u)];
+      const float4 o = np->box[side ^ 1u];
+      live += side ? np->live.x : np->live.y;
       b = unite(b, o);
       if (p == 0) {
         root_box[0] = b;
@@ -259,12 +258,12 @@ __global__ void __launch_bounds__(kThreads) k_clear_flags(uint32_t* flags, uint3
 // Sum of half-perimeters of all nodes below the root, divided by the root's:
 // the expected number of node visits of small random range queries, up to a
 // constant. Non-finite (empty) boxes contribute nothing.
-__global__ void __launch_bounds__(kThreads) k_tree_cost(const float4* __restrict__ cb, uint32_t m,
+__global__ void __launch_bounds__(kThreads) k_tree_cost(const NodeRec* __restrict__ nodes, uint32_t m,
                                                         double* __restrict__ out) {
   __shared__ double sh[kThreads / kWave];
   double acc = 0.0;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads) {
-    float4 b = cb[i];
+    float4 b = nodes[i >> 1].box[i & 1];
     double hp = (double)(b.z - b.x) + (double)(b.w - b.y);
     if (hp >= 0.0 && hp < 1e300) acc += hp;
   }
@@ -359,11 +358,11 @@ hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids,
   return hipGetLastError();
 }
 
-hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, uint4* meta, uint32_t* parent,
+hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes, uint32_t* parent,
                          uint32_t* flags, hipStream_t stream) {
   if (n < 2) return hipSuccess;
   hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, sorted_keys, n,
-                     meta, parent, flags);
+                     nodes, parent, flags);
   return hipGetLastError();
 }
 
@@ -375,18 +374,17 @@ hipError_t launch_refit(const RefitArgs& a, hipStream_t stream) {
                        n - 1);
   const uint32_t blocks = (n + kRefitChunk - 1) / kRefitChunk;
   hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(kThreads), 0, stream, a.leaf_boxes, n, a.parent,
-                     a.meta, a.child_bounds, a.child_live, a.root_box, a.root_live, a.flags,
-                     a.error);
+                     a.nodes, a.root_box, a.root_live, a.flags, a.error);
   return hipGetLastError();
 }
 
-hipError_t launch_tree_cost(const float4* child_bounds, const float4* root_box, uint32_t n,
+hipError_t launch_tree_cost(const NodeRec* nodes, const float4* root_box, uint32_t n,
                             double* out, hipStream_t stream) {
   hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(out), 0u, 4, stream);
   if (e != hipSuccess) return e;
   if (n > 1)
     hipLaunchKernelGGL(k_tree_cost, dim3(grid_for(2 * (uint64_t)(n - 1), 2048)), dim3(kThreads), 0,
-                       stream, child_bounds, 2 * (n - 1), out);
+                       stream, nodes, 2 * (n - 1), out);
   hipLaunchKernelGGL(k_cost_finish, dim3(1), dim3(1), 0, stream, root_box, out);
   return hipGetLastError();
 }

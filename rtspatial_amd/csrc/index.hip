@@ -93,9 +93,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   dev_free(ix->keys);
   dev_free(ix->keys_alt);
   dev_free(ix->vals_alt);
-  dev_free(ix->meta);
-  dev_free(ix->child_bounds);
-  dev_free(ix->child_live);
+  dev_free(ix->nodes);
   dev_free(ix->parent);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);
@@ -105,9 +103,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   RTS_HIP(dev_alloc(&ix->keys, cap));
   RTS_HIP(dev_alloc(&ix->keys_alt, cap));
   RTS_HIP(dev_alloc(&ix->vals_alt, cap));
-  RTS_HIP(dev_alloc(&ix->meta, cap));
-  RTS_HIP(dev_alloc(&ix->child_bounds, 2 * cap));
-  RTS_HIP(dev_alloc(&ix->child_live, 2 * cap));
+  RTS_HIP(dev_alloc(&ix->nodes, cap));
   RTS_HIP(dev_alloc(&ix->parent, 2 * cap));
   RTS_HIP(dev_alloc(&ix->flags, cap));
   RTS_HIP(dev_alloc(&ix->sort_scratch, sort_scratch_bytes(cap)));
@@ -142,7 +138,7 @@ int measure_cost(rts_index* ix, bool after_build, hipStream_t stream) {
   } else if (ix->cost_refit_pending || ix->cost_build_pending) {
     return RTS_OK;  // a measurement is still in flight; skip this one
   }
-  RTS_HIP(launch_tree_cost(ix->child_bounds, ix->root_box, (uint32_t)ix->n_tree, ix->cost, stream));
+  RTS_HIP(launch_tree_cost(ix->nodes, ix->root_box, (uint32_t)ix->n_tree, ix->cost, stream));
   double* dst = after_build ? ix->h_cost : ix->h_cost + 2;
   RTS_HIP(hipMemcpyAsync(dst, ix->cost, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
   if (after_build) {
@@ -161,9 +157,7 @@ RefitArgs refit_args(rts_index* ix, uint32_t n) {
   a.leaf_boxes = ix->leaf_boxes;
   a.n = n;
   a.parent = ix->parent;
-  a.meta = ix->meta;
-  a.child_bounds = ix->child_bounds;
-  a.child_live = ix->child_live;
+  a.nodes = ix->nodes;
   a.root_box = ix->root_box;
   a.root_live = ix->root_live;
   a.flags = ix->flags;
@@ -197,7 +191,7 @@ int build_tree(rts_index* ix, hipStream_t stream) {
   }
   ph = phase_begin(RTS_PHASE_BUILD_TREE, stream);
   RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids, n, ix->leaf_boxes, ix->leaf_of_id, stream));
-  RTS_HIP(launch_karras(ix->keys, n, ix->meta, ix->parent, ix->flags, stream));
+  RTS_HIP(launch_karras(ix->keys, n, ix->nodes, ix->parent, ix->flags, stream));
   RTS_HIP(launch_refit(refit_args(ix, n), stream));
   phase_end(ph, stream);
   ix->n_tree = N;
@@ -247,9 +241,7 @@ TreeView tree_view(const rts_index* ix) {
   TreeView t;
   t.leaf_boxes = ix->leaf_boxes;
   t.leaf_ids = ix->leaf_ids;
-  t.meta = ix->meta;
-  t.child_bounds = ix->child_bounds;
-  t.child_live = ix->child_live;
+  t.nodes = ix->nodes;
   t.root_box = ix->root_box;
   t.root_live = ix->root_live;
   t.n = (uint32_t)ix->n_tree;
@@ -336,9 +328,7 @@ int rts_index_destroy(rts_index* ix) {
   dev_free(ix->keys);
   dev_free(ix->keys_alt);
   dev_free(ix->vals_alt);
-  dev_free(ix->meta);
-  dev_free(ix->child_bounds);
-  dev_free(ix->child_live);
+  dev_free(ix->nodes);
   dev_free(ix->parent);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);

@@ -29,9 +29,7 @@ struct rts_index {
   uint32_t* keys = nullptr;
   uint32_t* keys_alt = nullptr;
   uint32_t* vals_alt = nullptr;
-  uint4* meta = nullptr;
-  float4* child_bounds = nullptr;
-  uint32_t* child_live = nullptr;
+  rts::NodeRec* nodes = nullptr;
   uint32_t* parent = nullptr;
   uint32_t* flags = nullptr;
   char* sort_scratch = nullptr;

@@ -154,16 +154,17 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
           have = true;
         } else if (e & kRangeFlag) {
           if ((e & kIndexMask) < t.n - 1) {
-            const uint4 m = t.meta[e & kIndexMask];
+            const uint4 m = t.nodes[e & kIndexMask].meta;
             prim = m.z;
             cur = m.z + 1;
             end = m.w < t.n ? m.w : t.n - 1;
             have = true;
           }
         } else if (e < t.n - 1) {
-          const uint4 m = t.meta[e];
-          const float4 lb = t.child_bounds[2 * e];
-          const float4 rbx = t.child_bounds[2 * e + 1];
+          const NodeRec* nd = t.nodes + e;
+          const uint4 m = nd->meta;
+          const float4 lb = nd->box[0];
+          const float4 rbx = nd->box[1];
           const uint32_t gamma = m.x & ~kLeafFlag;
           // right child first so that the left one is expanded next
           if (P::node(rbx, qb)) {
@@ -239,6 +240,8 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
 constexpr int kWaveStack = 1024;    // LDS stack entries per wave
 constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
 constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
+constexpr int kScanUnroll = 4;      // leaves per lane per scan step
+constexpr uint32_t kScanStep = kScanUnroll * kWave;
 enum { kWaveCount = 0, kWaveCountSink = 1, kWaveWrite = 2 };
 
 // Workgroups are dealt to the 8 XCDs round-robin; remapping the launch index
@@ -330,8 +333,11 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
       uint32_t push0 = 0, push1 = 0, np = 0;
       uint32_t rf0 = 0, rn0 = 0, rf1 = 0, rn1 = 0, nr = 0;
       if (act && e < t.n - 1) {
-        const uint4 mt = t.meta[e];
-        const float4 cb[2] = {t.child_bounds[2 * e], t.child_bounds[2 * e + 1]};
+        const NodeRec* nd = t.nodes + e;
+        const uint4 mt = nd->meta;
+        const uint4 lv = nd->live;
+        const float4 cb[2] = {nd->box[0], nd->box[1]};
+        const uint32_t live[2] = {lv.x, lv.y};
         const uint32_t gamma = mt.x & ~kLeafFlag;
         const uint32_t ref[2] = {mt.x, mt.y};
         const uint32_t first[2] = {mt.z, gamma + 1};
@@ -341,10 +347,10 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
           if (!P::node(cb[side], qb)) continue;
           const bool inside = P::inside(cb[side], qb);
           if (kCounting && inside) {
-            inside_live += t.child_live[2 * e + side];
+            inside_live += live[side];
           } else if ((ref[side] & kLeafFlag) || cnt[side] <= t.leaf_size || degrade || inside) {
             uint32_t f = first[side];
-            if (inside && t.child_live[2 * e + side] == cnt[side]) f |= kAllMatch;
+            if (inside && live[side] == cnt[side]) f |= kAllMatch;
             if (nr == 0) {
               rf0 = f;
               rn0 = cnt[side];
@@ -385,41 +391,60 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
       if (lane == 0) rpre[m] = total;
       lds_fence();
     } else {
-      // ---- scan the queued leaf ranges, 64 leaves per step
+      // ---- scan the queued leaf ranges, kScanUnroll x 64 leaves per step:
+      // all loads of a step are issued before any result is written.
       uint32_t e0 = 0;
-      for (uint32_t base = 0; base < total; base += kWave) {
-        const uint32_t j = base + lane;
-        bool hit = false;
-        uint32_t leaf = 0, lo = m - 1;
-        if (j < total) {
-          lo = e0;
-          if (rpre[e0 + 1] < base + kWave) {  // the step spans more than one range
-            uint32_t hi = m - 1;
-            while (lo < hi) {
-              const uint32_t mid = (lo + hi + 1) >> 1;
-              if (rpre[mid] <= j) lo = mid; else hi = mid - 1;
+      for (uint32_t base = 0; base < total; base += kScanStep) {
+        uint32_t leaf[kScanUnroll];
+        bool hit[kScanUnroll];
+        // Fast path: the whole step lies in entry e0.
+        const bool one_range = rpre[e0 + 1] >= base + kScanStep;
+        uint32_t lo = e0;
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; u++) {
+          const uint32_t j = base + u * kWave + lane;
+          hit[u] = false;
+          leaf[u] = 0;
+          if (j < total) {
+            if (!one_range) {
+              uint32_t hi = m - 1;
+              while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (rpre[mid] <= j) lo = mid; else hi = mid - 1;
+              }
             }
-          }
-          const uint32_t f = rfirst[lo];
-          leaf = (f & ~kAllMatch) + (j - rpre[lo]);
-          if (f & kAllMatch) {
-            hit = true;
-          } else if (leaf < t.n) {
-            hit = P::leaf(t.leaf_boxes[leaf], qb);
+            const uint32_t f = rfirst[lo];
+            leaf[u] = (f & ~kAllMatch) + (j - rpre[lo]);
+            hit[u] = (f & kAllMatch) != 0;
+            if (!hit[u] && leaf[u] < t.n) hit[u] = P::leaf(t.leaf_boxes[leaf[u]], qb);
+          } else {
+            lo = m - 1;
           }
         }
         e0 = readlane_u32(lo, kWave - 1);
-        const uint64_t mask = ballot(hit);
-        if (MODE == kWaveWrite && hit) {
-          const uint64_t slot = obase + emitted + mask_rank(mask);
-          if (slot < cap && leaf < t.n) {
-            rts_pair pr;
-            pr.geom_id = t.leaf_ids[leaf];
-            pr.query_id = qid;
-            out[slot] = pr;
+        if (MODE == kWaveWrite) {
+          uint32_t gid[kScanUnroll];
+#pragma unroll
+          for (int u = 0; u < kScanUnroll; u++)
+            gid[u] = (hit[u] && leaf[u] < t.n) ? t.leaf_ids[leaf[u]] : 0u;
+#pragma unroll
+          for (int u = 0; u < kScanUnroll; u++) {
+            const uint64_t mask = ballot(hit[u]);
+            if (hit[u]) {
+              const uint64_t slot = obase + emitted + mask_rank(mask);
+              if (slot < cap) {
+                rts_pair pr;
+                pr.geom_id = gid[u];
+                pr.query_id = qid;
+                out[slot] = pr;
+              }
+            }
+            emitted += (uint64_t)__popcll(mask);
           }
+        } else {
+#pragma unroll
+          for (int u = 0; u < kScanUnroll; u++) emitted += (uint64_t)__popcll(ballot(hit[u]));
         }
-        emitted += (uint64_t)__popcll(mask);
       }
       m = 0;
       total = 0;

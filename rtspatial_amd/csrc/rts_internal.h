@@ -21,24 +21,30 @@ constexpr int kMortonBits = 30;           // 15 bits per axis
 constexpr int kStackDepth = 72;           // >= max Karras depth (64) + 2
 
 // --------------------------------------------------------------- layout ----
-// Karras LBVH over n sorted leaves: internal nodes 0..n-2 (root 0).
-//   meta[i]          = {left ref, right ref, first leaf, last leaf}
-//                      ref = internal index, or sorted leaf position | kLeafFlag
-//   child_bounds[2i] = bounds of the left child, [2i+1] of the right child
-//   parent[j]        = (parent << 1) | side, for internal j in [0, n-1) and
-//                      leaf k at [n-1+k]
-// Leaves are the boxes in Morton order: leaf_boxes[k], with id leaf_ids[k].
+// Karras LBVH over n sorted leaves: internal nodes 0..n-2 (root 0), one
+// 64-byte record each, so expanding a node is one cache-line fetch:
+//   box[0], box[1]  bounds of the left / right child
+//   meta            {left ref, right ref, first leaf, last leaf}
+//                   ref = internal index, or sorted leaf position | kLeafFlag
+//   live            {live leaves below the left child, ... the right child}
+// parent[j] = (parent << 1) | side, for internal j in [0, n-1) and leaf k at
+// [n-1+k]. Leaves are the boxes in Morton order: leaf_boxes[k], id leaf_ids[k].
+struct __align__(64) NodeRec {
+  float4 box[2];
+  uint4 meta;
+  uint4 live;
+};
+static_assert(sizeof(NodeRec) == 64, "node record is one 64-byte line");
+
 struct TreeView {
   const float4* leaf_boxes;
   const uint32_t* leaf_ids;
-  const uint4* meta;
-  const float4* child_bounds;
-  const uint32_t* child_live;  // live leaves below child slot [2i + side]
-  const float4* root_box;   // device scalar
+  const NodeRec* nodes;
+  const float4* root_box;     // device scalar
   const uint32_t* root_live;  // device scalar: live leaves in the tree
-  uint32_t n;               // leaves
-  uint32_t leaf_size;       // ranges with <= leaf_size leaves are scanned directly
-  uint32_t* error;          // device word: bit 1 refit walk, bit 2 traversal bound
+  uint32_t n;                 // leaves
+  uint32_t leaf_size;         // ranges with <= leaf_size leaves are scanned directly
+  uint32_t* error;            // device word: bit 1 refit walk, bit 2 traversal bound
 };
 
 // Device error bits (the index's error word).
@@ -99,22 +105,20 @@ hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* sce
                                 uint32_t* keys, hipStream_t stream);
 hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids, uint32_t n,
                                 float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream);
-hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, uint4* meta,
+hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes,
                          uint32_t* parent, uint32_t* flags, hipStream_t stream);
 struct RefitArgs {
   const float4* leaf_boxes;
   uint32_t n;
   const uint32_t* parent;
-  const uint4* meta;
-  float4* child_bounds;
-  uint32_t* child_live;
+  NodeRec* nodes;
   float4* root_box;
   uint32_t* root_live;
   uint32_t* flags;
   uint32_t* error;
 };
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
-hipError_t launch_tree_cost(const float4* child_bounds, const float4* root_box, uint32_t n,
+hipError_t launch_tree_cost(const NodeRec* nodes, const float4* root_box, uint32_t n,
                             double* out, hipStream_t stream);
 hipError_t launch_delete(const uint64_t* ids, uint64_t n_ids, float4* boxes, uint64_t n_boxes,
                          float4* leaf_boxes, const uint32_t* leaf_of_id, uint64_t n_tree,
