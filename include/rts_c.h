@@ -204,6 +204,91 @@ int rts_index_get_stats(rts_index* index, rts_index_stats* out);
 int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                    uint32_t* vals_out, uint64_t n, int key_bits, rts_stream_t stream);
 
+/* ------------------------------------------------------------------------
+ * Served / batched execution (no reference counterpart: the reference answers
+ * one Query call at a time, spatial_index.cuh:352-515).
+ *
+ * rts_serve_run drives one request trace against a resident index and reports
+ * each request's admission and completion time. Requests become eligible at
+ * their arrival time (seconds from the start of the run); update batches are
+ * applied at their arrival time, and every request is answered on the index
+ * state of its admission (its epoch = update batches applied before it).
+ *
+ *   RTS_SERVE_BATCHED: continuous batching. Up to `slots` batches are in flight
+ *     on separate streams; whenever a slot is free, every arrived request
+ *     (split into chunks when large) is admitted while its predicted result
+ *     bytes fit the slot's arena, and the batch runs as one persistent
+ *     work-stealing launch sequence (count, scan, write), replayed from a
+ *     hipGraph per slot. Chunks whose results overflow the arena are
+ *     re-launched.
+ *   RTS_SERVE_PLAIN: the per-request path: each request (split the same way)
+ *     is one rts_index_query_* call on one stream, in arrival order.
+ * ------------------------------------------------------------------------ */
+typedef enum rts_serve_mode {
+  RTS_SERVE_BATCHED = 0,
+  RTS_SERVE_PLAIN = 1
+} rts_serve_mode;
+
+typedef struct rts_request {
+  double arrival_s;      /* eligible from this time (seconds after the run starts) */
+  const void* queries;   /* device (or pinned) rts_point or rts_envelope array */
+  uint64_t n;            /* queries */
+  int32_t points;        /* 1 = rts_point queries (predicate must be RTS_CONTAINS) */
+  int32_t predicate;     /* rts_predicate */
+  rts_pair* out;         /* optional device buffer for the request's pairs, (geom_id,
+                            query index within the request); NULL = results are
+                            dropped once complete */
+  uint64_t out_capacity; /* pairs `out` holds */
+} rts_request;
+
+typedef struct rts_update_batch {
+  double arrival_s;
+  const rts_update* updates; /* device (or pinned) */
+  uint64_t n;
+} rts_update_batch;
+
+typedef struct rts_request_result {
+  double admit_s;        /* first launch of the request */
+  double done_s;         /* all of its results complete */
+  uint64_t count;        /* results */
+  uint64_t hash;         /* with hash_results: sum of splitmix64((q << 32) | geom_id) */
+  uint32_t epoch;        /* update batches applied before the request ran */
+  uint32_t relaunches;   /* chunk re-launches after an arena overflow */
+  int32_t truncated;     /* `out` was too small for the results */
+  int32_t pad;
+} rts_request_result;
+
+typedef struct rts_serve_config {
+  uint32_t mode;               /* rts_serve_mode */
+  uint32_t slots;              /* batches in flight (batched mode) (3) */
+  uint32_t max_batch_requests; /* request chunks per batch (4096) */
+  uint32_t use_graphs;         /* replay each slot's launches as a hipGraph (1) */
+  uint64_t max_batch_queries;  /* queries per batch (4M) */
+  uint64_t chunk_queries;      /* largest chunk of one request (256K) */
+  uint64_t arena_pairs;        /* result arena per slot (or of the plain path), pairs */
+  uint32_t hash_results;       /* compute each request's pair hash (verification runs) */
+  uint32_t traversal;          /* rts_traversal for batched tiles: AUTO picks by predicate */
+} rts_serve_config;
+
+typedef struct rts_serve_stats {
+  double wall_s;          /* first arrival to last completion */
+  uint64_t requests;
+  uint64_t queries;
+  uint64_t results;
+  uint64_t batches;       /* launches (batches, or plain chunk queries) */
+  uint64_t relaunches;
+  uint64_t updates;
+} rts_serve_stats;
+
+void rts_serve_config_default(rts_serve_config* cfg);
+
+/* Runs the trace to completion (synchronous). `results` has n_requests
+ * entries; `stats` may be NULL. Requests and updates must be sorted by
+ * arrival time. */
+int rts_serve_run(rts_index* index, const rts_serve_config* cfg, const rts_request* requests,
+                  uint64_t n_requests, const rts_update_batch* updates, uint64_t n_updates,
+                  rts_request_result* results, rts_serve_stats* stats);
+
 /* Per-phase GPU timing: HIP events recorded on the caller's stream around
  * each phase's kernels. No reference counterpart (the reference's
  * IntersectsWhatQueryProfiling, spatial_index.cuh:566-683, times with host

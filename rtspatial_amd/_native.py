@@ -58,6 +58,63 @@ class rts_index_stats(ctypes.Structure):
     ]
 
 
+class rts_request(ctypes.Structure):
+    _fields_ = [
+        ("arrival_s", ctypes.c_double),
+        ("queries", ctypes.c_void_p),
+        ("n", ctypes.c_uint64),
+        ("points", ctypes.c_int32),
+        ("predicate", ctypes.c_int32),
+        ("out", ctypes.c_void_p),
+        ("out_capacity", ctypes.c_uint64),
+    ]
+
+
+class rts_update_batch(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("updates", ctypes.c_void_p), ("n", ctypes.c_uint64)]
+
+
+class rts_request_result(ctypes.Structure):
+    _fields_ = [
+        ("admit_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("count", ctypes.c_uint64),
+        ("hash", ctypes.c_uint64),
+        ("epoch", ctypes.c_uint32),
+        ("relaunches", ctypes.c_uint32),
+        ("truncated", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+class rts_serve_config(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_uint32),
+        ("slots", ctypes.c_uint32),
+        ("max_batch_requests", ctypes.c_uint32),
+        ("use_graphs", ctypes.c_uint32),
+        ("max_batch_queries", ctypes.c_uint64),
+        ("chunk_queries", ctypes.c_uint64),
+        ("arena_pairs", ctypes.c_uint64),
+        ("hash_results", ctypes.c_uint32),
+        ("traversal", ctypes.c_uint32),
+    ]
+
+
+class rts_serve_stats(ctypes.Structure):
+    _fields_ = [
+        ("wall_s", ctypes.c_double),
+        ("requests", ctypes.c_uint64),
+        ("queries", ctypes.c_uint64),
+        ("results", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("relaunches", ctypes.c_uint64),
+        ("updates", ctypes.c_uint64),
+    ]
+
+
+RTS_SERVE_BATCHED, RTS_SERVE_PLAIN = 0, 1
+
 RTS_PHASES = ["build_morton", "build_sort", "build_tree", "refit", "mutate", "query_order",
               "traverse", "query_count"]
 
@@ -83,6 +140,10 @@ SIGNATURES = [
     ("rts_index_sync", _I32, [_VP, _VP]),
     ("rts_index_get_stats", _I32, [_VP, ctypes.POINTER(rts_index_stats)]),
     ("rts_sort_pairs", _I32, [_VP, _VP, _VP, _VP, _U64, _I32, _VP]),
+    ("rts_serve_config_default", None, [ctypes.POINTER(rts_serve_config)]),
+    ("rts_serve_run", _I32, [_VP, ctypes.POINTER(rts_serve_config), ctypes.POINTER(rts_request),
+                             _U64, ctypes.POINTER(rts_update_batch), _U64,
+                             ctypes.POINTER(rts_request_result), ctypes.POINTER(rts_serve_stats)]),
     ("rts_profile_enable", None, [_I32]),
     ("rts_profile_read", _I32, [ctypes.POINTER(rts_profile), _I32]),
     ("rts_last_error", ctypes.c_char_p, []),

@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librtspatial_amd.so")
-SOURCES = ["sort.hip", "build.hip", "query.hip", "scan.hip", "index.hip", "profile.hip"]
+SOURCES = ["sort.hip", "build.hip", "query.hip", "scan.hip", "index.hip", "profile.hip",
+           "batch.hip", "server.hip"]
 ARCH = os.environ.get("RTS_OFFLOAD_ARCH", "gfx950")
 
 

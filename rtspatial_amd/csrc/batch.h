@@ -1,0 +1,60 @@
+// Batch descriptors shared by the batched kernels (batch.hip) and the host
+// engine that builds and launches batches (server.hip).
+#pragma once
+
+#include "rts_internal.h"
+
+namespace rts {
+
+// BatchRequest.flags
+constexpr uint32_t kReqPoints = 1u;       // queries are points (float2)
+constexpr uint32_t kReqPredShift = 1;     // bits 1-2: rts_predicate
+constexpr uint32_t kReqWaveTiles = 8u;    // one query per wavefront (else 64 per tile)
+
+struct BatchRequest {
+  const void* queries;  // device (or pinned host) queries of the request
+  uint32_t n;           // queries
+  uint32_t flags;
+  uint64_t qbase;       // first global query index of the request in the batch
+  uint32_t tbase;       // first tile of the request
+  uint32_t qid_base;    // added to query indices when reporting pairs
+};
+static_assert(sizeof(BatchRequest) == 32, "32-byte request descriptor");
+
+struct BatchHeader {
+  uint32_t n_requests;
+  uint32_t n_tiles;
+  uint64_t n_queries;
+};
+
+struct BatchArgs {
+  const BatchHeader* header;     // device
+  const BatchRequest* requests;  // device, sorted by tbase
+  uint32_t* tile_counters;       // 16 words: per-XCD counters of the count and write passes
+  uint32_t* counts;              // per global query
+  uint64_t* offsets;             // per global query: exclusive scan of counts
+  uint64_t* seg_sums;            // scan scratch (batch_seg_sums_words())
+  uint64_t* total;               // [0] results in the batch
+  uint64_t* ranges;              // per request: {first arena slot, results}
+  rts_pair* arena;               // results
+  uint64_t arena_cap;            // pairs the arena holds
+};
+
+// Count pass, scan, per-request ranges and write pass of one batch, on
+// `stream` (fixed grids: capturable in a graph and replayable for any batch
+// whose sizes fit the buffers). `t` is a device copy of the tree view.
+hipError_t launch_batch(const TreeView* t, const BatchArgs& a, hipStream_t stream);
+// hashes[c] = sum of splitmix64((query_id << 32) | geom_id) over range c of
+// the arena (ranges as written by the batch: {first slot, count}, in order).
+hipError_t launch_hash_ranges(const rts_pair* arena, uint64_t cap, const uint64_t* ranges,
+                              const uint32_t* n_ranges, const uint64_t* total,
+                              unsigned long long* hashes, hipStream_t stream);
+// Adds `add` to the query ids of the pairs in arena[span[0], span[1]) and, if
+// do_hash, adds their hash to *hash.
+hipError_t launch_rebase_hash_span(rts_pair* arena, uint64_t cap, const uint64_t* span,
+                                   uint32_t add, int do_hash, unsigned long long* hash,
+                                   hipStream_t stream);
+uint32_t batch_persistent_blocks();
+size_t batch_seg_sums_words();
+
+}  // namespace rts

@@ -1,0 +1,321 @@
+// Persistent work-stealing kernels that answer many uneven query requests in
+// one launch.
+//
+// A batch is a list of requests (queries pointer, count, point/envelope,
+// predicate, tile mode) laid out in one query space: request r owns global
+// queries [qbase_r, qbase_r + n_r) and tiles [tbase_r, tbase_r + tiles_r).
+// A tile is either 64 queries of one request answered one per lane (lane
+// tiles: points and data-contains-query, few results each) or one query
+// answered by a whole wavefront (wave tiles: intersects / within). Waves take
+// tiles from per-XCD counters -- each XCD first drains its own contiguous
+// eighth of the tiles, which keeps neighbouring queries on one L2 -- and then
+// steal from the other XCDs.
+//
+// Two passes: the count pass writes each query's result count, a scan turns
+// the counts into exclusive offsets into the slot's result arena, and the
+// write pass writes every pair at its offset. Every launch has a fixed grid
+// and reads the batch's sizes from device memory, so one captured hipGraph
+// replays any batch.
+#include "batch.h"
+#include "traverse.h"
+#include "wave.h"
+
+namespace rts {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t find_request(const BatchRequest* reqs, uint32_t nreq,
+                                                 uint32_t tile) {
+  uint32_t lo = 0, hi = nreq - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (reqs[mid].tbase <= tile) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Next tile for this wave, or >= n_tiles when all are taken.
+__device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, uint32_t n_tiles, uint32_t xcd) {
+  const uint32_t per = (n_tiles + kXcds - 1) / kXcds;
+  uint32_t got = 0xFFFFFFFFu;
+  if (lane_id() == 0) {
+    for (uint32_t k = 0; k < kXcds; k++) {
+      const uint32_t x = (xcd + k) % kXcds;
+      const uint32_t begin = x * per;
+      const uint32_t end = begin + per < n_tiles ? begin + per : n_tiles;
+      if (begin >= end) continue;
+      if (__hip_atomic_load(&counters[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= end - begin)
+        continue;
+      const uint32_t t = atomicAdd(&counters[x], 1u);
+      if (t < end - begin) {
+        got = begin + t;
+        break;
+      }
+    }
+  }
+  return readlane_u32(got, 0);
+}
+
+template <int PRED, bool POINTS, bool WRITE>
+__device__ void run_lane_tile(const TreeView& t, const BatchRequest& r, uint32_t q0,
+                              const BatchArgs& a) {
+  const uint32_t q = q0 + lane_id();
+  if (q >= r.n) return;
+  const float4 qb = load_query(r.queries, POINTS, q);
+  const uint64_t g = r.qbase + q;
+  if (WRITE) {
+    lane_query<PRED, POINTS, true>(t, qb, q + r.qid_base, a.arena, a.arena_cap, a.offsets[g]);
+  } else {
+    a.counts[g] = (uint32_t)lane_query<PRED, POINTS, false>(t, qb, q, nullptr, 0, 0);
+  }
+}
+
+template <int PRED, bool POINTS, bool WRITE>
+__device__ void run_wave_tile(const TreeView& t, const BatchRequest& r, uint32_t q,
+                              const BatchArgs& a, WaveLds& L) {
+  const float4 qb = load_query(r.queries, POINTS, q);
+  const uint64_t g = r.qbase + q;
+  if (WRITE) {
+    wave_query<PRED, POINTS, true>(t, qb, q + r.qid_base, a.arena, a.arena_cap, a.offsets[g], L);
+  } else {
+    const uint64_t c = wave_query<PRED, POINTS, false>(t, qb, q, nullptr, 0, 0, L);
+    if (lane_id() == 0) a.counts[g] = (uint32_t)c;
+  }
+}
+
+// The tree view is read from device memory so that a captured graph stays
+// valid when the index is rebuilt between batches.
+template <bool WRITE>
+__global__ void __launch_bounds__(kThreads) k_batch(const TreeView* tv, const BatchArgs a) {
+  __shared__ WaveLds s_lds[kWaves];
+  const TreeView t = *tv;
+  const uint32_t w = threadIdx.x / kWave;
+  const BatchHeader h = *a.header;
+  if (h.n_tiles == 0 || h.n_requests == 0 || t.n == 0) return;
+  uint32_t* counters = a.tile_counters + (WRITE ? kXcds : 0);
+  const uint32_t xcd = blockIdx.x % kXcds;
+  for (;;) {
+    const uint32_t tile = grab_tile(counters, h.n_tiles, xcd);
+    if (tile >= h.n_tiles) break;
+    const uint32_t ri = find_request(a.requests, h.n_requests, tile);
+    BatchRequest r = a.requests[ri];
+    const uint32_t local = tile - r.tbase;
+    const bool points = r.flags & kReqPoints;
+    const int pred = (r.flags >> kReqPredShift) & 3;
+    if (r.flags & kReqWaveTiles) {
+      if (local >= r.n) continue;
+      if (points) {
+        run_wave_tile<RTS_CONTAINS, true, WRITE>(t, r, local, a, s_lds[w]);
+      } else if (pred == RTS_INTERSECTS) {
+        run_wave_tile<RTS_INTERSECTS, false, WRITE>(t, r, local, a, s_lds[w]);
+      } else if (pred == RTS_WITHIN) {
+        run_wave_tile<RTS_WITHIN, false, WRITE>(t, r, local, a, s_lds[w]);
+      } else {
+        run_wave_tile<RTS_CONTAINS, false, WRITE>(t, r, local, a, s_lds[w]);
+      }
+    } else {
+      const uint32_t q0 = local * kWave;
+      if (points) {
+        run_lane_tile<RTS_CONTAINS, true, WRITE>(t, r, q0, a);
+      } else if (pred == RTS_INTERSECTS) {
+        run_lane_tile<RTS_INTERSECTS, false, WRITE>(t, r, q0, a);
+      } else if (pred == RTS_WITHIN) {
+        run_lane_tile<RTS_WITHIN, false, WRITE>(t, r, q0, a);
+      } else {
+        run_lane_tile<RTS_CONTAINS, false, WRITE>(t, r, q0, a);
+      }
+    }
+  }
+}
+
+// ---- scan of the per-query counts over the batch's query space, with a
+// fixed grid: block b owns the contiguous segment [b*S, (b+1)*S).
+constexpr uint32_t kScanBlocks = 1024;
+
+__device__ __forceinline__ uint64_t block_scan_excl(uint64_t v, uint64_t* sh, uint64_t* total) {
+  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+  uint64_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = __shfl_up(inc, d, 64);
+    if (lane >= (uint32_t)d) inc += o;
+  }
+  if (lane == kWave - 1) sh[w] = inc;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int i = 0; i < kWaves; i++) {
+    if (i < (int)w) before += sh[i];
+    all += sh[i];
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_seg_sums(const BatchArgs a) {
+  __shared__ uint64_t sh[kWaves];
+  const uint64_t n = a.header->n_queries;
+  const uint64_t seg = (n + kScanBlocks - 1) / kScanBlocks;
+  const uint64_t b0 = blockIdx.x * seg, b1 = b0 + seg < n ? b0 + seg : n;
+  uint64_t s = 0;
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += kThreads) s += a.counts[i];
+  uint64_t all;
+  block_scan_excl(s, sh, &all);
+  if (threadIdx.x == 0) a.seg_sums[blockIdx.x] = all;
+}
+
+__global__ void __launch_bounds__(kThreads) k_seg_scan(const BatchArgs a) {
+  __shared__ uint64_t sh[kWaves];
+  uint64_t carry = 0;
+  for (uint32_t c = 0; c < kScanBlocks; c += kThreads) {
+    const uint32_t i = c + threadIdx.x;
+    const uint64_t v = a.seg_sums[i];
+    uint64_t all;
+    const uint64_t ex = block_scan_excl(v, sh, &all);
+    a.seg_sums[i] = carry + ex;
+    carry += all;
+  }
+  if (threadIdx.x == 0) a.total[0] = carry;
+}
+
+__global__ void __launch_bounds__(kThreads) k_seg_apply(const BatchArgs a) {
+  __shared__ uint64_t sh[kWaves];
+  const uint64_t n = a.header->n_queries;
+  const uint64_t seg = (n + kScanBlocks - 1) / kScanBlocks;
+  const uint64_t b0 = blockIdx.x * seg, b1 = b0 + seg < n ? b0 + seg : n;
+  uint64_t carry = a.seg_sums[blockIdx.x];
+  for (uint64_t c = b0; c < b1; c += kThreads) {
+    const uint64_t i = c + threadIdx.x;
+    const uint64_t v = i < b1 ? a.counts[i] : 0;
+    uint64_t all;
+    const uint64_t ex = block_scan_excl(v, sh, &all);
+    if (i < b1) a.offsets[i] = carry + ex;
+    carry += all;
+  }
+}
+
+// Per request: first arena slot and number of results.
+__global__ void __launch_bounds__(kThreads) k_request_ranges(const BatchArgs a) {
+  const BatchHeader h = *a.header;
+  for (uint32_t r = blockIdx.x * kThreads + threadIdx.x; r < h.n_requests;
+       r += gridDim.x * kThreads) {
+    const BatchRequest q = a.requests[r];
+    const uint64_t start = q.n ? a.offsets[q.qbase] : 0;
+    const uint64_t end = (q.qbase + q.n < h.n_queries) ? a.offsets[q.qbase + q.n] : a.total[0];
+    a.ranges[2 * r] = start;
+    a.ranges[2 * r + 1] = q.n ? end - start : 0;
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+}  // namespace
+
+uint32_t batch_persistent_blocks() {
+  static uint32_t blocks = 0;
+  if (blocks == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<true>, kThreads, 0);
+    if (cus <= 0) cus = 256;
+    if (per_cu <= 0) per_cu = 1;
+    blocks = (uint32_t)(cus * per_cu);
+    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+  }
+  return blocks;
+}
+
+hipError_t launch_batch(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
+  const uint32_t blocks = batch_persistent_blocks();
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.tile_counters, 2 * kXcds);
+  hipLaunchKernelGGL(k_batch<false>, dim3(blocks), dim3(kThreads), 0, stream, t, a);
+  hipLaunchKernelGGL(k_seg_sums, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_seg_apply, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_request_ranges, dim3(16), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_batch<true>, dim3(blocks), dim3(kThreads), 0, stream, t, a);
+  return hipGetLastError();
+}
+
+size_t batch_seg_sums_words() { return kScanBlocks; }
+
+namespace {
+
+__device__ __forceinline__ uint64_t splitmix(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// hashes[c] = sum over the pairs of range c (ranges[2c] = first slot,
+// ranges[2c+1] = count; ranges tile [0, *total) in order) of
+// splitmix64((query_id << 32) | geom_id). hashes must be zero.
+__global__ void __launch_bounds__(kThreads) k_hash_ranges(const rts_pair* __restrict__ arena,
+                                                          uint64_t cap,
+                                                          const uint64_t* __restrict__ ranges,
+                                                          const uint32_t* __restrict__ n_ranges,
+                                                          const uint64_t* __restrict__ total,
+                                                          unsigned long long* __restrict__ hashes) {
+  const uint32_t nr = *n_ranges;
+  const uint64_t hi = *total < cap ? *total : cap;
+  if (nr == 0) return;
+  for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < hi;
+       i += (uint64_t)gridDim.x * kThreads) {
+    uint32_t a = 0, b = nr - 1;
+    while (a < b) {
+      const uint32_t mid = (a + b + 1) >> 1;
+      if (ranges[2 * mid] <= i) a = mid; else b = mid - 1;
+    }
+    if (i >= ranges[2 * a] + ranges[2 * a + 1]) continue;
+    const rts_pair p = arena[i];
+    atomicAdd(&hashes[a], (unsigned long long)splitmix(((uint64_t)p.query_id << 32) | p.geom_id));
+  }
+}
+
+// Pairs in arena[span[0], span[1]) get query_id += add; hash[0] += their hash.
+__global__ void __launch_bounds__(kThreads) k_rebase_hash_span(rts_pair* __restrict__ arena,
+                                                               uint64_t cap,
+                                                               const uint64_t* __restrict__ span,
+                                                               uint32_t add, int do_hash,
+                                                               unsigned long long* __restrict__ hash) {
+  const uint64_t lo = span[0], hi = span[1] < cap ? span[1] : cap;
+  uint64_t h = 0;
+  for (uint64_t i = lo + blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < hi;
+       i += (uint64_t)gridDim.x * kThreads) {
+    rts_pair p = arena[i];
+    p.query_id += add;
+    if (add) arena[i] = p;
+    if (do_hash) h += splitmix(((uint64_t)p.query_id << 32) | p.geom_id);
+  }
+  if (do_hash) {
+    h = wave_sum_u64(h);
+    if (lane_id() == 0 && h) atomicAdd(hash, (unsigned long long)h);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_hash_ranges(const rts_pair* arena, uint64_t cap, const uint64_t* ranges,
+                              const uint32_t* n_ranges, const uint64_t* total,
+                              unsigned long long* hashes, hipStream_t stream) {
+  hipLaunchKernelGGL(k_hash_ranges, dim3(4096), dim3(kThreads), 0, stream, arena, cap, ranges,
+                     n_ranges, total, hashes);
+  return hipGetLastError();
+}
+
+hipError_t launch_rebase_hash_span(rts_pair* arena, uint64_t cap, const uint64_t* span,
+                                   uint32_t add, int do_hash, unsigned long long* hash,
+                                   hipStream_t stream) {
+  hipLaunchKernelGGL(k_rebase_hash_span, dim3(1024), dim3(kThreads), 0, stream, arena, cap, span,
+                     add, do_hash, hash);
+  return hipGetLastError();
+}
+
+}  // namespace rts

@@ -1,0 +1,694 @@
+// Served and plain execution of a request trace against a resident index
+// (rts_serve_run). No reference counterpart: the reference answers one
+// Query call at a time (spatial_index.cuh:352-515).
+//
+// Batched mode is continuous batching over `slots` streams. Each slot owns
+// device buffers for one batch (descriptors, per-query counts and offsets,
+// result arena) and a pinned staging area; its launch sequence (descriptor
+// upload, count pass, scan, request ranges, write pass, range download) is
+// captured once as a hipGraph and replayed for every batch, since every
+// kernel has a fixed grid and reads the batch's sizes from device memory.
+//
+// Admission: arrived requests are taken in arrival order and split into
+// chunks of at most chunk_queries queries; chunks join the batch while their
+// predicted results (an estimate of results per query for each predicate,
+// refreshed from every completed chunk) fit the slot's arena. A chunk whose
+// actual results overflow the arena is re-launched in a later batch.
+//
+// Updates: once an update batch has arrived, no new request is admitted; when
+// the requests already started are fully admitted and their batches are done
+// (none can need a re-launch any more), the update is applied on an update
+// stream and later batches wait for it. So each request runs entirely on the
+// index state of its admission (its epoch).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "batch.h"
+#include "index_state.h"
+#include "rts_internal.h"
+#include "traverse.h"
+#include "wave.h"
+
+namespace rts {
+namespace {
+
+#define SRV_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error(std::string("serve: ") + #call + ": " + hipGetErrorString(e_));    \
+      return RTS_ERR_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint64_t kUnknown = ~0ull;
+
+struct Chunk {
+  uint32_t req;
+  uint64_t q0, n;
+  uint64_t known;  // exact result count when known (a re-launch), else kUnknown
+};
+
+struct ReqState {
+  uint64_t next_q = 0;      // first query not yet admitted
+  uint64_t chunks_out = 0;  // admitted chunks not yet complete
+  uint64_t filled = 0;      // results copied to `out`
+  bool admitted = false;
+  bool done = false;
+};
+
+// Results-per-query estimate for one kind of request.
+struct Estimate {
+  double per_query = 1.0;
+  bool seeded = false;
+  void observe(uint64_t results, uint64_t queries) {
+    if (!queries) return;
+    const double v = (double)results / (double)queries;
+    per_query = seeded ? 0.75 * per_query + 0.25 * v : v;
+    seeded = true;
+  }
+};
+
+int kind_of(const rts_request& r) { return r.points ? 3 : (r.predicate & 3); }
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  hipGraphExec_t exec = nullptr;
+  char* h_desc = nullptr;                // pinned: [TreeView | BatchHeader | BatchRequest x max]
+  uint64_t* h_ranges = nullptr;          // pinned: 2 x max
+  unsigned long long* h_hash = nullptr;  // pinned: max
+  char* d_desc = nullptr;
+  uint32_t* d_counters = nullptr;
+  uint32_t* d_counts = nullptr;
+  uint64_t* d_offsets = nullptr;
+  uint64_t* d_seg = nullptr;
+  uint64_t* d_total = nullptr;
+  uint64_t* d_ranges = nullptr;
+  unsigned long long* d_hash = nullptr;
+  rts_pair* d_arena = nullptr;
+  std::vector<Chunk> chunks;
+  bool busy = false;
+};
+
+constexpr size_t kViewOff = 0, kHeaderOff = 128, kReqsOff = 256;
+static_assert(sizeof(TreeView) <= kHeaderOff, "tree view fits its slot");
+
+struct Engine {
+  rts_index* ix = nullptr;
+  rts_serve_config cfg{};
+  const rts_request* reqs = nullptr;
+  uint64_t n_reqs = 0;
+  const rts_update_batch* ups = nullptr;
+  uint64_t n_ups = 0;
+  rts_request_result* res = nullptr;
+  std::vector<ReqState> st;
+  Estimate est[4];
+  Clock::time_point t0;
+  size_t desc_bytes = 0;
+  uint32_t epoch = 0;
+  uint64_t batches = 0, relaunches = 0, applied_updates = 0, done = 0;
+  std::deque<uint32_t> queue;  // requests with queries not yet admitted, in arrival order
+  std::deque<Chunk> retry;     // overflowed chunks, with their exact counts
+  uint64_t partial = 0;        // requests admitted in part
+  bool hold_new = false;       // an update is waiting: admit no new request
+  hipStream_t upd_stream = nullptr;
+  hipEvent_t upd_event = nullptr;
+  bool have_upd_event = false;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+double predicted(const Engine& E, const Chunk& c) {
+  if (c.known != kUnknown) return (double)c.known;
+  return E.est[kind_of(E.reqs[c.req])].per_query * 1.5 * (double)c.n + 16.0;
+}
+
+// Fills `out` with re-launches first, then chunks of arrived requests, while
+// they fit: at most max_req chunks, max_q queries and `budget` predicted
+// results. With `force`, one chunk is taken even if it does not fit.
+void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget, bool force,
+           std::vector<Chunk>& out) {
+  out.clear();
+  uint64_t q_used = 0;
+  double used = 0;
+  while (!E.retry.empty() && out.size() < max_req) {
+    Chunk c = E.retry.front();
+    if (c.known != kUnknown && (double)c.known > (double)E.cfg.arena_pairs && c.n > 1) {
+      // It can never fit one arena: split it; the halves' counts are unknown.
+      E.retry.pop_front();
+      const uint64_t h = c.n / 2;
+      E.retry.push_front(Chunk{c.req, c.q0 + h, c.n - h, kUnknown});
+      E.retry.push_front(Chunk{c.req, c.q0, h, kUnknown});
+      E.st[c.req].chunks_out++;
+      continue;
+    }
+    const double p = predicted(E, c);
+    if ((used + p > budget || q_used + c.n > max_q) && !(force && out.empty())) return;
+    out.push_back(c);
+    E.retry.pop_front();
+    used += p;
+    q_used += c.n;
+  }
+  while (!E.queue.empty() && out.size() < max_req) {
+    const uint32_t ri = E.queue.front();
+    const rts_request& r = E.reqs[ri];
+    if (r.arrival_s > t) break;
+    ReqState& s = E.st[ri];
+    if (E.hold_new && !s.admitted) break;
+    const uint64_t left = r.n - s.next_q;
+    if (left == 0) {  // an empty request completes on admission
+      E.queue.pop_front();
+      if (!s.done) {
+        s.admitted = s.done = true;
+        E.res[ri].admit_s = E.res[ri].done_s = t;
+        E.res[ri].epoch = E.epoch;
+        E.done++;
+      }
+      continue;
+    }
+    const double per = E.est[kind_of(r)].per_query * 1.5;
+    uint64_t k = std::min<uint64_t>(left, E.cfg.chunk_queries);
+    k = std::min<uint64_t>(k, max_q > q_used ? max_q - q_used : 0);
+    const double room = budget - used - 16.0;
+    if (per > 0) k = std::min<uint64_t>(k, (uint64_t)std::max(0.0, room / per));
+    if (k == 0) {
+      if (force && out.empty()) k = 1;
+      else break;
+    }
+    const Chunk c{ri, s.next_q, k, kUnknown};
+    out.push_back(c);
+    if (!s.admitted) {
+      s.admitted = true;
+      E.res[ri].admit_s = t;
+      E.res[ri].epoch = E.epoch;
+      E.partial++;
+    }
+    s.next_q += k;
+    s.chunks_out++;
+    q_used += k;
+    used += predicted(E, c);
+    if (s.next_q >= r.n) {
+      E.queue.pop_front();
+      E.partial--;
+    }
+  }
+}
+
+// A chunk's results are final.
+void finish_chunk(Engine& E, const Chunk& c, uint64_t count, uint64_t hash, double t) {
+  ReqState& s = E.st[c.req];
+  rts_request_result& R = E.res[c.req];
+  R.count += count;
+  R.hash += hash;
+  E.est[kind_of(E.reqs[c.req])].observe(count, c.n);
+  if (--s.chunks_out == 0 && s.next_q >= E.reqs[c.req].n) {
+    s.done = true;
+    R.done_s = t;
+    E.done++;
+  }
+}
+
+void overflowed(Engine& E, const Chunk& c, uint64_t count) {
+  E.retry.push_back(Chunk{c.req, c.q0, c.n, count});
+  E.res[c.req].relaunches++;
+  E.relaunches++;
+  E.est[kind_of(E.reqs[c.req])].observe(count, c.n);
+}
+
+// Copies a finished chunk's pairs from the arena to the request's `out`.
+int copy_out(Engine& E, const Chunk& c, const rts_pair* src, uint64_t cnt, hipStream_t stream) {
+  const rts_request& r = E.reqs[c.req];
+  if (!r.out || !cnt) return RTS_OK;
+  ReqState& st = E.st[c.req];
+  uint64_t take = cnt;
+  if (st.filled + take > r.out_capacity) {
+    take = r.out_capacity > st.filled ? r.out_capacity - st.filled : 0;
+    E.res[c.req].truncated = 1;
+  }
+  if (take) {
+    SRV_HIP(hipMemcpyAsync(r.out + st.filled, src, take * sizeof(rts_pair), hipMemcpyDeviceToDevice,
+                           stream));
+    SRV_HIP(hipStreamSynchronize(stream));
+  }
+  st.filled += take;
+  return RTS_OK;
+}
+
+uint32_t tile_flags(const Engine& E, const rts_request& r) {
+  uint32_t f = (r.points ? kReqPoints : 0u) | ((uint32_t)(r.predicate & 3) << kReqPredShift);
+  const uint32_t trav = E.cfg.traversal;
+  const bool wave = trav == RTS_TRAVERSAL_WAVE ||
+                    (trav != RTS_TRAVERSAL_THREAD && !r.points && r.predicate != RTS_CONTAINS);
+  if (wave) f |= kReqWaveTiles;
+  return f;
+}
+
+BatchArgs slot_args(const Engine& E, const Slot& s) {
+  BatchArgs a;
+  a.header = reinterpret_cast<const BatchHeader*>(s.d_desc + kHeaderOff);
+  a.requests = reinterpret_cast<const BatchRequest*>(s.d_desc + kReqsOff);
+  a.tile_counters = s.d_counters;
+  a.counts = s.d_counts;
+  a.offsets = s.d_offsets;
+  a.seg_sums = s.d_seg;
+  a.total = s.d_total;
+  a.ranges = s.d_ranges;
+  a.arena = s.d_arena;
+  a.arena_cap = E.cfg.arena_pairs;
+  return a;
+}
+
+// One batch: descriptors up, count / scan / write, ranges (and hashes) down.
+int record_slot_work(Engine& E, Slot& s, size_t up_bytes, size_t n_req) {
+  SRV_HIP(hipMemcpyAsync(s.d_desc, s.h_desc, up_bytes, hipMemcpyHostToDevice, s.stream));
+  const BatchArgs a = slot_args(E, s);
+  SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, s.stream));
+  SRV_HIP(hipMemcpyAsync(s.h_ranges, s.d_ranges, 2 * n_req * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, s.stream));
+  if (E.cfg.hash_results) {
+    SRV_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(s.d_hash), 0u, 2 * n_req, s.stream));
+    SRV_HIP(launch_hash_ranges(s.d_arena, E.cfg.arena_pairs, s.d_ranges,
+                               reinterpret_cast<const uint32_t*>(s.d_desc + kHeaderOff),
+                               s.d_total, s.d_hash, s.stream));
+    SRV_HIP(hipMemcpyAsync(s.h_hash, s.d_hash, n_req * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s.stream));
+  }
+  return RTS_OK;
+}
+
+int slot_init(Engine& E, Slot& s) {
+  const uint32_t maxr = E.cfg.max_batch_requests;
+  const uint64_t maxq = E.cfg.max_batch_queries;
+  SRV_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+  SRV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_desc), E.desc_bytes));
+  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_ranges), 2 * maxr * sizeof(uint64_t)));
+  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_hash), maxr * sizeof(unsigned long long)));
+  SRV_HIP(hipMalloc(&s.d_desc, E.desc_bytes));
+  SRV_HIP(hipMalloc(&s.d_counters, 64 * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_counts, maxq * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_offsets, maxq * sizeof(uint64_t)));
+  SRV_HIP(hipMalloc(&s.d_seg, batch_seg_sums_words() * sizeof(uint64_t)));
+  SRV_HIP(hipMalloc(&s.d_total, 2 * sizeof(uint64_t)));
+  SRV_HIP(hipMalloc(&s.d_ranges, 2 * maxr * sizeof(uint64_t)));
+  SRV_HIP(hipMalloc(&s.d_hash, maxr * sizeof(unsigned long long)));
+  SRV_HIP(hipMalloc(&s.d_arena, E.cfg.arena_pairs * sizeof(rts_pair)));
+  std::memset(s.h_desc, 0, E.desc_bytes);
+  if (E.cfg.use_graphs) {
+    SRV_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_slot_work(E, s, E.desc_bytes, maxr);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s.stream, &g);
+    if (rc) return rc;
+    SRV_HIP(ec);
+    SRV_HIP(hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0));
+    SRV_HIP(hipGraphDestroy(g));
+  }
+  return RTS_OK;
+}
+
+void slot_free(Slot& s) {
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
+  if (s.exec) (void)hipGraphExecDestroy(s.exec);
+  if (s.h_desc) (void)hipHostFree(s.h_desc);
+  if (s.h_ranges) (void)hipHostFree(s.h_ranges);
+  if (s.h_hash) (void)hipHostFree(s.h_hash);
+  for (void* p : {(void*)s.d_desc, (void*)s.d_counters, (void*)s.d_counts, (void*)s.d_offsets,
+                  (void*)s.d_seg, (void*)s.d_total, (void*)s.d_ranges, (void*)s.d_hash,
+                  (void*)s.d_arena})
+    if (p) (void)hipFree(p);
+  if (s.done) (void)hipEventDestroy(s.done);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+  s = Slot();
+}
+
+// Seeds the per-kind estimates with a counting query over a sample of the
+// first request of each kind.
+int probe_estimates(Engine& E, hipStream_t stream) {
+  unsigned long long* d_cnt = nullptr;
+  unsigned long long* h_cnt = nullptr;
+  SRV_HIP(hipMalloc(&d_cnt, sizeof(unsigned long long)));
+  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_cnt), sizeof(unsigned long long)));
+  bool seen[4] = {false, false, false, false};
+  int rc = RTS_OK;
+  for (uint64_t i = 0; i < E.n_reqs && !rc; i++) {
+    const rts_request& r = E.reqs[i];
+    const int k = kind_of(r);
+    if (seen[k] || r.n == 0) continue;
+    seen[k] = true;
+    const uint64_t m = std::min<uint64_t>(r.n, 4096);
+    if (launch_fill_u32(reinterpret_cast<uint32_t*>(d_cnt), 0u, 2, stream) != hipSuccess) {
+      rc = RTS_ERR_HIP;
+      break;
+    }
+    rc = r.points ? rts_index_query_points(E.ix, RTS_CONTAINS,
+                                           static_cast<const rts_point*>(r.queries), m,
+                                           RTS_HANDLER_COUNT, d_cnt, stream)
+                  : rts_index_query_envelopes(E.ix, (rts_predicate)r.predicate,
+                                              static_cast<const rts_envelope*>(r.queries), m,
+                                              RTS_HANDLER_COUNT, d_cnt, stream);
+    if (rc) break;
+    if (hipMemcpyAsync(h_cnt, d_cnt, sizeof(*h_cnt), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      rc = RTS_ERR_HIP;
+      break;
+    }
+    E.est[k].observe(*h_cnt, m);
+  }
+  (void)hipFree(d_cnt);
+  (void)hipHostFree(h_cnt);
+  return rc;
+}
+
+// An arrived update waits until no request is admitted in part, nothing is in
+// flight (an in-flight chunk may still overflow and need a re-launch) and no
+// chunk waits for a re-launch, so that every request runs on one index state;
+// new requests are held back meanwhile.
+bool update_ready(Engine& E, uint64_t next_up, double t, bool idle) {
+  if (next_up >= E.n_ups || E.ups[next_up].arrival_s > t) {
+    E.hold_new = false;
+    return false;
+  }
+  E.hold_new = true;
+  return idle && E.partial == 0 && E.retry.empty();
+}
+
+int apply_update(Engine& E, const rts_update_batch& u, const std::vector<Slot>& slots) {
+  for (const Slot& s : slots)
+    if (s.busy) SRV_HIP(hipStreamWaitEvent(E.upd_stream, s.done, 0));
+  int rc = rts_index_update(E.ix, u.updates, u.n, E.upd_stream);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(E.ix->mu);
+    rc = ensure_built(E.ix, E.upd_stream);
+  }
+  if (rc) return rc;
+  SRV_HIP(hipEventRecord(E.upd_event, E.upd_stream));
+  E.have_upd_event = true;
+  E.epoch++;
+  E.applied_updates++;
+  return RTS_OK;
+}
+
+// ------------------------------------------------------------ batched ----
+int complete_slot(Engine& E, Slot& s) {
+  const double tc = E.now();
+  for (size_t i = 0; i < s.chunks.size(); i++) {
+    const Chunk& c = s.chunks[i];
+    const uint64_t start = s.h_ranges[2 * i], cnt = s.h_ranges[2 * i + 1];
+    if (start + cnt > E.cfg.arena_pairs) {
+      overflowed(E, c, cnt);
+      continue;
+    }
+    const int rc = copy_out(E, c, s.d_arena + start, cnt, s.stream);
+    if (rc) return rc;
+    finish_chunk(E, c, cnt, E.cfg.hash_results ? s.h_hash[i] : 0, tc);
+  }
+  s.busy = false;
+  return RTS_OK;
+}
+
+int launch_slot(Engine& E, Slot& s, const std::vector<Chunk>& chunks) {
+  *reinterpret_cast<TreeView*>(s.h_desc + kViewOff) = tree_view(E.ix);
+  BatchHeader* h = reinterpret_cast<BatchHeader*>(s.h_desc + kHeaderOff);
+  BatchRequest* br = reinterpret_cast<BatchRequest*>(s.h_desc + kReqsOff);
+  uint64_t qb = 0;
+  uint32_t tb = 0;
+  for (size_t i = 0; i < chunks.size(); i++) {
+    const Chunk& c = chunks[i];
+    const rts_request& r = E.reqs[c.req];
+    const size_t qs = r.points ? sizeof(rts_point) : sizeof(rts_envelope);
+    br[i].queries = static_cast<const char*>(r.queries) + c.q0 * qs;
+    br[i].n = (uint32_t)c.n;
+    br[i].flags = tile_flags(E, r);
+    br[i].qbase = qb;
+    br[i].tbase = tb;
+    br[i].qid_base = (uint32_t)c.q0;  // pairs report query indices within the request
+    qb += c.n;
+    tb += (br[i].flags & kReqWaveTiles) ? (uint32_t)c.n : (uint32_t)((c.n + kWave - 1) / kWave);
+  }
+  h->n_requests = (uint32_t)chunks.size();
+  h->n_tiles = tb;
+  h->n_queries = qb;
+  if (E.have_upd_event) SRV_HIP(hipStreamWaitEvent(s.stream, E.upd_event, 0));
+  if (E.cfg.use_graphs) {
+    SRV_HIP(hipGraphLaunch(s.exec, s.stream));
+  } else {
+    const int rc = record_slot_work(E, s, kReqsOff + chunks.size() * sizeof(BatchRequest),
+                                    chunks.size());
+    if (rc) return rc;
+  }
+  SRV_HIP(hipEventRecord(s.done, s.stream));
+  s.chunks = chunks;
+  s.busy = true;
+  E.batches++;
+  return RTS_OK;
+}
+
+int run_batched(Engine& E) {
+  std::vector<Slot> slots(std::max<uint32_t>(1, E.cfg.slots));
+  int rc = RTS_OK;
+  for (auto& s : slots)
+    if ((rc = slot_init(E, s))) break;
+  uint64_t next_up = 0;
+  std::vector<Chunk> chunks;
+  E.t0 = Clock::now();
+  while (!rc && E.done < E.n_reqs) {
+    for (auto& s : slots)
+      if (!rc && s.busy && hipEventQuery(s.done) == hipSuccess) rc = complete_slot(E, s);
+    const double t = E.now();
+    bool idle = true;
+    for (const auto& s : slots) idle = idle && !s.busy;
+    while (!rc && update_ready(E, next_up, t, idle)) rc = apply_update(E, E.ups[next_up++], slots);
+    for (auto& s : slots) {
+      if (rc || s.busy) continue;
+      admit(E, t, E.cfg.max_batch_requests, E.cfg.max_batch_queries, (double)E.cfg.arena_pairs,
+            true, chunks);
+      if (chunks.empty()) break;
+      rc = launch_slot(E, s, chunks);
+    }
+  }
+  for (auto& s : slots) slot_free(s);
+  return rc;
+}
+
+// -------------------------------------------------------------- plain ----
+struct PlainChunk {
+  Chunk c;
+  uint32_t log;  // ring slot of its counter span, hash and event
+};
+
+int run_plain(Engine& E) {
+  constexpr uint32_t kRing = 1024;
+  hipStream_t stream = nullptr;
+  rts_pair* arena = nullptr;
+  unsigned long long* d_counter = nullptr;
+  rts_dev_queue* d_queue = nullptr;
+  uint64_t* d_spans = nullptr;  // per launch: queue counter before and after
+  uint64_t* h_spans = nullptr;  // pinned copy
+  unsigned long long* d_hash = nullptr;
+  unsigned long long* h_hash = nullptr;
+  std::vector<hipEvent_t> evs(kRing, nullptr);
+  SRV_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  SRV_HIP(hipMalloc(&arena, E.cfg.arena_pairs * sizeof(rts_pair)));
+  SRV_HIP(hipMalloc(&d_counter, sizeof(unsigned long long)));
+  SRV_HIP(hipMalloc(&d_queue, sizeof(rts_dev_queue)));
+  SRV_HIP(hipMalloc(&d_spans, 2 * kRing * sizeof(uint64_t)));
+  SRV_HIP(hipMalloc(&d_hash, kRing * sizeof(unsigned long long)));
+  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_spans), 2 * kRing * sizeof(uint64_t)));
+  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_hash), kRing * sizeof(unsigned long long)));
+  for (auto& e : evs) SRV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const rts_dev_queue q{arena, E.cfg.arena_pairs, d_counter};
+  SRV_HIP(hipMemcpy(d_queue, &q, sizeof(q), hipMemcpyHostToDevice));
+  SRV_HIP(hipMemset(d_counter, 0, sizeof(unsigned long long)));
+
+  std::deque<PlainChunk> inflight;
+  uint64_t next_up = 0, ring = 0;
+  double used = 0;  // predicted results in the arena since its last reset
+  int rc = RTS_OK;
+  std::vector<Chunk> one;
+  auto reset_arena = [&]() -> int {
+    SRV_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(d_counter), 0u, 2, stream));
+    used = 0;
+    return RTS_OK;
+  };
+  E.t0 = Clock::now();
+  while (!rc && E.done < E.n_reqs) {
+    // completions, in launch order
+    while (!rc && !inflight.empty() && hipEventQuery(evs[inflight.front().log]) == hipSuccess) {
+      const PlainChunk pc = inflight.front();
+      inflight.pop_front();
+      const uint64_t before = h_spans[2 * pc.log], after = h_spans[2 * pc.log + 1];
+      if (after > E.cfg.arena_pairs) {
+        overflowed(E, pc.c, after - before);
+        continue;
+      }
+      rc = copy_out(E, pc.c, arena + before, after - before, stream);
+      finish_chunk(E, pc.c, after - before, E.cfg.hash_results ? h_hash[pc.log] : 0, E.now());
+    }
+    if (rc) break;
+    const double t = E.now();
+    while (!rc && update_ready(E, next_up, t, inflight.empty())) {
+      rc = rts_index_update(E.ix, E.ups[next_up].updates, E.ups[next_up].n, stream);
+      next_up++;
+      E.epoch++;
+      E.applied_updates++;
+    }
+    if (rc || inflight.size() >= kRing - 1) continue;
+    // A re-launch waits until everything launched before it is done, then
+    // runs on an emptied arena.
+    if (!E.retry.empty() && used > 0) {
+      if (inflight.empty()) rc = reset_arena();
+      continue;
+    }
+    admit(E, t, 1, E.cfg.chunk_queries, (double)E.cfg.arena_pairs - used, used == 0, one);
+    if (one.empty()) {
+      // Arrived work does not fit the rest of the arena: empty it once idle.
+      const bool waiting = !E.queue.empty() && E.reqs[E.queue.front()].arrival_s <= t;
+      if (waiting && inflight.empty() && used > 0) rc = reset_arena();
+      continue;
+    }
+    const Chunk c = one[0];
+    const rts_request& r = E.reqs[c.req];
+    used += predicted(E, c);
+    const uint32_t li = (uint32_t)(ring++ % kRing);
+    SRV_HIP(hipMemcpyAsync(d_spans + 2 * li, d_counter, sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                           stream));
+    const size_t qs = r.points ? sizeof(rts_point) : sizeof(rts_envelope);
+    const void* qp = static_cast<const char*>(r.queries) + c.q0 * qs;
+    rc = r.points ? rts_index_query_points(E.ix, RTS_CONTAINS, static_cast<const rts_point*>(qp),
+                                           c.n, RTS_HANDLER_COLLECT, d_queue, stream)
+                  : rts_index_query_envelopes(E.ix, (rts_predicate)r.predicate,
+                                              static_cast<const rts_envelope*>(qp), c.n,
+                                              RTS_HANDLER_COLLECT, d_queue, stream);
+    if (rc) break;
+    SRV_HIP(hipMemcpyAsync(d_spans + 2 * li + 1, d_counter, sizeof(uint64_t),
+                           hipMemcpyDeviceToDevice, stream));
+    // Query ids come back relative to the launch: make them relative to the request.
+    if (c.q0 || E.cfg.hash_results) {
+      if (E.cfg.hash_results)
+        SRV_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(d_hash + li), 0u, 2, stream));
+      SRV_HIP(launch_rebase_hash_span(arena, E.cfg.arena_pairs, d_spans + 2 * li, (uint32_t)c.q0,
+                                      E.cfg.hash_results ? 1 : 0, d_hash + li, stream));
+    }
+    SRV_HIP(hipMemcpyAsync(h_spans + 2 * li, d_spans + 2 * li, 2 * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, stream));
+    if (E.cfg.hash_results)
+      SRV_HIP(hipMemcpyAsync(h_hash + li, d_hash + li, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, stream));
+    SRV_HIP(hipEventRecord(evs[li], stream));
+    inflight.push_back(PlainChunk{c, li});
+    E.batches++;
+  }
+  (void)hipStreamSynchronize(stream);
+  for (auto& e : evs)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipFree(arena);
+  (void)hipFree(d_counter);
+  (void)hipFree(d_queue);
+  (void)hipFree(d_spans);
+  (void)hipFree(d_hash);
+  (void)hipHostFree(h_spans);
+  (void)hipHostFree(h_hash);
+  (void)hipStreamDestroy(stream);
+  return rc;
+}
+
+}  // namespace
+}  // namespace rts
+
+using namespace rts;
+
+extern "C" {
+
+void rts_serve_config_default(rts_serve_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->mode = RTS_SERVE_BATCHED;
+  c->slots = 3;
+  c->max_batch_requests = 4096;
+  c->use_graphs = 1;
+  c->max_batch_queries = 4ull << 20;
+  c->chunk_queries = 256ull << 10;
+  c->arena_pairs = 1ull << 30;
+  c->hash_results = 0;
+  c->traversal = RTS_TRAVERSAL_AUTO;
+}
+
+int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request* reqs,
+                  uint64_t n_reqs, const rts_update_batch* ups, uint64_t n_ups,
+                  rts_request_result* results, rts_serve_stats* stats) {
+  if (!ix || !results || (n_reqs && !reqs) || (n_ups && !ups)) {
+    set_error("rts_serve_run: null argument");
+    return RTS_ERR_INVALID;
+  }
+  Engine E;
+  E.ix = ix;
+  if (cfg) E.cfg = *cfg; else rts_serve_config_default(&E.cfg);
+  if (E.cfg.max_batch_requests == 0 || E.cfg.max_batch_requests > 65536 ||
+      E.cfg.max_batch_queries == 0 || E.cfg.max_batch_queries >= (1ull << 31) ||
+      E.cfg.chunk_queries == 0 || E.cfg.arena_pairs == 0) {
+    set_error("rts_serve_run: invalid configuration");
+    return RTS_ERR_INVALID;
+  }
+  for (uint64_t i = 0; i < n_reqs; i++) {
+    const rts_request& r = reqs[i];
+    if ((r.n && !r.queries) || r.predicate < 0 || r.predicate > 2 ||
+        (r.points && r.predicate != RTS_CONTAINS) || r.n >= (1ull << 31) ||
+        (i && r.arrival_s < reqs[i - 1].arrival_s)) {
+      set_error("rts_serve_run: bad request " + std::to_string(i));
+      return RTS_ERR_INVALID;
+    }
+  }
+  E.reqs = reqs;
+  E.n_reqs = n_reqs;
+  E.ups = ups;
+  E.n_ups = n_ups;
+  E.res = results;
+  std::memset(results, 0, n_reqs * sizeof(rts_request_result));
+  E.st.assign(n_reqs, ReqState());
+  E.desc_bytes = kReqsOff + (size_t)E.cfg.max_batch_requests * sizeof(BatchRequest);
+  for (uint32_t i = 0; i < n_reqs; i++) E.queue.push_back(i);
+  int rc = hipSetDevice(ix->device) == hipSuccess ? RTS_OK : RTS_ERR_HIP;
+  if (!rc && hipStreamCreateWithFlags(&E.upd_stream, hipStreamNonBlocking) != hipSuccess)
+    rc = RTS_ERR_HIP;
+  if (!rc && hipEventCreateWithFlags(&E.upd_event, hipEventDisableTiming) != hipSuccess)
+    rc = RTS_ERR_HIP;
+  if (!rc) {
+    std::lock_guard<std::mutex> lk(ix->mu);
+    rc = ensure_built(ix, E.upd_stream);
+  }
+  if (!rc && hipStreamSynchronize(E.upd_stream) != hipSuccess) rc = RTS_ERR_HIP;
+  if (!rc) rc = probe_estimates(E, E.upd_stream);
+  if (!rc) rc = (E.cfg.mode == RTS_SERVE_PLAIN) ? run_plain(E) : run_batched(E);
+  if (E.upd_stream) {
+    (void)hipStreamSynchronize(E.upd_stream);
+    (void)hipStreamDestroy(E.upd_stream);
+  }
+  if (E.upd_event) (void)hipEventDestroy(E.upd_event);
+  if (stats) {
+    stats->requests = n_reqs;
+    stats->queries = 0;
+    stats->results = 0;
+    double first = n_reqs ? reqs[0].arrival_s : 0, last = 0;
+    for (uint64_t i = 0; i < n_reqs; i++) {
+      stats->queries += reqs[i].n;
+      stats->results += results[i].count;
+      last = std::max(last, results[i].done_s);
+    }
+    stats->wall_s = last - first;
+    stats->batches = E.batches;
+    stats->relaunches = E.relaunches;
+    stats->updates = E.applied_updates;
+  }
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,352 @@
+// Device-side LBVH traversal cores shared by the per-request query kernels
+// (query.hip) and the persistent batched kernel (batch.hip).
+//
+// Node tests are necessary conditions of the leaf predicate on exact FP32
+// bounds, so pruning never drops a true pair; every reported pair passed the
+// exact leaf predicate (reference geom/envelope.cuh:29-69), except that a
+// subtree lying inside the query whose leaves are all live is emitted (or
+// counted) without testing: refit bounds cover every live box below a node,
+// so each of them matches.
+#pragma once
+
+#include "rts_internal.h"
+#include "wave.h"
+
+namespace rts {
+
+template <int PRED, bool POINTS>
+struct Pred;
+
+// Point queries: Envelope::Contains(point) (envelope.cuh:29-38).
+template <int P>
+struct Pred<P, true> {
+  __device__ static bool node(const float4 n, const float4 q) { return contains_point(n, q.x, q.y); }
+  __device__ static bool inside(const float4, const float4) { return false; }
+  __device__ static bool leaf(const float4 b, const float4 q) { return contains_point(b, q.x, q.y); }
+};
+// Envelope intersects (envelope.cuh:55-69).
+template <>
+struct Pred<RTS_INTERSECTS, false> {
+  __device__ static bool node(const float4 n, const float4 q) { return intersects(n, q); }
+  __device__ static bool inside(const float4 n, const float4 q) { return contains_box(q, n); }
+  __device__ static bool leaf(const float4 b, const float4 q) { return intersects(b, q); }
+};
+// Data box contains the query (envelope.cuh:40-53, the reference's kContains).
+template <>
+struct Pred<RTS_CONTAINS, false> {
+  __device__ static bool node(const float4 n, const float4 q) { return contains_box(n, q); }
+  __device__ static bool inside(const float4, const float4) { return false; }
+  __device__ static bool leaf(const float4 b, const float4 q) { return contains_box(b, q); }
+};
+// Data box inside the query: the node must meet the query.
+template <>
+struct Pred<RTS_WITHIN, false> {
+  __device__ static bool node(const float4 n, const float4 q) { return intersects(n, q); }
+  __device__ static bool inside(const float4 n, const float4 q) { return contains_box(q, n); }
+  __device__ static bool leaf(const float4 b, const float4 q) { return contains_box(q, b); }
+};
+
+__device__ __forceinline__ void lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ float4 load_query(const void* queries, bool points, uint32_t i) {
+  if (points) {
+    const float2 p = static_cast<const float2*>(queries)[i];
+    return make_float4(p.x, p.y, p.x, p.y);
+  }
+  return static_cast<const float4*>(queries)[i];
+}
+
+// ---------------------------------------------------------------- wave ----
+// One query per wavefront. The wave expands up to 64 stack entries per step
+// (one per lane) from a stack in LDS; children that are leaves, small
+// subtrees (<= leaf_size leaves) or lie inside the query become leaf ranges
+// in an LDS list. Once the list holds enough leaves the wave scans it, 4 x 64
+// leaves per step -- consecutive lanes read consecutive leaves -- and writes
+// the matches of this query contiguously from slot `obase`.
+constexpr int kWaveStack = 1024;    // LDS stack entries per wave
+constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
+constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
+constexpr int kScanUnroll = 4;      // leaves per lane per scan step
+constexpr uint32_t kScanStep = kScanUnroll * kWave;
+
+// Range entries: first leaf | kAllMatch when every leaf of the range is a
+// live box inside the query (no test needed).
+constexpr uint32_t kAllMatch = 0x80000000u;
+
+struct WaveLds {
+  uint32_t stack[kWaveStack];
+  uint32_t rfirst[kWaveRanges];
+  uint32_t rpre[kWaveRanges + 1];
+};
+
+// Returns the query's number of results (wave-uniform). With WRITE the pairs
+// (geom_id, qid) go to out[obase ...], slots >= cap dropped; without WRITE,
+// subtrees inside the query are counted from their live-leaf counts.
+template <int PRED, bool POINTS, bool WRITE>
+__device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_t qid,
+                               rts_pair* out, const uint64_t cap, const uint64_t obase,
+                               WaveLds& L) {
+  using P = Pred<PRED, POINTS>;
+  constexpr bool kCounting = !WRITE;
+  const uint32_t lane = lane_id();
+  uint32_t* stack = L.stack;
+  uint32_t* rfirst = L.rfirst;
+  uint32_t* rpre = L.rpre;
+  uint32_t sp = 0, m = 0, total = 0;  // wave-uniform
+  uint64_t emitted = 0;               // wave-uniform: results of scanned ranges
+  uint64_t inside_live = 0;           // per lane: counting mode, subtrees inside the query
+
+  const float4 rb = t.root_box[0];
+  if (P::node(rb, qb)) {
+    const bool inside = P::inside(rb, qb);
+    if (kCounting && inside) {
+      if (lane == 0) inside_live = t.root_live[0];
+    } else if (t.n <= t.leaf_size || t.n == 1 || inside) {
+      if (lane == 0) {
+        rfirst[0] = (inside && t.root_live[0] == t.n) ? kAllMatch : 0u;
+        rpre[0] = 0;
+        rpre[1] = t.n;
+      }
+      m = 1;
+      total = t.n;
+    } else {
+      if (lane == 0) stack[0] = 0;
+      sp = 1;
+    }
+  }
+  lds_fence();
+
+  const uint64_t max_iters = 4ull * t.n + 64;
+  uint64_t iters = 0;
+  while (sp > 0 || m > 0) {
+    if (++iters > max_iters) {
+      if (lane == 0) atomicOr(t.error, kErrTraversal);
+      break;
+    }
+    if (sp > 0 && total < kFlushLeaves && m + 2 * kWave <= kWaveRanges) {
+      // ---- expand up to 64 stack entries, one per lane
+      const uint32_t c = sp < (uint32_t)kWave ? sp : (uint32_t)kWave;
+      const bool act = lane < c;
+      uint32_t e = act ? stack[sp - 1 - lane] : 0u;
+      lds_fence();
+      sp -= c;
+      // Near a full stack, subtrees are scanned as ranges instead of pushed.
+      const bool degrade = sp + 2 * kWave > (uint32_t)kWaveStack;
+      uint32_t push0 = 0, push1 = 0, np = 0;
+      uint32_t rf0 = 0, rn0 = 0, rf1 = 0, rn1 = 0, nr = 0;
+      if (act && e < t.n - 1) {
+        const NodeRec* nd = t.nodes + e;
+        const uint4 mt = nd->meta;
+        const uint4 lv = nd->live;
+        const float4 cb[2] = {nd->box[0], nd->box[1]};
+        const uint32_t live[2] = {lv.x, lv.y};
+        const uint32_t gamma = mt.x & ~kLeafFlag;
+        const uint32_t ref[2] = {mt.x, mt.y};
+        const uint32_t first[2] = {mt.z, gamma + 1};
+        const uint32_t cnt[2] = {gamma - mt.z + 1, mt.w - gamma};
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          if (!P::node(cb[side], qb)) continue;
+          const bool inside = P::inside(cb[side], qb);
+          if (kCounting && inside) {
+            inside_live += live[side];
+          } else if ((ref[side] & kLeafFlag) || cnt[side] <= t.leaf_size || degrade || inside) {
+            uint32_t f = first[side];
+            if (inside && live[side] == cnt[side]) f |= kAllMatch;
+            if (nr == 0) {
+              rf0 = f;
+              rn0 = cnt[side];
+            } else {
+              rf1 = f;
+              rn1 = cnt[side];
+            }
+            nr++;
+          } else {
+            if (np == 0) push0 = ref[side]; else push1 = ref[side];
+            np++;
+          }
+        }
+      }
+      const uint32_t pinc = wave_inclusive_sum(np);
+      const uint32_t ptot = readlane_u32(pinc, kWave - 1);
+      const uint32_t pex = pinc - np;
+      if (np > 0) stack[sp + pex] = push0;
+      if (np > 1) stack[sp + pex + 1] = push1;
+      sp += ptot;
+      const uint32_t rinc = wave_inclusive_sum(nr);
+      const uint32_t rtot = readlane_u32(rinc, kWave - 1);
+      const uint32_t rex = rinc - nr;
+      const uint32_t sz = rn0 + rn1;
+      const uint32_t sinc = wave_inclusive_sum(sz);
+      const uint32_t stot = readlane_u32(sinc, kWave - 1);
+      const uint32_t pre = total + (sinc - sz);
+      if (nr > 0) {
+        rfirst[m + rex] = rf0;
+        rpre[m + rex] = pre;
+      }
+      if (nr > 1) {
+        rfirst[m + rex + 1] = rf1;
+        rpre[m + rex + 1] = pre + rn0;
+      }
+      m += rtot;
+      total += stot;
+      if (lane == 0) rpre[m] = total;
+      lds_fence();
+    } else {
+      // ---- scan the queued leaf ranges, kScanUnroll x 64 leaves per step:
+      // all loads of a step are issued before any result is written.
+      uint32_t e0 = 0;
+      for (uint32_t base = 0; base < total; base += kScanStep) {
+        uint32_t leaf[kScanUnroll];
+        bool hit[kScanUnroll];
+        // Fast path: the whole step lies in entry e0.
+        const bool one_range = rpre[e0 + 1] >= base + kScanStep;
+        uint32_t lo = e0;
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; u++) {
+          const uint32_t j = base + u * kWave + lane;
+          hit[u] = false;
+          leaf[u] = 0;
+          if (j < total) {
+            if (!one_range) {
+              uint32_t hi = m - 1;
+              while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (rpre[mid] <= j) lo = mid; else hi = mid - 1;
+              }
+            }
+            const uint32_t f = rfirst[lo];
+            leaf[u] = (f & ~kAllMatch) + (j - rpre[lo]);
+            hit[u] = (f & kAllMatch) != 0;
+            if (!hit[u] && leaf[u] < t.n) hit[u] = P::leaf(t.leaf_boxes[leaf[u]], qb);
+          } else {
+            lo = m - 1;
+          }
+        }
+        e0 = readlane_u32(lo, kWave - 1);
+        if (WRITE) {
+          uint32_t gid[kScanUnroll];
+#pragma unroll
+          for (int u = 0; u < kScanUnroll; u++)
+            gid[u] = (hit[u] && leaf[u] < t.n) ? t.leaf_ids[leaf[u]] : 0u;
+#pragma unroll
+          for (int u = 0; u < kScanUnroll; u++) {
+            const uint64_t mask = ballot(hit[u]);
+            if (hit[u]) {
+              const uint64_t slot = obase + emitted + mask_rank(mask);
+              if (slot < cap) {
+                rts_pair pr;
+                pr.geom_id = gid[u];
+                pr.query_id = qid;
+                out[slot] = pr;
+              }
+            }
+            emitted += (uint64_t)__popcll(mask);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < kScanUnroll; u++) emitted += (uint64_t)__popcll(ballot(hit[u]));
+        }
+      }
+      m = 0;
+      total = 0;
+      lds_fence();
+    }
+  }
+  return emitted + wave_sum_u64(inside_live);
+}
+
+// ---------------------------------------------------------------- lane ----
+// One query per lane with a private stack; for queries with few results
+// (points, data-contains-query). Counting mode counts subtrees inside the
+// query from their live-leaf counts; writing mode writes the query's pairs to
+// out[obase ...] in traversal order. Returns the query's number of results.
+template <int PRED, bool POINTS, bool WRITE>
+__device__ uint64_t lane_query(const TreeView& t, const float4 qb, const uint32_t qid,
+                               rts_pair* out, const uint64_t cap, const uint64_t obase) {
+  using P = Pred<PRED, POINTS>;
+  uint32_t stack[kStackDepth];
+  int sp = 0;
+  uint32_t cur = 1, end = 0;  // leaf range being scanned: [cur, end]
+  bool all = false;           // the range needs no test
+  uint64_t found = 0;
+  const float4 rb = t.root_box[0];
+  if (!P::node(rb, qb)) return 0;
+  const bool root_inside = P::inside(rb, qb);
+  if (root_inside && !WRITE) return t.root_live[0];
+  if (t.n <= t.leaf_size || t.n == 1 || root_inside) {
+    cur = 0;
+    end = t.n - 1;
+    all = root_inside && t.root_live[0] == t.n;
+  } else {
+    stack[sp++] = 0;
+  }
+  const uint64_t max_steps = 4ull * t.n + 64;
+  for (uint64_t steps = 0;; steps++) {
+    if (steps > max_steps) {
+      atomicOr(t.error, kErrTraversal);
+      break;
+    }
+    if (cur <= end) {
+      const uint32_t prim = cur++;
+      if (prim < t.n && (all || P::leaf(t.leaf_boxes[prim], qb))) {
+        if (WRITE && obase + found < cap) {
+          rts_pair pr;
+          pr.geom_id = t.leaf_ids[prim];
+          pr.query_id = qid;
+          out[obase + found] = pr;
+        }
+        found++;
+      }
+      continue;
+    }
+    if (sp == 0) break;
+    const uint32_t e = stack[--sp];
+    if (e & kLeafFlag) {
+      cur = end = e & ~kLeafFlag;
+      all = false;
+      continue;
+    }
+    if (e & kRangeFlag) {
+      const uint32_t node = e & kIndexMask;
+      if (node < t.n - 1) {
+        const uint4 m = t.nodes[node].meta;
+        cur = m.z;
+        end = m.w < t.n ? m.w : t.n - 1;
+        all = false;
+      }
+      continue;
+    }
+    if (e >= t.n - 1) continue;
+    const NodeRec* nd = t.nodes + e;
+    const uint4 mt = nd->meta;
+    const uint4 lv = nd->live;
+    const float4 cb[2] = {nd->box[0], nd->box[1]};
+    const uint32_t live[2] = {lv.x, lv.y};
+    const uint32_t gamma = mt.x & ~kLeafFlag;
+    const uint32_t ref[2] = {mt.x, mt.y};
+    const uint32_t cnt[2] = {gamma - mt.z + 1, mt.w - gamma};
+    // right child first so that the left one is expanded next
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int side = 1 - k;
+      if (!P::node(cb[side], qb)) continue;
+      const bool inside = P::inside(cb[side], qb);
+      if (!WRITE && inside) {
+        found += live[side];
+      } else if (ref[side] & kLeafFlag) {
+        if (sp < kStackDepth) stack[sp++] = ref[side];
+      } else if (cnt[side] <= t.leaf_size || inside) {
+        if (sp < kStackDepth) stack[sp++] = ref[side] | kRangeFlag;
+      } else {
+        if (sp < kStackDepth) stack[sp++] = ref[side];
+      }
+    }
+  }
+  return found;
+}
+
+}  // namespace rts

@@ -1,0 +1,164 @@
+"""GPU tests of the served (continuous batching) and plain per-request paths:
+every request's pair set must equal the oracle's answer for that request
+alone, on the index state of the request's epoch, whatever else runs beside
+it (mixed predicates, uneven sizes, overlapping batches, arena overflows and
+re-launches, interleaved updates)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import rtspatial_amd as R  # noqa: E402
+from rtspatial_amd import serve as S  # noqa: E402
+from rtspatial_amd import workloads as W  # noqa: E402
+from gpu_util import canon, dev, log, make_index  # noqa: E402
+
+P = R.Predicate
+
+
+def make_trace(rng, n_req, means, max_size, rate_hz=None, mix=(0.4, 0.3, 0.3)):
+    """Requests of the served mix: points, within, intersects; sizes
+    log-uniform in [1, max_size]."""
+    reqs, host = [], []
+    t = 0.0
+    for i in range(n_req):
+        kind = rng.choice(3, p=list(mix))
+        n = int(np.rint(np.exp(rng.uniform(0, np.log(max_size)))))
+        if kind == 0:
+            q = W.mixture_boxes(rng, means, n, np.log(5e-3))[:, :2].copy()
+            pred = P.kContains
+        else:
+            q = W.mixture_boxes(rng, means, n, np.log(5e-3))
+            pred = P.kWithin if kind == 1 else P.kIntersects
+        if rate_hz:
+            t += rng.exponential(1.0 / rate_hz)
+        host.append((q, pred))
+        reqs.append(S.Request(queries=dev(q), predicate=pred, arrival_s=t, collect=0))
+    return reqs, host
+
+
+def expected(oracle, boxes, alive, q, pred):
+    return oracle.RTree(boxes, alive=alive).pairs(int(pred), q)
+
+
+@pytest.fixture(scope="module")
+def scene():
+    rng = np.random.default_rng(77)
+    means = W.mixture_means(rng)
+    boxes = W.mixture_boxes(rng, means, 300_000, np.log(1e-3))
+    return rng, means, boxes
+
+
+def _collect_caps(oracle, boxes, host, reqs):
+    tree = oracle.RTree(boxes)
+    want = []
+    for (q, pred), r in zip(host, reqs):
+        w = tree.pairs(int(pred), q)
+        want.append(w)
+        r.collect = w.shape[0] + 8
+    return want
+
+
+@pytest.mark.parametrize("mode,graphs", [("batched", 1), ("batched", 0), ("plain", 0)])
+def test_each_request_alone_equals_in_mix(oracle, scene, mode, graphs):
+    rng, means, boxes = scene
+    log(f"serve mix mode={mode} graphs={graphs}")
+    reqs, host = make_trace(np.random.default_rng(1), 80, means, 3000)
+    want = _collect_caps(oracle, boxes, host, reqs)
+    ix = make_index(boxes)
+    res, st = S.run(ix, reqs, mode=mode, use_graphs=graphs, slots=3)
+    assert st["requests"] == 80 and st["batches"] >= 1
+    for i, (r, w) in enumerate(zip(res, want)):
+        assert r["count"] == w.shape[0], i
+        assert not r["truncated"]
+        assert np.array_equal(canon(r["pairs"]), w), i
+
+
+def test_poisson_arrivals_overlapping_batches_and_hashes(oracle, scene):
+    # Open-loop arrivals; batched and plain runs report the same per-request
+    # counts and pair hashes, and both equal the oracle's.
+    rng, means, boxes = scene
+    log("serve poisson arrivals + hashes")
+    reqs, host = make_trace(np.random.default_rng(2), 120, means, 20000, rate_hz=2000.0)
+    ix = make_index(boxes)
+    rb, sb = S.run(ix, reqs, mode="batched", hash_results=1, slots=3)
+    rp, sp = S.run(ix, reqs, mode="plain", hash_results=1)
+    tree = oracle.RTree(boxes)
+    for i, (q, pred) in enumerate(host):
+        total, cnt, hs, hx = tree.summary(int(pred), q)
+        h = int(np.sum(hs, dtype=np.uint64)) if q.shape[0] else 0
+        assert rb[i]["count"] == rp[i]["count"] == total, i
+        assert rb[i]["hash"] == rp[i]["hash"] == h, i
+        assert rb[i]["done_s"] >= rb[i]["admit_s"] >= reqs[i].arrival_s - 1e-9
+    assert sb["queries"] == sp["queries"]
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_arena_overflow_relaunch_is_exact(oracle, scene, mode):
+    # A tiny arena forces overflows, re-launches and chunk splits.
+    rng, means, boxes = scene
+    log(f"serve arena overflow mode={mode}")
+    reqs, host = make_trace(np.random.default_rng(3), 30, means, 400)
+    want = _collect_caps(oracle, boxes, host, reqs)
+    biggest = max(w.shape[0] for w in want)
+    ix = make_index(boxes)
+    res, st = S.run(ix, reqs, mode=mode, arena_pairs=max(4096, biggest // 3 + 1), chunk_queries=64)
+    assert st["relaunches"] > 0
+    for i, (r, w) in enumerate(zip(res, want)):
+        assert np.array_equal(canon(r["pairs"]), w), i
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_updates_interleaved_each_request_sees_its_epoch(oracle, scene, mode):
+    rng, means, boxes0 = scene
+    log(f"serve with updates mode={mode}")
+    boxes0 = boxes0[:100_000].copy()
+    r2 = np.random.default_rng(4)
+    reqs, host = make_trace(r2, 60, means, 2000, rate_hz=3000.0)
+    # 4 update batches spread over the trace
+    span = reqs[-1].arrival_s
+    ups, states = [], [boxes0.copy()]
+    cur = boxes0.copy()
+    for k in range(4):
+        ids = r2.choice(cur.shape[0], 5000, replace=False).astype(np.int64)
+        shift = r2.uniform(-0.01, 0.01, size=(5000, 2)).astype(np.float32)
+        nb = cur[ids].copy()
+        nb[:, [0, 2]] += shift[:, :1]
+        nb[:, [1, 3]] += shift[:, 1:]
+        cur = cur.copy()
+        cur[ids] = nb
+        states.append(cur)
+        ups.append(S.UpdateBatch(ids=dev(ids), boxes=dev(nb), arrival_s=span * (k + 1) / 5))
+    caps = []
+    for (q, pred), r in zip(host, reqs):
+        m = max(oracle.RTree(s).count(int(pred), q) for s in states)
+        r.collect = m + 8
+    ix = make_index(boxes0)
+    res, st = S.run(ix, reqs, ups, mode=mode, slots=3)
+    assert st["updates"] == 4
+    trees = [oracle.RTree(s) for s in states]
+    epochs = set()
+    for i, ((q, pred), r) in enumerate(zip(host, res)):
+        e = r["epoch"]
+        epochs.add(e)
+        assert np.array_equal(canon(r["pairs"]), trees[e].pairs(int(pred), q)), (i, e)
+    assert len(epochs) >= 2
+
+
+def test_empty_and_single_query_requests(oracle, scene):
+    rng, means, boxes = scene
+    log("serve tiny requests")
+    q1 = np.float32([[0.5, 0.5, 0.51, 0.51]])
+    reqs = [S.Request(queries=dev(np.zeros((0, 4), np.float32)), collect=1),
+            S.Request(queries=dev(q1), predicate=P.kIntersects, collect=100000),
+            S.Request(queries=dev(np.float32([[0.5, 0.5]])), predicate=P.kContains, collect=1000)]
+    ix = make_index(boxes)
+    res, _ = S.run(ix, reqs)
+    assert res[0]["count"] == 0
+    assert np.array_equal(canon(res[1]["pairs"]), oracle.bf_pairs(oracle.INTERSECTS, boxes, q1))
+    assert np.array_equal(canon(res[2]["pairs"]),
+                          oracle.bf_pairs(oracle.CONTAINS, boxes, np.float32([[0.5, 0.5]])))
