@@ -1,0 +1,167 @@
+// rtspatial::SpatialIndex<float, 2> -- the reference's C++ surface
+// (include/rtspatial/spatial_index.cuh:20-931) as a header-only facade over
+// the C ABI in rts_c.h. Method names, argument meanings and id semantics
+// follow the reference; streams are hipStream_t; failures throw
+// rtspatial::Exception (the reference throws on CUDA/OptiX errors and aborts
+// on a bad predicate, spatial_index.cuh:387-389).
+//
+// The result handler of Query(..., void* arg, stream) is chosen at compile
+// time, as the reference chooses its shader handler: include
+// rtspatial/default_handlers/counting_handlers.h before this header for the
+// counting handler (arg: device unsigned long long*), otherwise arg is a
+// Queue::DeviceObject() (collecting handler).
+#ifndef RTSPATIAL_AMD_SPATIAL_INDEX_H
+#define RTSPATIAL_AMD_SPATIAL_INDEX_H
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <string>
+#include <type_traits>
+
+#include "rts_c.h"
+#include "rtspatial/default_handlers/collecting_handlers.h"
+#include "rtspatial/geom/envelope.h"
+#include "rtspatial/geom/point.h"
+#include "rtspatial/geom/predicates.h"
+#include "rtspatial/utils/array_view.h"
+#include "rtspatial/utils/exception.h"
+#include "rtspatial/utils/queue.h"
+
+namespace rtspatial {
+
+// spatial_index.cuh:22-37. ptx_root, prefer_fast_build_* and compact are
+// OptiX settings; they are accepted and ignored.
+struct Config {
+  std::string ptx_root;
+  size_t max_geometries = 1000 * 1000 * 10;
+  size_t max_queries = 1000 * 1000;
+  bool preallocate = false;
+  bool prefer_fast_build_geom = false;
+  bool prefer_fast_build_query = false;
+  bool compact = false;
+  float geom_sample_rate = 0.2;
+  float query_sample_rate = 0.2;
+  float intersect_cost_weight = 0.95;
+  uint32_t max_geom_samples = 10000;
+  uint32_t max_query_samples = 1000;
+  uint32_t max_parallelism = 512;
+  // MI355X additions (see rts_config)
+  uint32_t leaf_size = 8;
+  uint32_t rebuild_policy = RTS_REBUILD_SAH;
+  float sah_rebuild_ratio = 1.5f;
+  uint32_t traversal = RTS_TRAVERSAL_AUTO;
+  uint32_t sort_queries = 0;
+};
+
+template <typename COORD_T, int N_DIMS>
+class SpatialIndex;
+
+template <>
+class SpatialIndex<float, 2> {
+ public:
+  using point_t = Point<float, 2>;
+  using envelope_t = Envelope<point_t>;
+  using update_t = pair<size_t, envelope_t>;
+  static_assert(sizeof(update_t) == sizeof(rts_update), "pair<size_t, envelope> is 24 bytes");
+
+  SpatialIndex() = default;
+  ~SpatialIndex() {
+    if (ix_) rts_index_destroy(ix_);
+  }
+  SpatialIndex(const SpatialIndex&) = delete;
+  SpatialIndex& operator=(const SpatialIndex&) = delete;
+
+  // spatial_index.cuh:83-110
+  void Init(const Config& config) {
+    if (ix_) {
+      rts_index_destroy(ix_);
+      ix_ = nullptr;
+    }
+    rts_config c;
+    rts_config_default(&c);
+    c.max_geometries = config.max_geometries;
+    c.max_queries = config.max_queries;
+    c.preallocate = config.preallocate;
+    c.prefer_fast_build_geom = config.prefer_fast_build_geom;
+    c.prefer_fast_build_query = config.prefer_fast_build_query;
+    c.compact = config.compact;
+    c.geom_sample_rate = config.geom_sample_rate;
+    c.query_sample_rate = config.query_sample_rate;
+    c.intersect_cost_weight = config.intersect_cost_weight;
+    c.max_geom_samples = config.max_geom_samples;
+    c.max_query_samples = config.max_query_samples;
+    c.max_parallelism = config.max_parallelism;
+    c.leaf_size = config.leaf_size;
+    c.rebuild_policy = config.rebuild_policy;
+    c.sah_rebuild_ratio = config.sah_rebuild_ratio;
+    c.traversal = config.traversal;
+    c.sort_queries = config.sort_queries;
+    check(rts_index_create(&c, &ix_), "SpatialIndex::Init");
+  }
+
+  // spatial_index.cuh:118-129
+  void Clear() { check(rts_index_clear(handle()), "SpatialIndex::Clear"); }
+
+  // spatial_index.cuh:142-196
+  void Insert(ArrayView<envelope_t> envelopes, hipStream_t stream = nullptr) {
+    check(rts_index_insert(handle(), reinterpret_cast<const rts_envelope*>(envelopes.data()),
+                           envelopes.size(), stream),
+          "SpatialIndex::Insert");
+  }
+
+  // spatial_index.cuh:208-265
+  void Delete(const ArrayView<size_t> ids, hipStream_t stream) {
+    static_assert(sizeof(size_t) == sizeof(uint64_t), "size_t ids");
+    check(rts_index_delete(handle(), reinterpret_cast<const uint64_t*>(ids.data()), ids.size(),
+                           stream),
+          "SpatialIndex::Delete");
+  }
+
+  // spatial_index.cuh:280-336
+  void Update(const ArrayView<update_t> updates, hipStream_t stream) {
+    check(rts_index_update(handle(), reinterpret_cast<const rts_update*>(updates.data()),
+                           updates.size(), stream),
+          "SpatialIndex::Update");
+  }
+
+  // spatial_index.cuh:352-391
+  void Query(Predicate p, ArrayView<point_t> queries, void* arg, hipStream_t stream = nullptr) {
+    Query(p, queries, static_cast<rts_handler>(RTSPATIAL_HANDLER), arg, stream);
+  }
+  // spatial_index.cuh:407-515
+  void Query(Predicate p, ArrayView<envelope_t> queries, void* arg, hipStream_t stream = nullptr) {
+    Query(p, queries, static_cast<rts_handler>(RTSPATIAL_HANDLER), arg, stream);
+  }
+
+  // The same with the handler named at run time.
+  void Query(Predicate p, ArrayView<point_t> queries, rts_handler handler, void* arg,
+             hipStream_t stream) {
+    check(rts_index_query_points(handle(), static_cast<rts_predicate>(p),
+                                 reinterpret_cast<const rts_point*>(queries.data()), queries.size(),
+                                 handler, arg, stream),
+          "SpatialIndex::Query(points)");
+  }
+  void Query(Predicate p, ArrayView<envelope_t> queries, rts_handler handler, void* arg,
+             hipStream_t stream) {
+    check(rts_index_query_envelopes(handle(), static_cast<rts_predicate>(p),
+                                    reinterpret_cast<const rts_envelope*>(queries.data()),
+                                    queries.size(), handler, arg, stream),
+          "SpatialIndex::Query(envelopes)");
+  }
+
+  size_t size() const { return ix_ ? rts_index_size(ix_) : 0; }
+  void Build(hipStream_t stream = nullptr) { check(rts_index_build(handle(), stream), "Build"); }
+  void Sync(hipStream_t stream = nullptr) { check(rts_index_sync(handle(), stream), "Sync"); }
+  rts_index* c_handle() { return handle(); }
+
+ private:
+  rts_index* handle() const {
+    if (!ix_) throw Exception("SpatialIndex::Init has not been called");
+    return ix_;
+  }
+  rts_index* ix_ = nullptr;
+};
+
+}  // namespace rtspatial
+#endif

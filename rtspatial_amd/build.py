@@ -68,6 +68,30 @@ def build(verbose=False, jobs=4):
     return LIB
 
 
+CPP_TESTS = ["test_spatial_index"]
+
+
+def build_cpp_tests(verbose=False):
+    """Compiles the C++ facade tests (tests/cpp) against the library."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    bin_dir = os.path.join(src_dir, "bin")
+    os.makedirs(bin_dir, exist_ok=True)
+    outs = []
+    for name in CPP_TESTS:
+        out = os.path.join(bin_dir, name)
+        cmd = [hipcc(), "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{src_dir}",
+               os.path.join(src_dir, name + ".cpp"), "-o", out + ".tmp", f"-L{HERE}",
+               "-lrtspatial_amd", "-Wl,-rpath,$ORIGIN/../../../rtspatial_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {name} failed:\n{r.stdout}")
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
+
+
 def _wait(item):
     name, p = item
     out, _ = p.communicate()
@@ -83,3 +107,4 @@ def _wait(item):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_tests(verbose="-v" in sys.argv))

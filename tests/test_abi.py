@@ -83,3 +83,22 @@ def test_python_api_surface():
     for m in ("Init", "Clear", "Insert", "Delete", "Update", "Query"):
         assert hasattr(R.SpatialIndex, m)
     assert int(R.Predicate.kIntersects) == 0 and int(R.Predicate.kContains) == 1
+
+
+def test_cpp_facade_headers_compile_standalone(tmp_path):
+    # The C++ facade (include/rtspatial/) is host-only C++ over the C ABI: it
+    # must compile with a plain host compiler and the HIP runtime headers.
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if not gxx or not os.path.exists("/opt/rocm/include/hip/hip_runtime_api.h"):
+        pytest.skip("no g++ or HIP headers")
+    src = tmp_path / "use_facade.cpp"
+    src.write_text(
+        '#include "rtspatial/default_handlers/counting_handlers.h"\n'
+        '#include "rtspatial/rtspatial.h"\n'
+        "int main() { rtspatial::SpatialIndex<float, 2> ix; (void)ix.size(); return 0; }\n")
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__",
+                        "-I/opt/rocm/include", f"-I{os.path.join(ROOT, 'include')}", str(src)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
