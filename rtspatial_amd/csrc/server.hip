@@ -118,6 +118,7 @@ struct Engine {
   std::deque<Chunk> retry;     // overflowed chunks, with their exact counts
   uint64_t partial = 0;        // requests admitted in part
   bool hold_new = false;       // an update is waiting: admit no new request
+  double last_progress = 0;    // time of the last completion (watchdog)
   hipStream_t upd_stream = nullptr;
   hipEvent_t upd_event = nullptr;
   bool have_upd_event = false;
@@ -130,6 +131,21 @@ double predicted(const Engine& E, const Chunk& c) {
   return E.est[kind_of(E.reqs[c.req])].per_query * 1.5 * (double)c.n + 16.0;
 }
 
+// A chunk's results are final.
+void finish_chunk(Engine& E, const Chunk& c, uint64_t count, uint64_t hash, double t) {
+  ReqState& s = E.st[c.req];
+  rts_request_result& R = E.res[c.req];
+  R.count += count;
+  R.hash += hash;
+  E.last_progress = t;
+  E.est[kind_of(E.reqs[c.req])].observe(count, c.n);
+  if (--s.chunks_out == 0 && s.next_q >= E.reqs[c.req].n) {
+    s.done = true;
+    R.done_s = t;
+    E.done++;
+  }
+}
+
 // Fills `out` with re-launches first, then chunks of arrived requests, while
 // they fit: at most max_req chunks, max_q queries and `budget` predicted
 // results. With `force`, one chunk is taken even if it does not fit.
@@ -140,6 +156,14 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
   double used = 0;
   while (!E.retry.empty() && out.size() < max_req) {
     Chunk c = E.retry.front();
+    if (c.known != kUnknown && c.known > E.cfg.arena_pairs && c.n == 1) {
+      // One query with more results than an arena holds: report its exact
+      // count, mark the request truncated and drop its pairs.
+      E.retry.pop_front();
+      E.res[c.req].truncated = 1;
+      finish_chunk(E, c, c.known, 0, t);
+      continue;
+    }
     if (c.known != kUnknown && (double)c.known > (double)E.cfg.arena_pairs && c.n > 1) {
       // It can never fit one arena: split it; the halves' counts are unknown.
       E.retry.pop_front();
@@ -198,20 +222,6 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
       E.queue.pop_front();
       E.partial--;
     }
-  }
-}
-
-// A chunk's results are final.
-void finish_chunk(Engine& E, const Chunk& c, uint64_t count, uint64_t hash, double t) {
-  ReqState& s = E.st[c.req];
-  rts_request_result& R = E.res[c.req];
-  R.count += count;
-  R.hash += hash;
-  E.est[kind_of(E.reqs[c.req])].observe(count, c.n);
-  if (--s.chunks_out == 0 && s.next_q >= E.reqs[c.req].n) {
-    s.done = true;
-    R.done_s = t;
-    E.done++;
   }
 }
 
@@ -397,6 +407,23 @@ int apply_update(Engine& E, const rts_update_batch& u, const std::vector<Slot>& 
   return RTS_OK;
 }
 
+// The host loop gives up if work is in flight but nothing has completed for
+// this long, so that rts_serve_run always returns.
+constexpr double kStallSeconds = 120.0;
+
+bool stalled(Engine& E, bool in_flight) {
+  const double t = E.now();
+  if (!in_flight) {
+    E.last_progress = t;
+    return false;
+  }
+  if (t - E.last_progress > kStallSeconds) {
+    set_error("rts_serve_run: no batch completed for 120 s");
+    return true;
+  }
+  return false;
+}
+
 // ------------------------------------------------------------ batched ----
 int complete_slot(Engine& E, Slot& s) {
   const double tc = E.now();
@@ -466,6 +493,10 @@ int run_batched(Engine& E) {
     const double t = E.now();
     bool idle = true;
     for (const auto& s : slots) idle = idle && !s.busy;
+    if (stalled(E, !idle)) {
+      rc = RTS_ERR_DEVICE;
+      break;
+    }
     while (!rc && update_ready(E, next_up, t, idle)) rc = apply_update(E, E.ups[next_up++], slots);
     for (auto& s : slots) {
       if (rc || s.busy) continue;
@@ -534,6 +565,10 @@ int run_plain(Engine& E) {
       finish_chunk(E, pc.c, after - before, E.cfg.hash_results ? h_hash[pc.log] : 0, E.now());
     }
     if (rc) break;
+    if (stalled(E, !inflight.empty())) {
+      rc = RTS_ERR_DEVICE;
+      break;
+    }
     const double t = E.now();
     while (!rc && update_ready(E, next_up, t, inflight.empty())) {
       rc = rts_index_update(E.ix, E.ups[next_up].updates, E.ups[next_up].n, stream);

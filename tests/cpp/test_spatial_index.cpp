@@ -45,10 +45,10 @@ std::vector<point_t> uniform_points(size_t n) {
   return out;
 }
 
-SharedValue<unsigned long long>& counter() {
-  static SharedValue<unsigned long long> c;
-  return c;
-}
+// The counting handler's device counter; created in main() and released
+// before main() returns (no device memory is freed during static teardown).
+SharedValue<unsigned long long>* g_counter = nullptr;
+SharedValue<unsigned long long>& counter() { return *g_counter; }
 
 template <typename Q>
 unsigned long long count(index_t& index, Predicate p, device_vector<Q>& q, Stream& s) {
@@ -257,4 +257,13 @@ TEST(PointQueries, bad_predicate_throws) {
   ASSERT_TRUE(threw);
 }
 
-int main(int argc, char** argv) { return mini_test::run_all(argc > 1 ? argv[1] : nullptr); }
+int main(int argc, char** argv) {
+  int rc;
+  {
+    SharedValue<unsigned long long> c;
+    g_counter = &c;
+    rc = mini_test::run_all(argc > 1 ? argv[1] : nullptr);
+    g_counter = nullptr;
+  }
+  return rc;
+}

@@ -133,14 +133,12 @@ def test_updates_interleaved_each_request_sees_its_epoch(oracle, scene, mode):
         cur[ids] = nb
         states.append(cur)
         ups.append(S.UpdateBatch(ids=dev(ids), boxes=dev(nb), arrival_s=span * (k + 1) / 5))
-    caps = []
+    trees = [oracle.RTree(s) for s in states]
     for (q, pred), r in zip(host, reqs):
-        m = max(oracle.RTree(s).count(int(pred), q) for s in states)
-        r.collect = m + 8
+        r.collect = max(t.count(int(pred), q) for t in trees) + 8
     ix = make_index(boxes0)
     res, st = S.run(ix, reqs, ups, mode=mode, slots=3)
     assert st["updates"] == 4
-    trees = [oracle.RTree(s) for s in states]
     epochs = set()
     for i, ((q, pred), r) in enumerate(zip(host, res)):
         e = r["epoch"]
