@@ -262,7 +262,8 @@ typedef struct rts_serve_config {
   uint32_t mode;               /* rts_serve_mode */
   uint32_t slots;              /* batches in flight (batched mode) (3) */
   uint32_t max_batch_requests; /* request chunks per batch (4096) */
-  uint32_t use_graphs;         /* replay each slot's launches as a hipGraph (1) */
+  uint32_t use_graphs;         /* replay each slot's launches as a hipGraph (0: off by
+                                  default; see the DESIGN notes on graph replay) */
   uint64_t max_batch_queries;  /* queries per batch (4M) */
   uint64_t chunk_queries;      /* largest chunk of one request (256K) */
   uint64_t arena_pairs;        /* result arena per slot (or of the plain path), pairs */

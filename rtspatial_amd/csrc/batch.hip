@@ -97,7 +97,9 @@ __global__ void __launch_bounds__(kThreads) k_batch(const TreeView* tv, const Ba
   if (h.n_tiles == 0 || h.n_requests == 0 || t.n == 0) return;
   uint32_t* counters = a.tile_counters + (WRITE ? kXcds : 0);
   const uint32_t xcd = blockIdx.x % kXcds;
-  for (;;) {
+  // Every iteration consumes a tile (or finds none left), so n_tiles + 1
+  // iterations bound the loop.
+  for (uint32_t it = 0; it <= h.n_tiles; it++) {
     const uint32_t tile = grab_tile(counters, h.n_tiles, xcd);
     if (tile >= h.n_tiles) break;
     const uint32_t ri = find_request(a.requests, h.n_requests, tile);

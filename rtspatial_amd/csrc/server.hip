@@ -22,6 +22,8 @@
 // index state of its admission (its epoch).
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <string>
@@ -46,6 +48,23 @@ namespace {
   } while (0)
 
 using Clock = std::chrono::steady_clock;
+
+// RTS_SERVE_TRACE=1 prints each step of the host loop to stderr.
+bool trace_on() {
+  static const bool on = [] {
+    const char* v = std::getenv("RTS_SERVE_TRACE");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+#define SRV_TRACE(...)                          \
+  do {                                          \
+    if (trace_on()) {                           \
+      std::fprintf(stderr, "[rts_serve] " __VA_ARGS__); \
+      std::fputc('\n', stderr);                \
+      std::fflush(stderr);                      \
+    }                                           \
+  } while (0)
 
 constexpr uint64_t kUnknown = ~0ull;
 
@@ -311,7 +330,14 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipMalloc(&s.d_hash, maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_arena, E.cfg.arena_pairs * sizeof(rts_pair)));
   std::memset(s.h_desc, 0, E.desc_bytes);
+  SRV_TRACE("slot buffers ready (arena %llu pairs)", (unsigned long long)E.cfg.arena_pairs);
   if (E.cfg.use_graphs) {
+    // Run the sequence once outside capture (an empty batch) so that every
+    // kernel is loaded before the stream is captured.
+    const int wrc = record_slot_work(E, s, kReqsOff, 1);
+    if (wrc) return wrc;
+    SRV_HIP(hipStreamSynchronize(s.stream));
+    SRV_TRACE("warm-up done; capturing");
     SRV_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
     const int rc = record_slot_work(E, s, E.desc_bytes, maxr);
     hipGraph_t g = nullptr;
@@ -320,6 +346,7 @@ int slot_init(Engine& E, Slot& s) {
     SRV_HIP(ec);
     SRV_HIP(hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0));
     SRV_HIP(hipGraphDestroy(g));
+    SRV_TRACE("graph instantiated");
   }
   return RTS_OK;
 }
@@ -427,6 +454,7 @@ bool stalled(Engine& E, bool in_flight) {
 // ------------------------------------------------------------ batched ----
 int complete_slot(Engine& E, Slot& s) {
   const double tc = E.now();
+  SRV_TRACE("batch of %zu chunks complete at %.6f s", s.chunks.size(), tc);
   for (size_t i = 0; i < s.chunks.size(); i++) {
     const Chunk& c = s.chunks[i];
     const uint64_t start = s.h_ranges[2 * i], cnt = s.h_ranges[2 * i + 1];
@@ -476,6 +504,8 @@ int launch_slot(Engine& E, Slot& s, const std::vector<Chunk>& chunks) {
   s.chunks = chunks;
   s.busy = true;
   E.batches++;
+  SRV_TRACE("batch %llu launched: %zu chunks, %u tiles, %llu queries", (unsigned long long)E.batches,
+            chunks.size(), h->n_tiles, (unsigned long long)h->n_queries);
   return RTS_OK;
 }
 
@@ -649,7 +679,9 @@ void rts_serve_config_default(rts_serve_config* c) {
   c->mode = RTS_SERVE_BATCHED;
   c->slots = 3;
   c->max_batch_requests = 4096;
-  c->use_graphs = 1;
+  // hipGraph replay of the batch sequence is off by default: a run with it on
+  // did not complete on the MI355X and the cause has not been established.
+  c->use_graphs = 0;
   c->max_batch_queries = 4ull << 20;
   c->chunk_queries = 256ull << 10;
   c->arena_pairs = 1ull << 30;
