@@ -63,14 +63,14 @@ __device__ __forceinline__ float4 load_query(const void* queries, bool points, u
 // One query per wavefront. The wave expands up to 64 stack entries per step
 // (one per lane) from a stack in LDS; children that are leaves, small
 // subtrees (<= leaf_size leaves) or lie inside the query become leaf ranges
-// in an LDS list. Once the list holds enough leaves the wave scans it, 4 x 64
-// leaves per step -- consecutive lanes read consecutive leaves -- and writes
-// the matches of this query contiguously from slot `obase`.
+// in an LDS list. Once the list holds enough leaves the wave scans it, 512
+// leaves per step (8 consecutive leaves per lane), and writes the matches of
+// this query contiguously from slot `obase`.
 constexpr int kWaveStack = 1024;    // LDS stack entries per wave
 constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
 constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
-constexpr int kScanUnroll = 4;      // leaves per lane per scan step
-constexpr uint32_t kScanStep = kScanUnroll * kWave;
+constexpr int kPerLane = 8;         // consecutive leaves per lane per scan step
+constexpr uint32_t kScanStep = kPerLane * kWave;
 
 // Range entries: first leaf | kAllMatch when every leaf of the range is a
 // live box inside the query (no test needed).
@@ -196,60 +196,99 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
       if (lane == 0) rpre[m] = total;
       lds_fence();
     } else {
-      // ---- scan the queued leaf ranges, kScanUnroll x 64 leaves per step:
-      // all loads of a step are issued before any result is written.
-      uint32_t e0 = 0;
+      // ---- scan the queued leaf ranges, kScanStep leaves per step: lane L
+      // owns the kPerLane consecutive leaves from flat index base + L*kPerLane,
+      // so its ids come in two 16-byte loads and its matches go out as one
+      // contiguous run of pairs placed by a single wave prefix sum.
+      uint32_t e0 = 0;  // wave-uniform: entry holding flat index `base`
       for (uint32_t base = 0; base < total; base += kScanStep) {
-        uint32_t leaf[kScanUnroll];
-        bool hit[kScanUnroll];
-        // Fast path: the whole step lies in entry e0.
-        const bool one_range = rpre[e0 + 1] >= base + kScanStep;
-        uint32_t lo = e0;
+        const uint32_t j0 = base + lane * kPerLane;
+        uint32_t leaf[kPerLane];
+        uint32_t nhit = 0;
+        uint32_t hitmask = 0;
+        bool contiguous = false;
+        if (j0 < total) {
+          // entry holding j0: binary search in [e0, m) unless the step lies in e0
+          uint32_t e = e0;
+          if (rpre[e0 + 1] < base + kScanStep) {
+            uint32_t hi = m - 1;
+            while (e < hi) {
+              const uint32_t mid = (e + hi + 1) >> 1;
+              if (rpre[mid] <= j0) e = mid; else hi = mid - 1;
+            }
+          }
+          uint32_t f = rfirst[e], lo_j = rpre[e], hi_j = rpre[e + 1];
+          contiguous = j0 + kPerLane <= hi_j;
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; u++) {
-          const uint32_t j = base + u * kWave + lane;
-          hit[u] = false;
-          leaf[u] = 0;
-          if (j < total) {
-            if (!one_range) {
-              uint32_t hi = m - 1;
-              while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (rpre[mid] <= j) lo = mid; else hi = mid - 1;
+          for (int k = 0; k < kPerLane; k++) {
+            const uint32_t j = j0 + k;
+            if (j < total && j >= hi_j) {  // ranges are non-empty: one step forward
+              e++;
+              f = rfirst[e];
+              lo_j = hi_j;
+              hi_j = rpre[e + 1];
+            }
+            leaf[k] = (f & ~kAllMatch) + (j - lo_j);
+            bool h = false;
+            if (j < total) {
+              if (f & kAllMatch) {
+                h = true;
+              } else if (leaf[k] < t.n) {
+                h = P::leaf(t.leaf_boxes[leaf[k]], qb);
               }
             }
-            const uint32_t f = rfirst[lo];
-            leaf[u] = (f & ~kAllMatch) + (j - rpre[lo]);
-            hit[u] = (f & kAllMatch) != 0;
-            if (!hit[u] && leaf[u] < t.n) hit[u] = P::leaf(t.leaf_boxes[leaf[u]], qb);
+            hitmask |= (h ? 1u : 0u) << k;
+          }
+          nhit = __popc(hitmask);
+        }
+        // e0 for the next step: the entry of the step's last leaf
+        {
+          const uint32_t jl = base + kScanStep - 1 < total ? base + kScanStep - 1 : total - 1;
+          uint32_t e = e0, hi = m - 1;
+          while (e < hi) {
+            const uint32_t mid = (e + hi + 1) >> 1;
+            if (rpre[mid] <= jl) e = mid; else hi = mid - 1;
+          }
+          e0 = e;
+        }
+        const uint32_t inc = wave_inclusive_sum(nhit);
+        const uint32_t step_hits = readlane_u32(inc, kWave - 1);
+        if (WRITE && nhit) {
+          uint32_t gid[kPerLane];
+          if (contiguous && leaf[0] + kPerLane <= t.n) {
+            // consecutive ids; the address is only 4-byte aligned
+            __builtin_memcpy(gid, t.leaf_ids + leaf[0], sizeof(gid));
           } else {
-            lo = m - 1;
+#pragma unroll
+            for (int k = 0; k < kPerLane; k++)
+              gid[k] = ((hitmask >> k) & 1u) && leaf[k] < t.n ? t.leaf_ids[leaf[k]] : 0u;
           }
-        }
-        e0 = readlane_u32(lo, kWave - 1);
-        if (WRITE) {
-          uint32_t gid[kScanUnroll];
+          uint64_t slot = obase + emitted + (inc - nhit);
+          if (hitmask == (1u << kPerLane) - 1 && slot + kPerLane <= cap) {
+            // all kPerLane matched: one contiguous run of pairs (8-byte aligned)
+            rts_pair run[kPerLane];
 #pragma unroll
-          for (int u = 0; u < kScanUnroll; u++)
-            gid[u] = (hit[u] && leaf[u] < t.n) ? t.leaf_ids[leaf[u]] : 0u;
+            for (int k = 0; k < kPerLane; k++) {
+              run[k].geom_id = gid[k];
+              run[k].query_id = qid;
+            }
+            __builtin_memcpy(out + slot, run, sizeof(run));
+          } else {
 #pragma unroll
-          for (int u = 0; u < kScanUnroll; u++) {
-            const uint64_t mask = ballot(hit[u]);
-            if (hit[u]) {
-              const uint64_t slot = obase + emitted + mask_rank(mask);
-              if (slot < cap) {
-                rts_pair pr;
-                pr.geom_id = gid[u];
-                pr.query_id = qid;
-                out[slot] = pr;
+            for (int k = 0; k < kPerLane; k++) {
+              if ((hitmask >> k) & 1u) {
+                if (slot < cap) {
+                  rts_pair pr;
+                  pr.geom_id = gid[k];
+                  pr.query_id = qid;
+                  out[slot] = pr;
+                }
+                slot++;
               }
             }
-            emitted += (uint64_t)__popcll(mask);
           }
-        } else {
-#pragma unroll
-          for (int u = 0; u < kScanUnroll; u++) emitted += (uint64_t)__popcll(ballot(hit[u]));
         }
+        emitted += step_hits;
       }
       m = 0;
       total = 0;
