@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librtspatial_amd.so")
+# RTS_LIBRARY names another build of the same library (e.g. an instrumented one).
+LIB_PATH = os.environ.get("RTS_LIBRARY") or os.path.join(_HERE, "librtspatial_amd.so")
 
 RTS_OK = 0
 RTS_INTERSECTS, RTS_CONTAINS, RTS_WITHIN = 0, 1, 2
