@@ -1,12 +1,20 @@
-"""Benchmark of the LBVH spatial index on one MI355X.
+"""Benchmarks of the LBVH spatial index on one MI355X. Prints one JSON line.
 
-Default workload (the 10M-box index the headline metric is quoted on):
-range-intersects over 10M Gaussian-mixture boxes with 1M skewed query
-envelopes; one step = one Query(kIntersects) over the 1M queries with the
-collecting handler, every result pair written to a resident device queue.
-Prints one JSON line.
+Default workload (--workload served): the served mixed stream over the 10M-box
+index the headline metric is quoted on. 10M Gaussian-mixture boxes are held
+resident; requests arrive as a Poisson stream with sizes log-uniform in
+[1, 1e6] queries, 40% point / 30% within / 30% intersects, and 100k-box update
+batches arrive at 10 Hz. The same trace is answered by the served path
+(continuous batching: persistent work-stealing batches, up to 3 in flight)
+and by the plain per-request path (one query call per request, in arrival
+order). `value` is the served throughput (Mqueries/s) at the highest offered
+load whose p99 request latency does not exceed the plain path's p99 at 70% of
+the plain path's capacity; both capacities and latencies are reported.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+--workload c3: one plain Query(kIntersects) of 1M skewed query envelopes over
+the 10M boxes per step, all ~1e10 pairs collected on the device.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served|c3]
 """
 import argparse
 import json
@@ -35,82 +43,56 @@ def cpu_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
-def query_bytes(nq, s_q, pairs, n_boxes):
-    """Algorithmic HBM bytes of one query launch: read the queries, write the
-    8-byte pairs, and one root-to-leaf descent per query capped at reading the
-    whole tree once (48 bytes per box)."""
+def query_bytes(nq, s_q, pairs, n_boxes, launches=1):
+    """Algorithmic HBM bytes of answering nq queries in `launches` launches:
+    read the queries, write the 8-byte pairs, and one root-to-leaf descent
+    per query (32 bytes per level) capped at reading the whole tree (48
+    bytes per box) once per launch."""
     descent = nq * math.ceil(math.log2(max(n_boxes, 2))) * 32
-    return nq * s_q + pairs * 8 + min(descent, 48 * n_boxes)
+    return nq * s_q + pairs * 8 + min(descent, 48 * n_boxes * launches)
 
 
-def cpu_baseline(boxes, queries, pred, threads, sample_rtree=20000, sample_bf=64):
+def cpu_baseline(boxes, samples, threads):
     """The oracle's STR R-tree (bulk load + parallel queries) and brute-force
-    scan, timed on the host on a bounded sample of the same workload."""
+    scan, timed on the host on a bounded sample of the workload's queries.
+    `samples` is a list of (predicate, queries) groups."""
     from oracle import oracle as O
 
-    rng = np.random.default_rng(99)
     t0 = time.perf_counter()
     tree = O.RTree(boxes)
     t_build = time.perf_counter() - t0
-    qs = queries[rng.choice(queries.shape[0], min(sample_rtree, queries.shape[0]), replace=False)]
-    tree.count(pred, qs[:256], threads)  # warm-up
+    for pred, qs in samples:
+        tree.count(pred, qs[:64], threads)  # warm-up
+    nq, npairs = 0, 0
     t0 = time.perf_counter()
-    n_rt = tree.count(pred, qs, threads)
+    for pred, qs in samples:
+        npairs += tree.count(pred, qs, threads)
+        nq += qs.shape[0]
     t_rt = time.perf_counter() - t0
-    qb = qs[:sample_bf]
+    pred0, q0 = samples[-1]
+    qb = q0[:32]
     t0 = time.perf_counter()
-    n_bf = O.bf_count(pred, boxes, qb, threads=threads)
+    n_bf = O.bf_count(pred0, boxes, qb, threads=threads)
     t_bf = time.perf_counter() - t0
     del tree
     return {
-        "value": round(qs.shape[0] / t_rt / 1e6, 6),
+        "value": round(nq / t_rt / 1e6, 6),
         "unit": "Mqueries/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle STR R-tree (fan-out 16) over all {boxes.shape[0]} boxes, "
-                   f"{qs.shape[0]} random queries of the workload ({n_rt} pairs, counted); "
+        "sample": (f"oracle STR R-tree (fan-out 16) over all {boxes.shape[0]} boxes, {nq} queries "
+                   f"drawn from the workload ({npairs} results, counted) on {threads} threads; "
                    f"R-tree bulk load {boxes.shape[0] / t_build / 1e6:.3f} Mboxes/s single-threaded; "
                    f"brute-force scan of {qb.shape[0]} queries x {boxes.shape[0]} boxes: "
-                   f"{qb.shape[0] / t_bf / 1e6:.6f} Mqueries/s ({n_bf} pairs)"),
+                   f"{qb.shape[0] / t_bf / 1e6:.6f} Mqueries/s ({n_bf} results)"),
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--boxes", type=int, default=10_000_000)
-    ap.add_argument("--queries", type=int, default=1_000_000)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    import torch
-
-    import rtspatial_amd as R
-    from rtspatial_amd import workloads as W
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-
-    log(f"generating workload: {args.boxes} boxes, {args.queries} queries")
-    boxes_np, queries_np = W.config3(args.boxes, args.queries)
-    boxes = torch.from_numpy(boxes_np).to(dev)
-    queries = torch.from_numpy(queries_np).to(dev)
-    pred = R.Predicate.kIntersects
-
-    ix = R.SpatialIndex()
-    ix.Init(R.Config(max_geometries=args.boxes))
-
-    # Build rate: rebuild from the same boxes a few times, phase-timed.
+def build_rate(R, ix, boxes, torch):
+    """Median LBVH build time over 3 rebuilds, phase-timed on the GPU."""
     R.profile_enable(True)
-    build_ms = []
-    for rep in range(3):
+    ms, sort_ms = [], []
+    for _ in range(3):
         ix.Clear()
         ix.Insert(boxes)
         torch.cuda.synchronize()
@@ -118,24 +100,237 @@ def main():
         ix.Build()
         ix.Sync()
         p = R.profile_read(reset=True)
-        build_ms.append(sum(p[k][1] for k in ("build_morton", "build_sort", "build_tree")))
-        sort_ms = p["build_sort"][1]
-    build_ms_med = float(np.median(build_ms))
-    log(f"build: {build_ms_med:.3f} ms ({args.boxes / build_ms_med / 1e3:.1f} Mboxes/s), sort {sort_ms:.3f} ms")
+        ms.append(sum(p[k][1] for k in ("build_morton", "build_sort", "build_tree")))
+        sort_ms.append(p["build_sort"][1])
+    R.profile_enable(False)
+    n = boxes.shape[0]
+    b, s = float(np.median(ms)), float(np.median(sort_ms))
+    return {"ms": round(b, 4), "mboxes_per_s": round(n / b / 1e3, 3), "sort_ms": round(s, 4),
+            "sort_mkeys_per_s": round(n / s / 1e3, 3)}
 
+
+# ---------------------------------------------------------------- served --
+def run_served(args):
+    import torch
+
+    import rtspatial_amd as R
+    from rtspatial_amd import serve as S
+    from rtspatial_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    n_boxes = args.boxes
+    log(f"generating 10M-box index and query pools ({n_boxes} boxes)")
+    rb = np.random.default_rng(3001)
+    means = W.mixture_means(rb)
+    boxes_np = W.mixture_boxes(rb, means, n_boxes, math.log(1e-4))
+    rq = np.random.default_rng(5002)
+    pool_n = 1 << 20
+    env_pool_np = W.mixture_boxes(rq, means, pool_n, math.log(5e-3))
+    pt_pool_np = W.mixture_boxes(rq, means, pool_n, math.log(5e-3))[:, :2].copy()
+    boxes = torch.from_numpy(boxes_np).to(dev)
+    env_pool = torch.from_numpy(env_pool_np).to(dev)
+    pt_pool = torch.from_numpy(pt_pool_np).to(dev)
+
+    ix = R.SpatialIndex()
+    ix.Init(R.Config(max_geometries=n_boxes))
+    build = build_rate(R, ix, boxes, torch)
+    log(f"build {build['ms']} ms ({build['mboxes_per_s']} Mboxes/s)")
+
+    # The trace: sizes log-uniform in [1, max_req], kinds 40/30/30, slices of
+    # the pools; arrival gaps exponential with mean 1 (scaled per rate).
+    rt = np.random.default_rng(5004)
+    n_req = args.requests
+    kinds = rt.choice(3, size=n_req, p=[0.4, 0.3, 0.3])
+    sizes = np.maximum(1, np.rint(np.exp(rt.uniform(0, math.log(args.max_request), n_req)))).astype(np.int64)
+    offs = rt.integers(0, pool_n - sizes + 1)
+    gaps = rt.exponential(1.0, size=n_req)
+    preds = [R.Predicate.kContains, R.Predicate.kWithin, R.Predicate.kIntersects]
+    slices = []
+    for k, n, o in zip(kinds, sizes, offs):
+        q = pt_pool[o:o + n] if k == 0 else env_pool[o:o + n]
+        slices.append((q, preds[k]))
+    total_q = int(sizes.sum())
+    s_q_total = int(sum(int(n) * (8 if k == 0 else 16) for k, n in zip(kinds, sizes)))
+
+    # Updates: 100k boxes shifted by U[-1e-3, 1e-3], 10 per second of trace.
+    ru = np.random.default_rng(5003)
+    n_upd_batches = 64
+    upd = []
+    for _ in range(n_upd_batches):
+        ids = ru.choice(n_boxes, 100_000, replace=False).astype(np.int64)
+        sh = ru.uniform(-1e-3, 1e-3, size=(100_000, 2)).astype(np.float32)
+        nb = boxes_np[ids].copy()
+        nb[:, [0, 2]] += sh[:, :1]
+        nb[:, [1, 3]] += sh[:, 1:]
+        upd.append((torch.from_numpy(ids).to(dev), torch.from_numpy(nb).to(dev)))
+
+    def trace(rate, with_updates=True):
+        t = np.cumsum(gaps) / rate if rate else np.zeros(n_req)
+        reqs = [S.Request(queries=q, predicate=p, arrival_s=float(a)) for (q, p), a in zip(slices, t)]
+        ups = []
+        if with_updates:
+            horizon = float(t[-1]) + 1.0 if rate else 5.0
+            for i in range(min(n_upd_batches, int(horizon * 10) + 1)):
+                ups.append(S.UpdateBatch(ids=upd[i][0], boxes=upd[i][1], arrival_s=0.1 * i))
+        return reqs, ups
+
+    def reset_index():
+        ix.Clear()
+        ix.Insert(boxes)
+        ix.Build()
+        ix.Sync()
+
+    cfg = dict(arena_pairs=args.arena_pairs, chunk_queries=256 << 10, slots=3)
+
+    def one(mode, rate, with_updates=True, profile=False, **kw):
+        reset_index()
+        reqs, ups = trace(rate, with_updates)
+        if profile:
+            R.profile_enable(True)
+            R.profile_read(True)
+        res, st = S.run(ix, reqs, ups, mode=mode, **cfg, **kw)
+        prof = R.profile_read(True) if profile else None
+        R.profile_enable(False)
+        lat = np.array([r["done_s"] - q.arrival_s for r, q in zip(res, reqs)])
+        out = {"mode": mode, "rate_rps": rate, "wall_s": round(st["wall_s"], 4),
+               "mq_per_s": round(st["queries"] / st["wall_s"] / 1e6, 4),
+               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+               "results": int(st["results"]), "batches": int(st["batches"]),
+               "relaunches": int(st["relaunches"]), "updates": int(st["updates"])}
+        log(json.dumps(out))
+        return out, res, prof
+
+    # Warm-up, then saturation (all requests at t = 0) of both paths.
+    for _ in range(args.warmup):
+        one("batched", 0.0, with_updates=False)
+    sat_plain, _, _ = one("plain", 0.0)
+    sat_served = None
+    for _ in range(max(1, args.steps)):
+        s, _, prof = one("batched", 0.0, profile=True)
+        if sat_served is None or s["mq_per_s"] > sat_served["mq_per_s"]:
+            sat_served, best_prof = s, prof
+
+    # Operating point: 70% of the plain path's capacity (requests per second).
+    plain_rps = n_req / sat_plain["wall_s"]
+    lam0 = 0.7 * plain_rps
+    op_plain, _, _ = one("plain", lam0)
+    target = op_plain["p99_ms"]
+    sweep, best = [], None
+    for m in (1.0, 1.25, 1.5, 2.0, 2.5, 3.0, 4.0):
+        lam = lam0 * m
+        s, _, _ = one("batched", lam)
+        sweep.append(s)
+        if s["p99_ms"] <= target:
+            best = s
+        elif m > 1.0:
+            break
+    if best is None:
+        best = sweep[0]
+
+    # Bit-exact check on the first requests: per-request pair hashes of both paths.
+    reset_index()
+    reqs_v, _ = trace(0.0, with_updates=False)
+    reqs_v = reqs_v[: args.verify]
+    rv_b, _ = S.run(ix, reqs_v, (), mode="batched", hash_results=1, **cfg)
+    rv_p, _ = S.run(ix, reqs_v, (), mode="plain", hash_results=1, **cfg)
+    same = all(a["count"] == b["count"] and a["hash"] == b["hash"] for a, b in zip(rv_b, rv_p))
+    assert same, "served and plain result hashes differ"
+
+    # Roofline of the dominant kernel (the batched write pass) over the
+    # saturated served run.
+    wn, wms = best_prof["batch_write"]
+    wbytes = query_bytes(total_q, 0, sat_served["results"], n_boxes, launches=max(wn, 1)) + s_q_total
+    achieved = wbytes / (wms / 1e3) / 1e9 if wms > 0 else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": best["mq_per_s"],
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(best["wall_s"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"served mixed stream: {n_boxes} Gaussian-mixture boxes (16 components, "
+                         f"sigma 0.02, log-normal sides mu=ln 1e-4) resident; {n_req} requests, "
+                         f"Poisson arrivals, sizes log-uniform in [1, {args.max_request}] queries "
+                         f"({total_q} queries), 40% point / 30% within / 30% intersects with "
+                         f"log-normal query sides mu=ln 5e-3; 100k-box update batches at 10 Hz; "
+                         f"served = continuous batching (3 slots, persistent work-stealing "
+                         f"batches), plain = one query call per request"),
+            "path": "served",
+            "boxes": n_boxes,
+            "requests": n_req,
+            "queries": total_q,
+            "step": "one replay of the request trace",
+            "seeds": {"boxes": 3001, "queries": 5002, "trace": 5004, "updates": 5003},
+        },
+        "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"]},
+        "plain": {"capacity_mq_per_s": sat_plain["mq_per_s"], "at_70pct": op_plain},
+        "served": {"capacity_mq_per_s": sat_served["mq_per_s"], "at_equal_p99": best,
+                   "sweep": sweep},
+        "p99_target_ms": target,
+        "bit_exact_check": {"requests": len(reqs_v), "counts_and_hashes_equal": same},
+        "build": build,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_batch<write> (batched write pass)",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": int(wbytes),
+            "launches": int(wn),
+            "kernel_ms": round(wms, 3),
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle R-tree / brute force on a sample)")
+        rs = np.random.default_rng(7)
+        samples = [(1, pt_pool_np[rs.choice(pool_n, 8000, replace=False)]),
+                   (2, env_pool_np[rs.choice(pool_n, 6000, replace=False)]),
+                   (0, env_pool_np[rs.choice(pool_n, 6000, replace=False)])]
+        out["cpu_baseline"] = cpu_baseline(boxes_np, samples, cpu_threads())
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+# -------------------------------------------------------------------- c3 --
+def run_c3(args):
+    import torch
+
+    import rtspatial_amd as R
+    from rtspatial_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    log(f"generating workload: {args.boxes} boxes, {args.queries} queries")
+    boxes_np, queries_np = W.config3(args.boxes, args.queries)
+    boxes = torch.from_numpy(boxes_np).to(dev)
+    queries = torch.from_numpy(queries_np).to(dev)
+    pred = R.Predicate.kIntersects
+    ix = R.SpatialIndex()
+    ix.Init(R.Config(max_geometries=args.boxes))
+    build = build_rate(R, ix, boxes, torch)
+    log(f"build: {build}")
     cnt = R.SharedValue()
     cnt.set(0)
     ix.Query(pred, queries, cnt)
     n_pairs = cnt.get()
     log(f"pairs per step: {n_pairs}")
     queue = R.Queue(n_pairs + 1024)
-
     for _ in range(args.warmup):
         queue.Clear()
         ix.Query(pred, queries, queue)
     torch.cuda.synchronize()
-    assert queue.size() == n_pairs, (queue.size(), n_pairs)
-
+    R.profile_enable(True)
     R.profile_read(reset=True)
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -152,20 +347,16 @@ def main():
     R.profile_enable(False)
     ix.Sync()
     assert queue.size() == n_pairs and not queue.overflowed()
-
     step_ms = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
     ms_per_step = wall * 1e3 / args.steps
     nq = queries.shape[0]
-    value = nq / (ms_per_step / 1e3) / 1e6
     trav_n, trav_ms = prof["traverse"]
     trav_avg = trav_ms / max(trav_n, 1)
-    order_ms = prof["query_order"][1] / max(prof["query_order"][0], 1)
     qbytes = query_bytes(nq, 16, n_pairs, args.boxes)
     achieved = qbytes / (trav_avg / 1e3) / 1e9
-
     out = {
         "metric": METRIC,
-        "value": round(value, 4),
+        "value": round(nq / (ms_per_step / 1e3) / 1e6, 4),
         "unit": "Mqueries/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -177,37 +368,54 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"range-intersects, skewed: {args.boxes} Gaussian-mixture boxes (16 "
-                         f"components, sigma 0.02, log-normal sides mu=ln 1e-4) x {nq} query "
-                         f"envelopes (log-normal sides mu=ln 5e-3); plain per-batch LBVH, one "
-                         f"Query(kIntersects) per step, all pairs collected on the device"),
-            "path": "plain",
-            "boxes": args.boxes,
-            "queries": nq,
-            "pairs_per_step": n_pairs,
+            "workload": (f"range-intersects, skewed: {args.boxes} Gaussian-mixture boxes x {nq} "
+                         f"query envelopes; plain per-batch LBVH, one Query(kIntersects) per step, "
+                         f"all pairs collected on the device"),
+            "path": "plain", "boxes": args.boxes, "queries": nq, "pairs_per_step": n_pairs,
             "seeds": {"boxes": 3001, "queries": 3002},
         },
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                        "p99": round(float(np.percentile(step_ms, 99)), 4)},
-        "build": {"ms": round(build_ms_med, 4), "mboxes_per_s": round(args.boxes / build_ms_med / 1e3, 3),
-                  "sort_ms": round(sort_ms, 4), "sort_mkeys_per_s": round(args.boxes / sort_ms / 1e3, 3)},
-        "phases_ms_per_step": {"query_order": round(order_ms, 4), "traverse": round(trav_avg, 4)},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_query_thread<intersects, envelopes, collect>",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "algorithmic_bytes_per_launch": int(qbytes),
-        },
+        "build": build,
+        "phases_ms_per_step": {"count_scan": round(prof["query_count"][1] / max(prof["query_count"][0], 1), 4),
+                               "write": round(trav_avg, 4)},
+        "roofline": {"bound": "hbm", "kernel": "k_query_wave<intersects, write>",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(qbytes)},
     }
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle R-tree / brute force on a sample)")
-        out["cpu_baseline"] = cpu_baseline(boxes_np, queries_np, int(pred), cpu_threads())
+        rs = np.random.default_rng(99)
+        out["cpu_baseline"] = cpu_baseline(
+            boxes_np, [(int(pred), queries_np[rs.choice(nq, 20000, replace=False)])], cpu_threads())
     else:
         out["cpu_baseline"] = None
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["served", "c3"], default="served")
+    ap.add_argument("--boxes", type=int, default=10_000_000)
+    ap.add_argument("--queries", type=int, default=1_000_000)
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--max-request", type=int, default=1_000_000)
+    ap.add_argument("--arena-pairs", type=int, default=3_000_000_000)
+    ap.add_argument("--verify", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    out = run_served(args) if args.workload == "served" else run_c3(args)
     print(json.dumps(out), flush=True)
 
 

@@ -303,7 +303,9 @@ enum {
   RTS_PHASE_QUERY_ORDER = 5,   /* Morton ordering of the queries */
   RTS_PHASE_TRAVERSE = 6,      /* the traversal kernel that emits the results */
   RTS_PHASE_QUERY_COUNT = 7,   /* count pass + scan + queue reservation (two-pass queries) */
-  RTS_PHASE_COUNT = 8
+  RTS_PHASE_BATCH_COUNT = 8,   /* batched: count pass + scan + request ranges */
+  RTS_PHASE_BATCH_WRITE = 9,   /* batched: write pass */
+  RTS_PHASE_COUNT = 12
 };
 
 typedef struct rts_profile {

@@ -117,11 +117,11 @@ class rts_serve_stats(ctypes.Structure):
 RTS_SERVE_BATCHED, RTS_SERVE_PLAIN = 0, 1
 
 RTS_PHASES = ["build_morton", "build_sort", "build_tree", "refit", "mutate", "query_order",
-              "traverse", "query_count"]
+              "traverse", "query_count", "batch_count", "batch_write"]
 
 
 class rts_profile(ctypes.Structure):
-    _fields_ = [("count", ctypes.c_uint64 * 8), ("ms", ctypes.c_double * 8)]
+    _fields_ = [("count", ctypes.c_uint64 * 12), ("ms", ctypes.c_double * 12)]
 
 
 # (name, restype, argtypes) of every exported entry point declared in rts_c.h.

@@ -235,13 +235,22 @@ uint32_t batch_persistent_blocks() {
 
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
   const uint32_t blocks = batch_persistent_blocks();
+  // Phase events are only recorded outside stream capture (profiling off
+  // or graphs off); a captured sequence is timed as a whole by its caller.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(stream, &cap);
+  const bool timed = cap == hipStreamCaptureStatusNone;
+  PhaseToken ph = timed ? phase_begin(RTS_PHASE_BATCH_COUNT, stream) : PhaseToken{-1, nullptr};
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.tile_counters, 2 * kXcds);
   hipLaunchKernelGGL(k_batch<false>, dim3(blocks), dim3(kThreads), 0, stream, t, a);
   hipLaunchKernelGGL(k_seg_sums, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_apply, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_request_ranges, dim3(16), dim3(kThreads), 0, stream, a);
+  phase_end(ph, stream);
+  ph = timed ? phase_begin(RTS_PHASE_BATCH_WRITE, stream) : PhaseToken{-1, nullptr};
   hipLaunchKernelGGL(k_batch<true>, dim3(blocks), dim3(kThreads), 0, stream, t, a);
+  phase_end(ph, stream);
   return hipGetLastError();
 }
 
