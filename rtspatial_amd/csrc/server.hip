@@ -5,9 +5,11 @@
 // Batched mode is continuous batching over `slots` streams. Each slot owns
 // device buffers for one batch (descriptors, per-query counts and offsets,
 // result arena) and a pinned staging area; its launch sequence (descriptor
-// upload, count pass, scan, request ranges, write pass, range download) is
-// captured once as a hipGraph and replayed for every batch, since every
-// kernel has a fixed grid and reads the batch's sizes from device memory.
+// upload, count pass, scan, request ranges, write pass, range download) has
+// fixed grids and reads the batch's sizes from device memory, so it can be
+// captured once as a hipGraph and replayed for every batch (use_graphs); by
+// default it is launched directly, because replay did not complete on the
+// MI355X and the cause is not established.
 //
 // Admission: arrived requests are taken in arrival order and split into
 // chunks of at most chunk_queries queries; chunks join the batch while their
