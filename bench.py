@@ -397,12 +397,113 @@ def run_c3(args):
     return out
 
 
+# -------------------------------------------------------------------- c4 --
+def run_c4(args):
+    import torch
+
+    import rtspatial_amd as R
+    from rtspatial_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    n_rounds = args.rounds
+    log(f"generating mutable-index workload: {args.boxes} boxes, {n_rounds} rounds")
+    m = W.MutationRounds(args.boxes, args.boxes // 10, args.queries)
+    initial = torch.from_numpy(m.initial).to(dev)
+    rounds = []
+    for _ in range(n_rounds):
+        d, i, u, up, p = m.next_round()
+        rounds.append(tuple(torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (d, i, u, up, p)))
+    final_boxes, final_alive = m.boxes, m.alive
+    del m
+    policies = {"refit": R._native.RTS_REBUILD_NEVER, "sah": R._native.RTS_REBUILD_SAH,
+                "rebuild": R._native.RTS_REBUILD_ALWAYS}
+    res, counts = {}, {}
+    for name, pol in policies.items():
+        for inserts in (True, False):
+            ix = R.SpatialIndex()
+            ix.Init(R.Config(max_geometries=args.boxes, rebuild_policy=pol))
+            ix.Insert(initial)
+            ix.Build()
+            torch.cuda.synchronize()
+            R.profile_enable(True)
+            R.profile_read(True)
+            cnt = R.SharedValue()
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(n_rounds)]
+            per = []
+            t0 = time.perf_counter()
+            for k, (d, i, u, up, p) in enumerate(rounds):
+                ev[k][0].record()
+                ix.Delete(d)
+                if inserts:
+                    ix.Insert(i)
+                ix.Update((u, up))
+                cnt.set(0)
+                ix.Query(R.Predicate.kContains, p, cnt)
+                ev[k][1].record()
+                if inserts:
+                    per.append(cnt.tensor().clone())
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            prof = R.profile_read(True)
+            R.profile_enable(False)
+            st = ix.stats()
+            ms = [a.elapsed_time(b) for a, b in ev]
+            key = f"{name}{'' if inserts else '_no_inserts'}"
+            res[key] = {"round_ms_median": round(float(np.median(ms)), 3),
+                        "round_ms_max": round(float(np.max(ms)), 3), "wall_s": round(wall, 3),
+                        "builds": int(st["builds"]), "refits": int(st["refits"]),
+                        "phases_ms": {k: round(v[1], 3) for k, v in prof.items() if v[0]}}
+            log(f"c4 {key}: {res[key]}")
+            if inserts:
+                counts[name] = [int(c.item()) for c in per]
+            del ix
+            torch.cuda.empty_cache()
+    same = counts["refit"] == counts["sah"] == counts["rebuild"]
+    assert same, "policies disagree on per-round result counts"
+    # Final round against the oracle on a sample of the points.
+    from oracle import oracle as O
+    pts = rounds[-1][4].cpu().numpy()
+    sample = pts[np.random.default_rng(11).choice(pts.shape[0], 20000, replace=False)]
+    ix = R.SpatialIndex()
+    ix.Init(R.Config(max_geometries=args.boxes))
+    ix.Insert(initial)
+    for d, i, u, up, _ in rounds:
+        ix.Delete(d)
+        ix.Insert(i)
+        ix.Update((u, up))
+    cnt = R.SharedValue()
+    cnt.set(0)
+    ix.Query(R.Predicate.kContains, torch.from_numpy(sample).to(dev), cnt)
+    want = O.RTree(final_boxes, alive=final_alive.astype(np.uint8)).count(O.CONTAINS, sample, cpu_threads())
+    assert cnt.get() == want, "final round differs from the oracle"
+    nq = args.queries
+    best = min(("refit", "sah", "rebuild"), key=lambda k: res[k]["round_ms_median"])
+    ms = res[best]["round_ms_median"]
+    return {
+        "metric": "mutable-index round time (1 delete + 1 insert + 1 update batch + 1 point query batch)",
+        "value": round(nq / (ms / 1e3) / 1e6, 4),
+        "unit": "Mqueries/s (point queries incl. the round's mutations)",
+        "n_gpus": 1, "steps": n_rounds, "warmup": 0, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": (f"mutable index: {args.boxes} uniform boxes (sides U[0,1e-3]); "
+                                f"{n_rounds} rounds of {args.boxes // 10} deletes, inserts and "
+                                f"updates (shift U[-1e-3,1e-3]) then {nq} uniform point queries"),
+                   "best_policy": best},
+        "policies": res,
+        "per_round_counts_equal_across_policies": same,
+        "oracle_check": {"sample_points": int(sample.shape[0]), "count": int(want)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["served", "c3"], default="served")
+    ap.add_argument("--workload", choices=["served", "c3", "c4"], default="served")
+    ap.add_argument("--rounds", type=int, default=20)
     ap.add_argument("--boxes", type=int, default=10_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000)
     ap.add_argument("--requests", type=int, default=300)
@@ -419,7 +520,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    out = run_served(args) if args.workload == "served" else run_c3(args)
+    out = {"served": run_served, "c3": run_c3, "c4": run_c4}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -5,7 +5,8 @@
 //
 // Build policy: Insert appends boxes by id and marks the tree stale; the next
 // query (or rts_index_build) rebuilds it once, however many Insert calls came
-// before. Update and Delete write the boxes in place and refit; under
+// before. Update and Delete write the boxes in place and mark the tree for a
+// refit, which runs once at the next query (or rts_index_build); under
 // RTS_REBUILD_SAH a refit whose tree cost exceeds sah_rebuild_ratio times the
 // cost measured right after the last build marks the tree stale instead of
 // letting it degrade (the reference never rebuilds, spatial_index.cuh:208-336).
@@ -196,18 +197,28 @@ int build_tree(rts_index* ix, hipStream_t stream) {
   phase_end(ph, stream);
   ix->n_tree = N;
   ix->stale = false;
+  ix->refit_needed = false;
   ix->builds++;
   ix->build_epoch++;
   if (ix->cfg.rebuild_policy == RTS_REBUILD_SAH) return measure_cost(ix, true, stream);
   return RTS_OK;
 }
 
-int after_mutation(rts_index* ix, hipStream_t stream) {
+// Update and Delete only mark the tree: the refit runs once, at the next
+// query or rts_index_build, however many mutations came before it, and not at
+// all when an Insert in between makes the tree stale anyway.
+int after_mutation(rts_index* ix) {
   if (ix->stale) return RTS_OK;  // the next query rebuilds anyway
   if (ix->cfg.rebuild_policy == RTS_REBUILD_ALWAYS) {
     ix->stale = true;
     return RTS_OK;
   }
+  ix->refit_needed = true;
+  return RTS_OK;
+}
+
+int refit_tree(rts_index* ix, hipStream_t stream) {
+  ix->refit_needed = false;
   PhaseToken ph = phase_begin(RTS_PHASE_REFIT, stream);
   RTS_HIP(launch_refit(refit_args(ix, (uint32_t)ix->n_tree), stream));
   phase_end(ph, stream);
@@ -253,6 +264,7 @@ TreeView tree_view(const rts_index* ix) {
 int ensure_built(rts_index* ix, hipStream_t stream) {
   poll_cost(ix);
   if (ix->stale) return build_tree(ix, stream);
+  if (ix->refit_needed) return refit_tree(ix, stream);
   return RTS_OK;
 }
 
@@ -349,6 +361,7 @@ int rts_index_clear(rts_index* ix) {
   ix->n = 0;
   ix->n_tree = 0;
   ix->stale = true;
+  ix->refit_needed = false;
   ix->cost_build_pending = ix->cost_refit_pending = false;
   ix->sah_at_build = ix->sah_current = 0;
   return RTS_OK;
@@ -381,7 +394,7 @@ int rts_index_delete(rts_index* ix, const uint64_t* ids, uint64_t n, rts_stream_
   RTS_HIP(launch_delete(ids, n, ix->boxes, ix->n, ix->leaf_boxes, ix->leaf_of_id,
                         ix->stale ? 0 : ix->n_tree, stream));
   phase_end(ph, stream);
-  return after_mutation(ix, stream);
+  return after_mutation(ix);
 }
 
 int rts_index_update(rts_index* ix, const rts_update* ups, uint64_t n, rts_stream_t stream) {
@@ -396,7 +409,7 @@ int rts_index_update(rts_index* ix, const rts_update* ups, uint64_t n, rts_strea
   RTS_HIP(launch_update(ups, n, ix->boxes, ix->n, ix->leaf_boxes, ix->leaf_of_id,
                         ix->stale ? 0 : ix->n_tree, stream));
   phase_end(ph, stream);
-  return after_mutation(ix, stream);
+  return after_mutation(ix);
 }
 
 }  // extern "C"

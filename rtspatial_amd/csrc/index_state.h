@@ -24,6 +24,7 @@ struct rts_index {
   uint64_t tcap = 0;
   uint64_t n_tree = 0;
   bool stale = true;
+  bool refit_needed = false;  // boxes changed in place since the last refit/build
   float4* leaf_boxes = nullptr;
   uint32_t* leaf_ids = nullptr;
   uint32_t* keys = nullptr;

@@ -350,3 +350,42 @@ def test_radix_sort_matches_stable_argsort(n, bits):
     assert np.array_equal(v.cpu().numpy().view(np.uint32), vals[order])
     k2, v2 = R.sort_pairs(dev(keys.view(np.int32)), None, key_bits=bits)
     assert np.array_equal(v2.cpu().numpy().view(np.uint32), order.astype(np.uint32))
+
+
+@pytest.mark.parametrize("policy", ["refit", "sah", "rebuild"])
+def test_mutable_rounds_match_oracle_every_round(oracle, policy):
+    # The mutable-index workload at 1/10 scale: 1M uniform boxes (sides
+    # U[0, 1e-3]); each round deletes 100k live ids, inserts 100k fresh boxes,
+    # shifts 100k other live boxes by U[-1e-3, 1e-3], then answers 100k
+    # uniform points. Every policy must give the oracle's pair set each round.
+    from rtspatial_amd import workloads as W
+    pol = {"refit": R._native.RTS_REBUILD_NEVER, "sah": R._native.RTS_REBUILD_SAH,
+           "rebuild": R._native.RTS_REBUILD_ALWAYS}[policy]
+    log(f"mutable rounds policy={policy}")
+    m = W.MutationRounds(1_000_000, 100_000, 100_000)
+    ix = make_index(m.initial, rebuild_policy=pol)
+    for rnd in range(5):
+        del_ids, ins, upd_ids, upd, pts = m.next_round()
+        before = ix.stats()
+        ix.Delete(dev(del_ids))
+        if rnd < 3:
+            ix.Insert(dev(ins))
+        else:
+            # Rounds without inserts: the tree is refitted (or rebuilt under
+            # the rebuild policy) instead of rebuilt for new ids.
+            m.alive[-ins.shape[0]:] = False
+        ix.Update((dev(upd_ids), dev(upd)))
+        got, n = collect(ix, P.kContains, pts)
+        want = oracle.RTree(m.boxes, alive=m.alive.astype(np.uint8)).pairs(oracle.CONTAINS, pts)
+        assert n == want.shape[0], (rnd, n, want.shape[0])
+        assert np.array_equal(got, want), rnd
+        after = ix.stats()
+        if rnd < 3:
+            # One rebuild per round, and no refit wasted before it.
+            assert after["builds"] == before["builds"] + 1
+            assert after["refits"] == before["refits"]
+        elif policy == "refit":
+            assert after["refits"] == before["refits"] + 1 and after["builds"] == before["builds"]
+        elif policy == "rebuild":
+            assert after["builds"] == before["builds"] + 1
+    ix.Sync()
