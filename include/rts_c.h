@@ -198,6 +198,39 @@ typedef struct rts_index_stats {
 
 int rts_index_get_stats(rts_index* index, rts_index_stats* out);
 
+/* Autotuner. No reference counterpart: it stands where the reference's
+ * CalculateBestParallelism sampling (spatial_index.cuh:41-57,690-760) picks a
+ * ray parallelism. It times the query over an evenly strided sample of
+ * `queries` under every candidate plan -- leaf size, one query per lane or
+ * per wavefront, Morton-ordered or input-ordered queries -- with the given
+ * handler. With `apply` set, it stores the fastest plan in the index for this
+ * (predicate, query kind); later queries of that kind use it for every
+ * setting the config leaves on auto. Results are identical under every plan;
+ * only the time differs. `queries` is rts_point* when `points` is 1, else
+ * rts_envelope*, in device or pinned host memory. Synchronous. */
+typedef struct rts_tune_config {
+  uint32_t sample;      /* queries probed (32768; all of them if fewer) */
+  uint32_t leaf_sizes;  /* bit k set: try leaf size 2^k (bits 2..5: 4, 8, 16, 32) */
+  uint32_t reps;        /* timed runs per plan; the fastest counts (3) */
+  int32_t apply;        /* store the winner in the index (1) */
+} rts_tune_config;
+
+typedef struct rts_tune_result {
+  uint32_t leaf_size;      /* the winner */
+  uint32_t traversal;      /* RTS_TRAVERSAL_THREAD or RTS_TRAVERSAL_WAVE */
+  uint32_t sort_queries;   /* 1 = Morton-ordered, 2 = input order */
+  uint32_t plans;          /* plans timed */
+  uint64_t sample;         /* queries in the sample */
+  uint64_t sample_results; /* results of the sample */
+  double best_ms;          /* the winner's time over the sample */
+  double default_ms;       /* the untuned plan's time over the sample */
+} rts_tune_result;
+
+void rts_tune_config_default(rts_tune_config* cfg);
+int rts_index_autotune(rts_index* index, rts_predicate pred, const void* queries, uint64_t n,
+                       int points, rts_handler handler, const rts_tune_config* cfg,
+                       rts_tune_result* out, rts_stream_t stream);
+
 /* Standalone LDS-staged onesweep radix sort of (uint32 key, uint32 value)
  * pairs on bits [0, key_bits) of the key, stable. `vals_in` NULL means values
  * 0..n-1. Outputs must not alias inputs. Scratch is allocated on `stream`. */

@@ -47,6 +47,28 @@ class rts_dev_queue(ctypes.Structure):
     ]
 
 
+class rts_tune_config(ctypes.Structure):
+    _fields_ = [
+        ("sample", ctypes.c_uint32),
+        ("leaf_sizes", ctypes.c_uint32),
+        ("reps", ctypes.c_uint32),
+        ("apply", ctypes.c_int32),
+    ]
+
+
+class rts_tune_result(ctypes.Structure):
+    _fields_ = [
+        ("leaf_size", ctypes.c_uint32),
+        ("traversal", ctypes.c_uint32),
+        ("sort_queries", ctypes.c_uint32),
+        ("plans", ctypes.c_uint32),
+        ("sample", ctypes.c_uint64),
+        ("sample_results", ctypes.c_uint64),
+        ("best_ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+    ]
+
+
 class rts_index_stats(ctypes.Structure):
     _fields_ = [
         ("n_boxes", ctypes.c_uint64),
@@ -141,6 +163,9 @@ SIGNATURES = [
     ("rts_index_sync", _I32, [_VP, _VP]),
     ("rts_index_get_stats", _I32, [_VP, ctypes.POINTER(rts_index_stats)]),
     ("rts_sort_pairs", _I32, [_VP, _VP, _VP, _VP, _U64, _I32, _VP]),
+    ("rts_tune_config_default", None, [ctypes.POINTER(rts_tune_config)]),
+    ("rts_index_autotune", _I32, [_VP, _I32, _VP, _U64, _I32, _I32, ctypes.POINTER(rts_tune_config),
+                                  ctypes.POINTER(rts_tune_result), _VP]),
     ("rts_serve_config_default", None, [ctypes.POINTER(rts_serve_config)]),
     ("rts_serve_run", _I32, [_VP, ctypes.POINTER(rts_serve_config), ctypes.POINTER(rts_request),
                              _U64, ctypes.POINTER(rts_update_batch), _U64,

@@ -358,6 +358,27 @@ hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids,
   return hipGetLastError();
 }
 
+// Every stride-th element of `src` (8-byte points or 16-byte envelopes) into
+// dst[0, m): the autotuner's query sample.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_gather_strided(const T* __restrict__ src, uint64_t stride,
+                                                             T* __restrict__ dst, uint32_t m) {
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads)
+    dst[i] = src[(uint64_t)i * stride];
+}
+
+hipError_t launch_gather_strided(const void* src, bool points, uint64_t stride, void* dst, uint32_t m,
+                                 hipStream_t stream) {
+  if (m == 0) return hipSuccess;
+  if (points)
+    hipLaunchKernelGGL(k_gather_strided<float2>, dim3(grid_for(m)), dim3(kThreads), 0, stream,
+                       static_cast<const float2*>(src), stride, static_cast<float2*>(dst), m);
+  else
+    hipLaunchKernelGGL(k_gather_strided<float4>, dim3(grid_for(m)), dim3(kThreads), 0, stream,
+                       static_cast<const float4*>(src), stride, static_cast<float4*>(dst), m);
+  return hipGetLastError();
+}
+
 hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes, uint32_t* parent,
                          uint32_t* flags, hipStream_t stream) {
   if (n < 2) return hipSuccess;

@@ -362,6 +362,8 @@ int rts_index_clear(rts_index* ix) {
   ix->n_tree = 0;
   ix->stale = true;
   ix->refit_needed = false;
+  for (auto& row : ix->tuned)
+    for (auto& t : row) t = rts_index::TunedPlan{};
   ix->cost_build_pending = ix->cost_refit_pending = false;
   ix->sah_at_build = ix->sah_current = 0;
   return RTS_OK;
@@ -487,33 +489,44 @@ int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, 
   return RTS_OK;
 }
 
-// AUTO: one query per wavefront where queries have many results (envelope
-// intersects / within), one per lane otherwise (points, envelope contains).
-int resolve_traversal(const rts_index* ix, int pred, bool points) {
+// How one query call runs: traversal mode, query ordering, leaf-range size.
+struct Plan {
+  int traversal;
+  bool sort;
+  uint32_t leaf_size;
+};
+
+// The plan of a query call. Explicit config settings win; a plan stored by
+// rts_index_autotune fills the settings left on auto; otherwise: one query
+// per wavefront where queries have many results (envelope intersects /
+// within), one per lane otherwise (points, envelope contains), and Morton
+// ordering only for per-lane traversal of large batches (neighbouring lanes
+// then share their descent; wave traversal gains nothing from it).
+Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
+  const rts_index::TunedPlan& t = ix->tuned[pred][points ? 1 : 0];
+  Plan p;
   if (ix->cfg.traversal == RTS_TRAVERSAL_THREAD || ix->cfg.traversal == RTS_TRAVERSAL_WAVE)
-    return (int)ix->cfg.traversal;
-  if (points || pred == RTS_CONTAINS) return RTS_TRAVERSAL_THREAD;
-  return RTS_TRAVERSAL_WAVE;
+    p.traversal = (int)ix->cfg.traversal;
+  else if (t.valid)
+    p.traversal = (int)t.traversal;
+  else
+    p.traversal = (points || pred == RTS_CONTAINS) ? RTS_TRAVERSAL_THREAD : RTS_TRAVERSAL_WAVE;
+  if (ix->cfg.sort_queries == 1 || ix->cfg.sort_queries == 2)
+    p.sort = ix->cfg.sort_queries == 1;
+  else if (t.valid)
+    p.sort = t.sort;
+  else
+    p.sort = p.traversal == RTS_TRAVERSAL_THREAD && nq >= 65536;
+  p.leaf_size = t.valid ? t.leaf_size : ix->cfg.leaf_size;
+  return p;
 }
 
-int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool points,
-              rts_handler handler, void* arg, hipStream_t stream) {
-  RTS_CHECK_ARG(handler == RTS_HANDLER_COUNT || handler == RTS_HANDLER_COLLECT,
-                "query: unknown handler");
-  RTS_CHECK_ARG(arg, "query: null handler argument");
-  RTS_CHECK_ARG(pred >= RTS_INTERSECTS && pred <= RTS_WITHIN, "query: unknown predicate");
-  RTS_CHECK_ARG(!points || pred == RTS_CONTAINS,
-                "query: point queries support only RTS_CONTAINS (spatial_index.cuh:385-389)");
-  if (nq == 0) return RTS_OK;
-  RTS_CHECK_ARG(queries, "query: null queries");
-  std::lock_guard<std::mutex> lk(ix->mu);
-  RTS_HIP(hipSetDevice(ix->device));
-  if (ix->n == 0) return RTS_OK;  // spatial_index.cuh:354-356
-  int rc = ensure_built(ix, stream);
-  if (rc) return rc;
-  const uint32_t mode = ix->cfg.sort_queries;
-  const bool sort = (mode == 1) || (mode == 0 && nq >= 65536);
-  const int trav = resolve_traversal(ix, pred, points);
+// Runs one query call under `plan`; the caller holds the index lock and has
+// brought the tree up to date.
+int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uint64_t nq,
+             bool points, rts_handler handler, void* arg, hipStream_t stream) {
+  const bool sort = plan.sort;
+  const int trav = plan.traversal;
   const bool two_pass = trav == RTS_TRAVERSAL_WAVE && handler == RTS_HANDLER_COLLECT;
   const uint64_t kChunk = 1ull << 30;
   for (uint64_t off = 0; off < nq; off += kChunk) {
@@ -522,7 +535,7 @@ int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool po
                             : static_cast<const void*>(static_cast<const float4*>(queries) + off);
     QueryScratch qs;
     size_t o_keys = 0, o_sort = 0;
-    rc = alloc_scratch(m, sort, two_pass, stream, &qs, &o_keys, &o_sort);
+    int rc = alloc_scratch(m, sort, two_pass, stream, &qs, &o_keys, &o_sort);
     if (rc) return rc;
     if (sort) {
       PhaseToken ph = phase_begin(RTS_PHASE_QUERY_ORDER, stream);
@@ -532,6 +545,7 @@ int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool po
     }
     QueryLaunch ql;
     ql.tree = tree_view(ix);
+    ql.tree.leaf_size = plan.leaf_size;
     ql.queries = qp;
     ql.n = (uint32_t)m;
     ql.qid_base = (uint32_t)off;
@@ -565,6 +579,46 @@ int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool po
   }
   return RTS_OK;
 }
+
+int check_query_args(int pred, const void* queries, uint64_t nq, bool points, rts_handler handler,
+                     const void* arg) {
+  RTS_CHECK_ARG(handler == RTS_HANDLER_COUNT || handler == RTS_HANDLER_COLLECT,
+                "query: unknown handler");
+  RTS_CHECK_ARG(arg, "query: null handler argument");
+  RTS_CHECK_ARG(pred >= RTS_INTERSECTS && pred <= RTS_WITHIN, "query: unknown predicate");
+  RTS_CHECK_ARG(!points || pred == RTS_CONTAINS,
+                "query: point queries support only RTS_CONTAINS (spatial_index.cuh:385-389)");
+  RTS_CHECK_ARG(nq == 0 || queries, "query: null queries");
+  return RTS_OK;
+}
+
+int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool points,
+              rts_handler handler, void* arg, hipStream_t stream) {
+  int rc = check_query_args(pred, queries, nq, points, handler, arg);
+  if (rc || nq == 0) return rc;
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  if (ix->n == 0) return RTS_OK;  // spatial_index.cuh:354-356
+  rc = ensure_built(ix, stream);
+  if (rc) return rc;
+  return run_plan(ix, plan_query(ix, pred, points, nq), pred, queries, nq, points, handler, arg,
+                  stream);
+}
+
+// Device buffers of one tuning run, released on every exit path.
+struct TuneBuffers {
+  void* sample = nullptr;
+  rts_pair* pairs = nullptr;
+  char* small = nullptr;  // [0, 24): rts_dev_queue, [32, 40): counter
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~TuneBuffers() {
+    if (sample) (void)hipFree(sample);
+    if (pairs) (void)hipFree(pairs);
+    if (small) (void)hipFree(small);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
 
 }  // namespace
 
@@ -648,6 +702,137 @@ int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* k
     set_error("rts_sort_pairs: look-back exceeded its wait bound");
     return RTS_ERR_DEVICE;
   }
+  return RTS_OK;
+}
+
+void rts_tune_config_default(rts_tune_config* c) {
+  if (!c) return;
+  c->sample = 32768;
+  c->leaf_sizes = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5);
+  c->reps = 3;
+  c->apply = 1;
+}
+
+int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, uint64_t n,
+                       int points, rts_handler handler, const rts_tune_config* cfg_in,
+                       rts_tune_result* out, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_autotune: null index");
+  RTS_CHECK_ARG(out, "rts_index_autotune: null result");
+  std::memset(out, 0, sizeof(*out));
+  rts_tune_config cfg;
+  rts_tune_config_default(&cfg);
+  if (cfg_in) cfg = *cfg_in;
+  RTS_CHECK_ARG(cfg.sample > 0 && cfg.reps > 0, "rts_index_autotune: sample and reps must be > 0");
+  RTS_CHECK_ARG((cfg.leaf_sizes & 0x7Eu) != 0,
+                "rts_index_autotune: leaf_sizes needs a bit in 1..6 (leaf sizes 2..64)");
+  const bool pts = points != 0;
+  int rc = check_query_args(pred, queries, n, pts, handler, ix);
+  if (rc || n == 0) return rc;
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  if (ix->n == 0) return RTS_OK;
+  rc = ensure_built(ix, stream);
+  if (rc) return rc;
+
+  // An evenly strided sample, shrunk until its results fit a 2^30-pair queue.
+  const size_t qbytes = pts ? sizeof(float2) : sizeof(float4);
+  const uint64_t kMaxPairs = 1ull << 30;
+  TuneBuffers tb;
+  RTS_HIP(hipMalloc(&tb.small, 64));
+  RTS_HIP(hipEventCreate(&tb.ev[0]));
+  RTS_HIP(hipEventCreate(&tb.ev[1]));
+  unsigned long long* counter = reinterpret_cast<unsigned long long*>(tb.small + 32);
+  const Plan base = plan_query(ix, pred, pts, n);
+  uint64_t m = std::min<uint64_t>(cfg.sample, n);
+  uint64_t total = 0;
+  for (int attempt = 0;; ++attempt) {
+    if (tb.sample) RTS_HIP(hipFree(tb.sample));
+    tb.sample = nullptr;
+    RTS_HIP(hipMalloc(&tb.sample, m * qbytes));
+    RTS_HIP(launch_gather_strided(queries, pts, n / m, tb.sample, (uint32_t)m, stream));
+    RTS_HIP(hipMemsetAsync(counter, 0, sizeof(*counter), stream));
+    rc = run_plan(ix, base, pred, tb.sample, m, pts, RTS_HANDLER_COUNT, counter, stream);
+    if (rc) return rc;
+    RTS_HIP(hipMemcpyAsync(&total, counter, sizeof(total), hipMemcpyDeviceToHost, stream));
+    RTS_HIP(hipStreamSynchronize(stream));
+    if (handler == RTS_HANDLER_COUNT || total <= kMaxPairs || m <= 64 || attempt == 3) break;
+    m = std::max<uint64_t>(64, (uint64_t)((double)m * (double)kMaxPairs / (double)total * 0.9));
+  }
+  void* arg = counter;
+  if (handler == RTS_HANDLER_COLLECT) {
+    const uint64_t cap = std::min(total, kMaxPairs) + 1;
+    RTS_HIP(hipMalloc(reinterpret_cast<void**>(&tb.pairs), cap * sizeof(rts_pair)));
+    rts_dev_queue q{tb.pairs, cap, counter};
+    RTS_HIP(hipMemcpyAsync(tb.small, &q, sizeof(q), hipMemcpyHostToDevice, stream));
+    arg = tb.small;
+  }
+
+  // Time one plan: one untimed run, then the fastest of `reps` runs. A plan
+  // whose untimed run is already 3x slower than the best so far is not
+  // repeated.
+  double best = 1e300;
+  auto time_plan = [&](const Plan& p, double* ms_out) -> int {
+    float ms = 0.0f;
+    double fastest = 1e300;
+    for (uint32_t r = 0; r <= cfg.reps; ++r) {
+      RTS_HIP(hipMemsetAsync(counter, 0, sizeof(*counter), stream));
+      RTS_HIP(hipEventRecord(tb.ev[0], stream));
+      int e = run_plan(ix, p, pred, tb.sample, m, pts, handler, arg, stream);
+      if (e) return e;
+      RTS_HIP(hipEventRecord(tb.ev[1], stream));
+      RTS_HIP(hipEventSynchronize(tb.ev[1]));
+      RTS_HIP(hipEventElapsedTime(&ms, tb.ev[0], tb.ev[1]));
+      if (r == 0 && ms > 3.0 * best) {
+        fastest = ms;
+        break;
+      }
+      if (r > 0) fastest = std::min(fastest, (double)ms);
+    }
+    *ms_out = fastest;
+    return RTS_OK;
+  };
+
+  double default_ms = 0;
+  rc = time_plan(base, &default_ms);
+  if (rc) return rc;
+  best = default_ms;
+  Plan winner = base;
+  uint32_t plans = 1;
+  for (uint32_t k = 1; k <= 6; ++k) {
+    if (!(cfg.leaf_sizes & (1u << k))) continue;
+    for (int trav : {RTS_TRAVERSAL_THREAD, RTS_TRAVERSAL_WAVE}) {
+      for (bool sort : {false, true}) {
+        Plan p{trav, sort, 1u << k};
+        if (p.traversal == base.traversal && p.sort == base.sort && p.leaf_size == base.leaf_size)
+          continue;
+        double ms = 0;
+        rc = time_plan(p, &ms);
+        if (rc) return rc;
+        ++plans;
+        if (ms < best) {
+          best = ms;
+          winner = p;
+        }
+      }
+    }
+  }
+  rc = check_device_error(ix);
+  if (rc) return rc;
+  if (cfg.apply) {
+    rts_index::TunedPlan& t = ix->tuned[pred][pts ? 1 : 0];
+    t.valid = true;
+    t.traversal = (uint32_t)winner.traversal;
+    t.sort = winner.sort;
+    t.leaf_size = winner.leaf_size;
+  }
+  out->leaf_size = winner.leaf_size;
+  out->traversal = (uint32_t)winner.traversal;
+  out->sort_queries = winner.sort ? 1 : 2;
+  out->plans = plans;
+  out->sample = m;
+  out->sample_results = total;
+  out->best_ms = best;
+  out->default_ms = default_ms;
   return RTS_OK;
 }
 

@@ -59,6 +59,15 @@ struct rts_index {
 
   uint64_t builds = 0;
   uint64_t refits = 0;
+
+  // Plans stored by rts_index_autotune, per [predicate][points].
+  struct TunedPlan {
+    bool valid = false;
+    uint32_t traversal = 0;
+    bool sort = false;
+    uint32_t leaf_size = 0;
+  };
+  TunedPlan tuned[3][2];
 };
 
 namespace rts {

@@ -105,6 +105,8 @@ hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* sce
                                 uint32_t* keys, hipStream_t stream);
 hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids, uint32_t n,
                                 float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream);
+hipError_t launch_gather_strided(const void* src, bool points, uint64_t stride, void* dst, uint32_t m,
+                                 hipStream_t stream);
 hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes,
                          uint32_t* parent, uint32_t* flags, hipStream_t stream);
 struct RefitArgs {

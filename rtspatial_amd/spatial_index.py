@@ -255,6 +255,29 @@ class SpatialIndex:
     def Sync(self, stream=None):
         N.check(N.lib().rts_index_sync(self._handle(), _stream_handle(stream)), "Sync")
 
+    def Autotune(self, predicate, queries, collect=True, sample=32768, leaf_sizes=(4, 8, 16, 32),
+                 reps=3, apply=True, stream=None):
+        """Times the query over a strided sample of `queries` under every
+        plan (leaf size, per-lane or per-wavefront traversal, Morton or input
+        order) and, with `apply`, keeps the fastest for later queries of this
+        predicate and query kind. Returns the choice and the sample timings."""
+        _check_array(queries, None, torch.float32, "Autotune")
+        if queries.dim() != 2 or queries.shape[1] not in (2, 4):
+            raise ValueError("Autotune: queries must be [n, 2] points or [n, 4] envelopes")
+        c = N.rts_tune_config()
+        N.lib().rts_tune_config_default(ctypes.byref(c))
+        c.sample = int(sample)
+        c.leaf_sizes = sum(1 << (int(ls).bit_length() - 1) for ls in leaf_sizes)
+        c.reps = int(reps)
+        c.apply = 1 if apply else 0
+        r = N.rts_tune_result()
+        handler = N.RTS_HANDLER_COLLECT if collect else N.RTS_HANDLER_COUNT
+        N.check(N.lib().rts_index_autotune(self._handle(), int(predicate), queries.data_ptr(),
+                                           queries.shape[0], 1 if queries.shape[1] == 2 else 0,
+                                           handler, ctypes.byref(c), ctypes.byref(r),
+                                           _stream_handle(stream)), "Autotune")
+        return {f: getattr(r, f) for f, _ in N.rts_tune_result._fields_}
+
     def stats(self):
         s = N.rts_index_stats()
         N.check(N.lib().rts_index_get_stats(self._handle(), ctypes.byref(s)), "stats")

@@ -389,3 +389,32 @@ def test_mutable_rounds_match_oracle_every_round(oracle, policy):
         elif policy == "rebuild":
             assert after["builds"] == before["builds"] + 1
     ix.Sync()
+
+
+@pytest.mark.parametrize("kind", ["points", "intersects", "within"])
+def test_autotune_keeps_results_exact(oracle, kind):
+    # Every plan the tuner times must give the same results; after tuning,
+    # queries run under the stored plan and still equal the oracle's.
+    from rtspatial_amd import workloads as W
+    log(f"autotune {kind}")
+    rng = np.random.default_rng(21)
+    means = W.mixture_means(rng)
+    b = W.mixture_boxes(rng, means, 200_000, np.log(1e-3))
+    if kind == "points":
+        q, pred = W.mixture_boxes(rng, means, 30_000, np.log(1e-3))[:, :2].copy(), P.kContains
+    else:
+        q = W.mixture_boxes(rng, means, 30_000, np.log(5e-3))
+        pred = P.kIntersects if kind == "intersects" else P.kWithin
+    ix = make_index(b)
+    want = oracle.RTree(b).pairs(int(pred), q)
+    dq = dev(q)
+    r = ix.Autotune(pred, dq, sample=4096, reps=2)
+    assert r["sample"] == 4096 and r["plans"] == 16
+    assert r["leaf_size"] in (4, 8, 16, 32) and r["traversal"] in (THREAD, WAVE)
+    assert r["sort_queries"] in (1, 2) and 0 < r["best_ms"] <= r["default_ms"]
+    sample_want = oracle.RTree(b).count(int(pred), q[:: q.shape[0] // 4096][:4096])
+    assert r["sample_results"] == sample_want
+    got, n = collect(ix, pred, dq)
+    assert n == want.shape[0] and np.array_equal(got, want)
+    assert count(ix, pred, dq) == want.shape[0]
+    ix.Sync()
