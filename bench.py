@@ -497,12 +497,31 @@ def run_c4(args):
     }
 
 
+def run_copy_calibration(args):
+    """A plain 1 GiB device-to-device copy, three times: the known-size
+    kernel whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the
+    index's kernels to bytes."""
+    import torch
+
+    n = 1 << 30
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    return {"metric": "device copy calibration", "value": round(2 * n / dt / 1e9, 2), "unit": "GB/s",
+            "n_gpus": 1, "bytes_read_per_copy": n, "bytes_written_per_copy": n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["served", "c3", "c4"], default="served")
+    ap.add_argument("--workload", choices=["served", "c3", "c4", "copy"], default="served")
     ap.add_argument("--rounds", type=int, default=20)
     ap.add_argument("--boxes", type=int, default=10_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -520,7 +539,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    out = {"served": run_served, "c3": run_c3, "c4": run_c4}[args.workload](args)
+    out = {"served": run_served, "c3": run_c3, "c4": run_c4,
+           "copy": run_copy_calibration}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

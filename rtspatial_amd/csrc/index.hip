@@ -528,7 +528,9 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
   const bool sort = plan.sort;
   const int trav = plan.traversal;
   const bool two_pass = trav == RTS_TRAVERSAL_WAVE && handler == RTS_HANDLER_COLLECT;
-  const uint64_t kChunk = 1ull << 30;
+  // At most 2^26 queries per launch: one wavefront per query keeps
+  // grid x workgroup size below 2^32 work-items.
+  const uint64_t kChunk = 1ull << 26;
   for (uint64_t off = 0; off < nq; off += kChunk) {
     const uint64_t m = std::min(kChunk, nq - off);
     const void* qp = points ? static_cast<const void*>(static_cast<const float2*>(queries) + off)

@@ -204,22 +204,24 @@ __device__ __forceinline__ uint32_t xcd_slice_block(uint32_t b, uint32_t grid) {
   return (b % kXcds) * per + b / kXcds;
 }
 
+// One wavefront per workgroup: query costs are heavy-tailed, and a workgroup
+// holds its LDS until its last wave ends, so waves packed together would keep
+// the LDS of finished queries from new ones.
 template <int PRED, bool POINTS, int MODE>
-__global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, const WaveArgs a) {
-  __shared__ WaveLds s_lds[kWaves];
+__global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const WaveArgs a) {
+  __shared__ WaveLds s_lds;
   const TreeView t = q.tree;
-  const uint32_t w = threadIdx.x / kWave;
-  const uint32_t lin = xcd_slice_block(blockIdx.x, gridDim.x) * kWaves + w;
+  const uint32_t lin = xcd_slice_block(blockIdx.x, gridDim.x);
   if (lin >= q.n || t.n == 0) return;  // wave-uniform
   const uint32_t qi = q.order ? q.order[lin] : lin;
   const float4 qb = load_query(q.queries, POINTS, qi);
   if (MODE == kWaveWrite) {
     const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
     wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base, dq->data, dq->capacity,
-                                   a.base[0] + a.offsets[qi], s_lds[w]);
+                                   a.base[0] + a.offsets[qi], s_lds);
     return;
   }
-  const uint64_t all = wave_query<PRED, POINTS, false>(t, qb, 0, nullptr, 0, 0, s_lds[w]);
+  const uint64_t all = wave_query<PRED, POINTS, false>(t, qb, 0, nullptr, 0, 0, s_lds);
   if (lane_id() == 0) {
     if (MODE == kWaveCount) {
       a.counts[qi] = (uint32_t)all;
@@ -232,9 +234,8 @@ __global__ void __launch_bounds__(kThreads) k_query_wave(const QueryLaunch q, co
 
 template <int PRED, bool POINTS>
 hipError_t launch_wave(const QueryLaunch& q, const WaveArgs& a, int phase, hipStream_t stream) {
-  uint32_t blocks = (q.n + kWaves - 1) / kWaves;
-  blocks = (blocks + kXcds - 1) / kXcds * kXcds;  // whole rounds over the XCDs
-  const dim3 grid(blocks), block(kThreads);
+  const uint32_t blocks = (q.n + kXcds - 1) / kXcds * kXcds;  // whole rounds over the XCDs
+  const dim3 grid(blocks), block(kWave);
   switch (phase) {
     case kWaveCount:
       hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveCount>), grid, block, 0, stream, q, a);

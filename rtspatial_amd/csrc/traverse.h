@@ -197,15 +197,17 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
       lds_fence();
     } else {
       // ---- scan the queued leaf ranges, kScanStep leaves per step: lane L
-      // owns the kPerLane consecutive leaves from flat index base + L*kPerLane,
-      // so its ids come in two 16-byte loads and its matches go out as one
-      // contiguous run of pairs placed by a single wave prefix sum.
+      // owns the kPerLane consecutive leaves from flat index base + L*kPerLane.
+      // Its leaf ids are loaded together with the boxes it must test (not
+      // after the test), so a step waits for one memory round trip; ids of a
+      // lane inside one range come in two 16-byte loads, and its matches go
+      // out as one contiguous run of pairs placed by a single prefix sum.
       uint32_t e0 = 0;  // wave-uniform: entry holding flat index `base`
       for (uint32_t base = 0; base < total; base += kScanStep) {
         const uint32_t j0 = base + lane * kPerLane;
         uint32_t leaf[kPerLane];
-        uint32_t nhit = 0;
-        uint32_t hitmask = 0;
+        uint32_t gid[kPerLane];
+        uint32_t valid = 0, allm = 0;  // per-leaf bits
         bool contiguous = false;
         if (j0 < total) {
           // entry holding j0: binary search in [e0, m) unless the step lies in e0
@@ -229,18 +231,31 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
               hi_j = rpre[e + 1];
             }
             leaf[k] = (f & ~kAllMatch) + (j - lo_j);
-            bool h = false;
-            if (j < total) {
-              if (f & kAllMatch) {
-                h = true;
-              } else if (leaf[k] < t.n) {
-                h = P::leaf(t.leaf_boxes[leaf[k]], qb);
-              }
-            }
-            hitmask |= (h ? 1u : 0u) << k;
+            const bool ok = j < total && leaf[k] < t.n;
+            valid |= (ok ? 1u : 0u) << k;
+            allm |= ((ok && (f & kAllMatch)) ? 1u : 0u) << k;
           }
-          nhit = __popc(hitmask);
+          contiguous = contiguous && leaf[0] + kPerLane <= t.n;
+          if (WRITE) {
+            if (contiguous) {
+              // consecutive ids; the address is only 4-byte aligned
+              __builtin_memcpy(gid, t.leaf_ids + leaf[0], sizeof(gid));
+            } else {
+#pragma unroll
+              for (int k = 0; k < kPerLane; k++)
+                gid[k] = ((valid >> k) & 1u) ? t.leaf_ids[leaf[k]] : 0u;
+            }
+          }
         }
+        uint32_t hitmask = allm;
+        if (valid & ~allm) {
+#pragma unroll
+          for (int k = 0; k < kPerLane; k++) {
+            if (((valid & ~allm) >> k) & 1u)
+              hitmask |= (P::leaf(t.leaf_boxes[leaf[k]], qb) ? 1u : 0u) << k;
+          }
+        }
+        const uint32_t nhit = __popc(hitmask);
         // e0 for the next step: the entry of the step's last leaf
         {
           const uint32_t jl = base + kScanStep - 1 < total ? base + kScanStep - 1 : total - 1;
@@ -254,15 +269,6 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
         const uint32_t inc = wave_inclusive_sum(nhit);
         const uint32_t step_hits = readlane_u32(inc, kWave - 1);
         if (WRITE && nhit) {
-          uint32_t gid[kPerLane];
-          if (contiguous && leaf[0] + kPerLane <= t.n) {
-            // consecutive ids; the address is only 4-byte aligned
-            __builtin_memcpy(gid, t.leaf_ids + leaf[0], sizeof(gid));
-          } else {
-#pragma unroll
-            for (int k = 0; k < kPerLane; k++)
-              gid[k] = ((hitmask >> k) & 1u) && leaf[k] < t.n ? t.leaf_ids[leaf[k]] : 0u;
-          }
           uint64_t slot = obase + emitted + (inc - nhit);
           if (hitmask == (1u << kPerLane) - 1 && slot + kPerLane <= cap) {
             // all kPerLane matched: one contiguous run of pairs (8-byte aligned)
