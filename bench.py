@@ -233,6 +233,7 @@ def run_served(args):
         best = sweep[0]
 
     # Bit-exact check on the first requests: per-request pair hashes of both paths.
+    log(f"hash check of both paths on the first {args.verify} requests")
     reset_index()
     reqs_v, _ = trace(0.0, with_updates=False)
     reqs_v = reqs_v[: args.verify]

@@ -296,12 +296,17 @@ typedef struct rts_serve_config {
   uint32_t slots;              /* batches in flight (batched mode) (3) */
   uint32_t max_batch_requests; /* request chunks per batch (4096) */
   uint32_t use_graphs;         /* replay each slot's launches as a hipGraph (0: off by
-                                  default; see the DESIGN notes on graph replay) */
+                                  default, because a replay run did not complete on the
+                                  MI355X and the cause is not established) */
   uint64_t max_batch_queries;  /* queries per batch (4M) */
   uint64_t chunk_queries;      /* largest chunk of one request (256K) */
   uint64_t arena_pairs;        /* result arena per slot (or of the plain path), pairs */
   uint32_t hash_results;       /* compute each request's pair hash (verification runs) */
   uint32_t traversal;          /* rts_traversal for batched tiles: AUTO picks by predicate */
+  uint32_t head_share_pct;     /* while later requests wait, the largest share of a batch's
+                                  remaining result budget one request may take, in percent
+                                  (50); requests that do not fit are skipped for later ones */
+  uint32_t reserved;           /* 0 */
 } rts_serve_config;
 
 typedef struct rts_serve_stats {

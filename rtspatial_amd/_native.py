@@ -121,6 +121,8 @@ class rts_serve_config(ctypes.Structure):
         ("arena_pairs", ctypes.c_uint64),
         ("hash_results", ctypes.c_uint32),
         ("traversal", ctypes.c_uint32),
+        ("head_share_pct", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

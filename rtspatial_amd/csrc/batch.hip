@@ -268,6 +268,22 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t x) {
 // hashes[c] = sum over the pairs of range c (ranges[2c] = first slot,
 // ranges[2c+1] = count; ranges tile [0, *total) in order) of
 // splitmix64((query_id << 32) | geom_id). hashes must be zero.
+// A wavefront takes blocks of kHashBlock consecutive slots (lanes interleaved,
+// so loads coalesce). Each lane keeps a running sum for its current range and
+// adds it to hashes[] only when its range changes, so a request's pairs cost
+// about one atomic per lane per block rather than one per pair; where the
+// whole wave ends a block in one range, one wave-reduced atomic suffices.
+constexpr uint32_t kHashBlock = 64 * 256;
+
+__device__ __forceinline__ uint32_t range_of(const uint64_t* ranges, uint32_t nr, uint64_t i) {
+  uint32_t a = 0, b = nr - 1;
+  while (a < b) {
+    const uint32_t mid = (a + b + 1) >> 1;
+    if (ranges[2 * mid] <= i) a = mid; else b = mid - 1;
+  }
+  return a;
+}
+
 __global__ void __launch_bounds__(kThreads) k_hash_ranges(const rts_pair* __restrict__ arena,
                                                           uint64_t cap,
                                                           const uint64_t* __restrict__ ranges,
@@ -276,17 +292,33 @@ __global__ void __launch_bounds__(kThreads) k_hash_ranges(const rts_pair* __rest
                                                           unsigned long long* __restrict__ hashes) {
   const uint32_t nr = *n_ranges;
   const uint64_t hi = *total < cap ? *total : cap;
-  if (nr == 0) return;
-  for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < hi;
-       i += (uint64_t)gridDim.x * kThreads) {
-    uint32_t a = 0, b = nr - 1;
-    while (a < b) {
-      const uint32_t mid = (a + b + 1) >> 1;
-      if (ranges[2 * mid] <= i) a = mid; else b = mid - 1;
+  if (nr == 0 || hi == 0) return;
+  const uint32_t lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * kWaves;
+  const uint64_t w0 = (uint64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  for (uint64_t b0 = w0 * kHashBlock; b0 < hi; b0 += waves * kHashBlock) {
+    const uint64_t b1 = b0 + kHashBlock < hi ? b0 + kHashBlock : hi;
+    uint32_t a = range_of(ranges, nr, b0 + lane < b1 ? b0 + lane : b0);
+    uint64_t end = ranges[2 * a] + ranges[2 * a + 1];
+    uint64_t h = 0;
+    for (uint64_t i = b0 + lane; i < b1; i += kWave) {
+      if (i >= end) {
+        if (h) atomicAdd(&hashes[a], (unsigned long long)h);
+        h = 0;
+        a = range_of(ranges, nr, i);
+        end = ranges[2 * a] + ranges[2 * a + 1];
+      }
+      if (i >= end) continue;  // a slot in no range (ranges tile [0, total): not expected)
+      const rts_pair p = arena[i];
+      h += splitmix(((uint64_t)p.query_id << 32) | p.geom_id);
     }
-    if (i >= ranges[2 * a] + ranges[2 * a + 1]) continue;
-    const rts_pair p = arena[i];
-    atomicAdd(&hashes[a], (unsigned long long)splitmix(((uint64_t)p.query_id << 32) | p.geom_id));
+    const uint32_t a0 = readlane_u32(a, 0);
+    if (__all(a == a0)) {
+      h = wave_sum_u64(h);
+      if (lane == 0 && h) atomicAdd(&hashes[a0], (unsigned long long)h);
+    } else if (h) {
+      atomicAdd(&hashes[a], (unsigned long long)h);
+    }
   }
 }
 

@@ -14,8 +14,12 @@
 // Admission: arrived requests are taken in arrival order and split into
 // chunks of at most chunk_queries queries; chunks join the batch while their
 // predicted results (an estimate of results per query for each predicate,
-// refreshed from every completed chunk) fit the slot's arena. A chunk whose
-// actual results overflow the arena is re-launched in a later batch.
+// refreshed from every completed chunk) fit the slot's arena. While later
+// requests wait, one request takes at most head_share_pct percent of the
+// remaining budget, and
+// requests that do not fit are skipped for later ones that do (backfill). A
+// chunk whose actual results overflow the arena is re-launched in a later
+// batch.
 //
 // Updates: once an update batch has arrived, no new request is admitted; when
 // the requests already started are fully admitted and their batches are done
@@ -169,9 +173,10 @@ void finish_chunk(Engine& E, const Chunk& c, uint64_t count, uint64_t hash, doub
 
 // Fills `out` with re-launches first, then chunks of arrived requests, while
 // they fit: at most max_req chunks, max_q queries and `budget` predicted
-// results. With `force`, one chunk is taken even if it does not fit.
+// results. With `force`, one chunk is taken even if it does not fit. With
+// `fifo` (the plain path), requests are taken strictly in arrival order.
 void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget, bool force,
-           std::vector<Chunk>& out) {
+           bool fifo, std::vector<Chunk>& out) {
   out.clear();
   uint64_t q_used = 0;
   double used = 0;
@@ -201,15 +206,24 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
     used += p;
     q_used += c.n;
   }
-  while (!E.queue.empty() && out.size() < max_req) {
-    const uint32_t ri = E.queue.front();
+  // Arrived requests in arrival order. Unless `fifo`, a request may take at
+  // most head_share_pct percent of the batch's remaining result budget while
+  // later requests wait behind it, and a request that does not fit is skipped rather than
+  // blocking the ones behind it (backfill): small requests that arrive during
+  // a large one run beside its chunks instead of after all of them.
+  for (size_t j = 0; j < E.queue.size() && out.size() < max_req;) {
+    const uint32_t ri = E.queue[j];
     const rts_request& r = E.reqs[ri];
     if (r.arrival_s > t) break;
     ReqState& s = E.st[ri];
-    if (E.hold_new && !s.admitted) break;
+    if (E.hold_new && !s.admitted) {
+      if (fifo) break;
+      ++j;
+      continue;
+    }
     const uint64_t left = r.n - s.next_q;
     if (left == 0) {  // an empty request completes on admission
-      E.queue.pop_front();
+      E.queue.erase(E.queue.begin() + j);
       if (!s.done) {
         s.admitted = s.done = true;
         E.res[ri].admit_s = E.res[ri].done_s = t;
@@ -218,14 +232,22 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
       }
       continue;
     }
+    const bool others =
+        !fifo && j + 1 < E.queue.size() && E.reqs[E.queue[j + 1]].arrival_s <= t;
     const double per = E.est[kind_of(r)].per_query * 1.5;
     uint64_t k = std::min<uint64_t>(left, E.cfg.chunk_queries);
     k = std::min<uint64_t>(k, max_q > q_used ? max_q - q_used : 0);
-    const double room = budget - used - 16.0;
+    const double room = (budget - used - 16.0) * (others ? E.cfg.head_share_pct / 100.0 : 1.0);
     if (per > 0) k = std::min<uint64_t>(k, (uint64_t)std::max(0.0, room / per));
     if (k == 0) {
-      if (force && out.empty()) k = 1;
-      else break;
+      if (force && out.empty()) {
+        k = 1;
+      } else if (fifo) {
+        break;
+      } else {
+        ++j;
+        continue;
+      }
     }
     const Chunk c{ri, s.next_q, k, kUnknown};
     out.push_back(c);
@@ -240,8 +262,10 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
     q_used += k;
     used += predicted(E, c);
     if (s.next_q >= r.n) {
-      E.queue.pop_front();
+      E.queue.erase(E.queue.begin() + j);
       E.partial--;
+    } else {
+      ++j;
     }
   }
 }
@@ -533,7 +557,7 @@ int run_batched(Engine& E) {
     for (auto& s : slots) {
       if (rc || s.busy) continue;
       admit(E, t, E.cfg.max_batch_requests, E.cfg.max_batch_queries, (double)E.cfg.arena_pairs,
-            true, chunks);
+            true, false, chunks);
       if (chunks.empty()) break;
       rc = launch_slot(E, s, chunks);
     }
@@ -615,7 +639,7 @@ int run_plain(Engine& E) {
       if (inflight.empty()) rc = reset_arena();
       continue;
     }
-    admit(E, t, 1, E.cfg.chunk_queries, (double)E.cfg.arena_pairs - used, used == 0, one);
+    admit(E, t, 1, E.cfg.chunk_queries, (double)E.cfg.arena_pairs - used, used == 0, true, one);
     if (one.empty()) {
       // Arrived work does not fit the rest of the arena: empty it once idle.
       const bool waiting = !E.queue.empty() && E.reqs[E.queue.front()].arrival_s <= t;
@@ -689,6 +713,7 @@ void rts_serve_config_default(rts_serve_config* c) {
   c->arena_pairs = 1ull << 30;
   c->hash_results = 0;
   c->traversal = RTS_TRAVERSAL_AUTO;
+  c->head_share_pct = 50;
 }
 
 int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request* reqs,
@@ -703,7 +728,8 @@ int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request*
   if (cfg) E.cfg = *cfg; else rts_serve_config_default(&E.cfg);
   if (E.cfg.max_batch_requests == 0 || E.cfg.max_batch_requests > 65536 ||
       E.cfg.max_batch_queries == 0 || E.cfg.max_batch_queries >= (1ull << 31) ||
-      E.cfg.chunk_queries == 0 || E.cfg.arena_pairs == 0) {
+      E.cfg.chunk_queries == 0 || E.cfg.arena_pairs == 0 || E.cfg.head_share_pct == 0 ||
+      E.cfg.head_share_pct > 100) {
     set_error("rts_serve_run: invalid configuration");
     return RTS_ERR_INVALID;
   }

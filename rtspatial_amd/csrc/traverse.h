@@ -64,22 +64,30 @@ __device__ __forceinline__ float4 load_query(const void* queries, bool points, u
 // (one per lane) from a stack in LDS; children that are leaves, small
 // subtrees (<= leaf_size leaves) or lie inside the query become leaf ranges
 // in an LDS list. Once the list holds enough leaves the wave scans it, 512
-// leaves per step (8 consecutive leaves per lane), and writes the matches of
-// this query contiguously from slot `obase`.
-constexpr int kWaveStack = 1024;    // LDS stack entries per wave
+// leaves per step, and writes the matches of this query contiguously from
+// slot `obase`.
+constexpr int kWaveStack = 512;     // LDS stack entries per wave
 constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
 constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
-constexpr int kPerLane = 8;         // consecutive leaves per lane per scan step
+constexpr int kPerLane = 8;         // leaves per lane per scan step
 constexpr uint32_t kScanStep = kPerLane * kWave;
 
 // Range entries: first leaf | kAllMatch when every leaf of the range is a
 // live box inside the query (no test needed).
 constexpr uint32_t kAllMatch = 0x80000000u;
+// Leaf indices and flat positions are below 2^30, so a leaf offset (first
+// leaf - flat start) biased by 2^30 lies in (0, 2^31) and leaves bit 31 to
+// kAllMatch.
+constexpr uint32_t kDeltaBias = 1u << 30;
 
 struct WaveLds {
   uint32_t stack[kWaveStack];
   uint32_t rfirst[kWaveRanges];
   uint32_t rpre[kWaveRanges + 1];
+  // Scan step map, position p = leaf base + p: first the (entry + 1) of the
+  // ranges that start at p (0 elsewhere), then, for every p, the leaf index
+  // minus the flat index, with kAllMatch of its range.
+  uint32_t map[kScanStep];
 };
 
 // Returns the query's number of results (wave-uniform). With WRITE the pairs
@@ -196,105 +204,94 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
       if (lane == 0) rpre[m] = total;
       lds_fence();
     } else {
-      // ---- scan the queued leaf ranges, kScanStep leaves per step: lane L
-      // owns the kPerLane consecutive leaves from flat index base + L*kPerLane.
-      // Its leaf ids are loaded together with the boxes it must test (not
-      // after the test), so a step waits for one memory round trip; ids of a
-      // lane inside one range come in two 16-byte loads, and its matches go
-      // out as one contiguous run of pairs placed by a single prefix sum.
+      // ---- scan the queued leaf ranges, kScanStep leaves per step. Lane L
+      // takes the leaves at step positions k*64 + L (k < kPerLane), so every
+      // load and store instruction of the wave covers 64 consecutive leaves
+      // (pairs). The range of each position comes from an LDS map: range
+      // starts are marked, a prefix maximum gives each position its range,
+      // and the map then holds each position's leaf offset.
       uint32_t e0 = 0;  // wave-uniform: entry holding flat index `base`
+      uint32_t* map = L.map;
       for (uint32_t base = 0; base < total; base += kScanStep) {
-        const uint32_t j0 = base + lane * kPerLane;
+        const uint32_t lim = total - base < kScanStep ? total - base : kScanStep;
+        // (1) clear this lane's 8 map words, then mark the ranges that start
+        // inside the step (entries after e0, in rpre order).
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++) map[lane * kPerLane + k] = 0u;
+        lds_fence();
+        for (uint32_t e = e0 + 1 + lane;; e += kWave) {
+          const bool act = e < m && rpre[e] < base + lim;
+          if (act) map[rpre[e] - base] = e + 1;
+          if (!__any(act)) break;
+        }
+        lds_fence();
+        // (2) owner (entry + 1) of positions L*8 .. L*8+7: running maximum
+        // within the lane, then across lanes, seeded with e0 + 1.
+        uint32_t own[kPerLane];
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++) {
+          const uint32_t v = map[lane * kPerLane + k];
+          run = v > run ? v : run;
+          own[k] = run;
+        }
+        uint32_t carry = wave_inclusive_max(run);
+        carry = __shfl_up(carry, 1, 64);
+        carry = lane == 0 ? e0 + 1 : (carry > e0 + 1 ? carry : e0 + 1);
+        uint32_t last = 0;
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++) {
+          const uint32_t e = (own[k] > carry ? own[k] : carry) - 1;
+          last = e;
+          const uint32_t f = rfirst[e];
+          own[k] = ((f & ~kAllMatch) - rpre[e] + kDeltaBias) | (f & kAllMatch);
+        }
+        e0 = readlane_u32(last, (lim - 1) / kPerLane);  // entry of the step's last leaf
+        lds_fence();
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++) map[lane * kPerLane + k] = own[k];
+        lds_fence();
+        // (3) leaves of positions k*64 + L: ids loaded together with the boxes
+        // that need a test, so a step waits for one memory round trip.
         uint32_t leaf[kPerLane];
         uint32_t gid[kPerLane];
-        uint32_t valid = 0, allm = 0;  // per-leaf bits
-        bool contiguous = false;
-        if (j0 < total) {
-          // entry holding j0: binary search in [e0, m) unless the step lies in e0
-          uint32_t e = e0;
-          if (rpre[e0 + 1] < base + kScanStep) {
-            uint32_t hi = m - 1;
-            while (e < hi) {
-              const uint32_t mid = (e + hi + 1) >> 1;
-              if (rpre[mid] <= j0) e = mid; else hi = mid - 1;
-            }
-          }
-          uint32_t f = rfirst[e], lo_j = rpre[e], hi_j = rpre[e + 1];
-          contiguous = j0 + kPerLane <= hi_j;
+        uint32_t valid = 0, allm = 0;
 #pragma unroll
-          for (int k = 0; k < kPerLane; k++) {
-            const uint32_t j = j0 + k;
-            if (j < total && j >= hi_j) {  // ranges are non-empty: one step forward
-              e++;
-              f = rfirst[e];
-              lo_j = hi_j;
-              hi_j = rpre[e + 1];
-            }
-            leaf[k] = (f & ~kAllMatch) + (j - lo_j);
-            const bool ok = j < total && leaf[k] < t.n;
-            valid |= (ok ? 1u : 0u) << k;
-            allm |= ((ok && (f & kAllMatch)) ? 1u : 0u) << k;
-          }
-          contiguous = contiguous && leaf[0] + kPerLane <= t.n;
-          if (WRITE) {
-            if (contiguous) {
-              // consecutive ids; the address is only 4-byte aligned
-              __builtin_memcpy(gid, t.leaf_ids + leaf[0], sizeof(gid));
-            } else {
-#pragma unroll
-              for (int k = 0; k < kPerLane; k++)
-                gid[k] = ((valid >> k) & 1u) ? t.leaf_ids[leaf[k]] : 0u;
-            }
-          }
+        for (int k = 0; k < kPerLane; k++) {
+          const uint32_t pos = k * kWave + lane;
+          const uint32_t d = map[pos < kScanStep ? pos : 0];
+          leaf[k] = base + pos + (d & ~kAllMatch) - kDeltaBias;
+          const bool ok = pos < lim && leaf[k] < t.n;
+          valid |= (ok ? 1u : 0u) << k;
+          allm |= ((ok && (d & kAllMatch)) ? 1u : 0u) << k;
+          if (WRITE) gid[k] = ok ? t.leaf_ids[leaf[k]] : 0u;
         }
         uint32_t hitmask = allm;
-        if (valid & ~allm) {
 #pragma unroll
-          for (int k = 0; k < kPerLane; k++) {
-            if (((valid & ~allm) >> k) & 1u)
-              hitmask |= (P::leaf(t.leaf_boxes[leaf[k]], qb) ? 1u : 0u) << k;
-          }
+        for (int k = 0; k < kPerLane; k++) {
+          if (((valid & ~allm) >> k) & 1u)
+            hitmask |= (P::leaf(t.leaf_boxes[leaf[k]], qb) ? 1u : 0u) << k;
         }
-        const uint32_t nhit = __popc(hitmask);
-        // e0 for the next step: the entry of the step's last leaf
-        {
-          const uint32_t jl = base + kScanStep - 1 < total ? base + kScanStep - 1 : total - 1;
-          uint32_t e = e0, hi = m - 1;
-          while (e < hi) {
-            const uint32_t mid = (e + hi + 1) >> 1;
-            if (rpre[mid] <= jl) e = mid; else hi = mid - 1;
-          }
-          e0 = e;
-        }
-        const uint32_t inc = wave_inclusive_sum(nhit);
-        const uint32_t step_hits = readlane_u32(inc, kWave - 1);
-        if (WRITE && nhit) {
-          uint64_t slot = obase + emitted + (inc - nhit);
-          if (hitmask == (1u << kPerLane) - 1 && slot + kPerLane <= cap) {
-            // all kPerLane matched: one contiguous run of pairs (8-byte aligned)
-            rts_pair run[kPerLane];
+        // (4) slots: position order is k-major, so the pairs of round k go
+        // right after those of rounds < k, in lane order.
+        uint32_t step_hits = 0;  // wave-uniform
 #pragma unroll
-            for (int k = 0; k < kPerLane; k++) {
-              run[k].geom_id = gid[k];
-              run[k].query_id = qid;
-            }
-            __builtin_memcpy(out + slot, run, sizeof(run));
-          } else {
-#pragma unroll
-            for (int k = 0; k < kPerLane; k++) {
-              if ((hitmask >> k) & 1u) {
-                if (slot < cap) {
-                  rts_pair pr;
-                  pr.geom_id = gid[k];
-                  pr.query_id = qid;
-                  out[slot] = pr;
-                }
-                slot++;
-              }
+        for (int k = 0; k < kPerLane; k++) {
+          const bool h = (hitmask >> k) & 1u;
+          const uint64_t mask = ballot(h);
+          if (WRITE && h) {
+            const uint64_t slot = obase + emitted + step_hits + mask_rank(mask);
+            if (slot < cap) {
+              rts_pair pr;
+              pr.geom_id = gid[k];
+              pr.query_id = qid;
+              out[slot] = pr;
             }
           }
+          step_hits += (uint32_t)__popcll(mask);
         }
         emitted += step_hits;
+        lds_fence();  // the map is rewritten by the next step
       }
       m = 0;
       total = 0;
