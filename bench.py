@@ -139,6 +139,13 @@ def run_served(args):
     ix.Init(R.Config(max_geometries=n_boxes))
     build = build_rate(R, ix, boxes, torch)
     log(f"build {build['ms']} ms ({build['mboxes_per_s']} Mboxes/s)")
+    tuned = {}
+    if not args.no_tune:
+        for name, pred, pool in (("point", R.Predicate.kContains, pt_pool),
+                                 ("within", R.Predicate.kWithin, env_pool),
+                                 ("intersects", R.Predicate.kIntersects, env_pool)):
+            tuned[name] = ix.Autotune(pred, pool)
+            log(f"autotune {name}: {tuned[name]}")
 
     # The trace: sizes log-uniform in [1, max_req], kinds 40/30/30, slices of
     # the pools; arrival gaps exponential with mean 1 (scaled per rate).
@@ -158,10 +165,11 @@ def run_served(args):
 
     # Updates: 100k boxes shifted by U[-1e-3, 1e-3], 10 per second of trace.
     ru = np.random.default_rng(5003)
-    n_upd_batches = 64
+    n_upd_batches = 100
+    perm = ru.permutation(n_boxes)
     upd = []
-    for _ in range(n_upd_batches):
-        ids = ru.choice(n_boxes, 100_000, replace=False).astype(np.int64)
+    for b in range(n_upd_batches):
+        ids = np.sort(perm[(b * 100_000) % n_boxes:][:100_000]).astype(np.int64)
         sh = ru.uniform(-1e-3, 1e-3, size=(100_000, 2)).astype(np.float32)
         nb = boxes_np[ids].copy()
         nb[:, [0, 2]] += sh[:, :1]
@@ -184,17 +192,38 @@ def run_served(args):
         ix.Build()
         ix.Sync()
 
-    cfg = dict(arena_pairs=args.arena_pairs, chunk_queries=256 << 10, slots=3)
+    # Served: 3 batches in flight, 6e9-pair arenas, 128K-query chunks, and a
+    # request may take 75% of a batch's remaining budget while others wait.
+    # Plain: one query call per 256K-query chunk into one 3e9-pair queue.
+    cfgs = {"batched": dict(arena_pairs=args.arena_pairs, chunk_queries=128 << 10, slots=3,
+                            head_share_pct=75),
+            "plain": dict(arena_pairs=3_000_000_000, chunk_queries=256 << 10)}
 
-    def one(mode, rate, with_updates=True, profile=False, **kw):
+    # Every batched write launch of this workload is timed (HIP events on its
+    # slot's stream) and its algorithmic bytes are summed, so the roofline
+    # averages over the same launches as a kernel trace of the whole command.
+    acc = {"launches": 0, "ms": 0.0, "bytes": 0}
+
+    def account(prof, queries, s_q_bytes, results):
+        wn, wms = prof["batch_write"]
+        if wn:
+            acc["launches"] += wn
+            acc["ms"] += wms
+            acc["bytes"] += query_bytes(queries, 0, results, n_boxes, launches=wn) + s_q_bytes
+
+    def one(mode, rate, with_updates=True, reqs_cut=None, **kw):
         reset_index()
         reqs, ups = trace(rate, with_updates)
-        if profile:
-            R.profile_enable(True)
-            R.profile_read(True)
-        res, st = S.run(ix, reqs, ups, mode=mode, **cfg, **kw)
-        prof = R.profile_read(True) if profile else None
+        if reqs_cut:
+            reqs = reqs[:reqs_cut]
+        R.profile_enable(True)
+        R.profile_read(True)
+        res, st = S.run(ix, reqs, ups, mode=mode, **{**cfgs[mode], **kw})
+        prof = R.profile_read(True)
         R.profile_enable(False)
+        nq = sum(int(r.queries.shape[0]) for r in reqs)
+        sq = sum(int(r.queries.shape[0]) * (8 if r.queries.shape[1] == 2 else 16) for r in reqs)
+        account(prof, nq, sq, int(st["results"]))
         lat = np.array([r["done_s"] - q.arrival_s for r, q in zip(res, reqs)])
         out = {"mode": mode, "rate_rps": rate, "wall_s": round(st["wall_s"], 4),
                "mq_per_s": round(st["queries"] / st["wall_s"] / 1e6, 4),
@@ -211,9 +240,9 @@ def run_served(args):
     sat_plain, _, _ = one("plain", 0.0)
     sat_served = None
     for _ in range(max(1, args.steps)):
-        s, _, prof = one("batched", 0.0, profile=True)
+        s, _, _ = one("batched", 0.0)
         if sat_served is None or s["mq_per_s"] > sat_served["mq_per_s"]:
-            sat_served, best_prof = s, prof
+            sat_served = s
 
     # Operating point: 70% of the plain path's capacity (requests per second).
     plain_rps = n_req / sat_plain["wall_s"]
@@ -221,14 +250,12 @@ def run_served(args):
     op_plain, _, _ = one("plain", lam0)
     target = op_plain["p99_ms"]
     sweep, best = [], None
-    for m in (1.0, 1.25, 1.5, 2.0, 2.5, 3.0, 4.0):
-        lam = lam0 * m
-        s, _, _ = one("batched", lam)
+    for m in (1.0, 1.15, 1.3, 1.45, 1.6, 1.8):
+        s, _, _ = one("batched", lam0 * m)
         sweep.append(s)
-        if s["p99_ms"] <= target:
+        if s["p99_ms"] <= target and (best is None or s["mq_per_s"] > best["mq_per_s"]):
             best = s
-        elif m > 1.0:
-            break
+    met = best is not None
     if best is None:
         best = sweep[0]
 
@@ -237,16 +264,48 @@ def run_served(args):
     reset_index()
     reqs_v, _ = trace(0.0, with_updates=False)
     reqs_v = reqs_v[: args.verify]
-    rv_b, _ = S.run(ix, reqs_v, (), mode="batched", hash_results=1, **cfg)
-    rv_p, _ = S.run(ix, reqs_v, (), mode="plain", hash_results=1, **cfg)
-    same = all(a["count"] == b["count"] and a["hash"] == b["hash"] for a, b in zip(rv_b, rv_p))
-    assert same, "served and plain result hashes differ"
+    R.profile_enable(True)
+    R.profile_read(True)
+    rv_b, stv = S.run(ix, reqs_v, (), mode="batched", hash_results=1, **cfgs["batched"])
+    account(R.profile_read(True), sum(int(r.queries.shape[0]) for r in reqs_v),
+            sum(int(r.queries.shape[0]) * (8 if r.queries.shape[1] == 2 else 16) for r in reqs_v),
+            int(stv["results"]))
+    R.profile_enable(False)
+    rv_p, _ = S.run(ix, reqs_v, (), mode="plain", hash_results=1, **cfgs["plain"])
+    mism = [i for i, (a, b) in enumerate(zip(rv_b, rv_p))
+            if a["count"] != b["count"] or a["hash"] != b["hash"]]
+    same = not mism
+    detail = []
+    if mism:
+        from oracle import oracle as O
+        tree = O.RTree(boxes_np)
+        for i in mism[:8]:
+            a, b = rv_b[i], rv_p[i]
+            d = {"request": i, "kind": int(kinds[i]), "queries": int(sizes[i]),
+                 "served": [int(a["count"]), int(a["hash"]), int(a["relaunches"]), int(a["truncated"])],
+                 "plain": [int(b["count"]), int(b["hash"]), int(b["relaunches"]), int(b["truncated"])]}
+            if sizes[i] <= 20000:
+                qh = reqs_v[i].queries.cpu().numpy()
+                tot, _, hs, _ = tree.summary(int(reqs_v[i].predicate), qh)
+                d["oracle"] = [int(tot), int(np.sum(hs, dtype=np.uint64))]
+            detail.append(d)
+        log(f"MISMATCH between served and plain: {json.dumps(detail)}")
 
     # Roofline of the dominant kernel (the batched write pass) over the
     # saturated served run.
-    wn, wms = best_prof["batch_write"]
-    wbytes = query_bytes(total_q, 0, sat_served["results"], n_boxes, launches=max(wn, 1)) + s_q_total
-    achieved = wbytes / (wms / 1e3) / 1e9 if wms > 0 else 0.0
+    # The write kernel alone: one saturated replay with one batch in flight,
+    # so no launch shares the GPU with another.
+    log("standalone write-kernel replay (one batch in flight)")
+    a0 = dict(acc)
+    one("batched", 0.0, with_updates=False, slots=1)
+    solo_n = acc["launches"] - a0["launches"]
+    solo_ms = acc["ms"] - a0["ms"]
+    solo_bytes = acc["bytes"] - a0["bytes"]
+    solo_gbs = solo_bytes / (solo_ms / 1e3) / 1e9 if solo_ms > 0 else 0.0
+    wn = max(acc["launches"], 1)
+    wbytes = acc["bytes"] / wn
+    wavg = acc["ms"] / wn
+    achieved = wbytes / (wavg / 1e3) / 1e9 if wavg > 0 else 0.0
 
     out = {
         "metric": METRIC,
@@ -267,8 +326,11 @@ def run_served(args):
                          f"Poisson arrivals, sizes log-uniform in [1, {args.max_request}] queries "
                          f"({total_q} queries), 40% point / 30% within / 30% intersects with "
                          f"log-normal query sides mu=ln 5e-3; 100k-box update batches at 10 Hz; "
-                         f"served = continuous batching (3 slots, persistent work-stealing "
-                         f"batches), plain = one query call per request"),
+                         f"served = continuous batching (3 batches in flight, persistent "
+                         f"work-stealing batches, backfill admission), plain = one query call "
+                         f"per request chunk in arrival order; value = served Mqueries/s at the "
+                         f"highest offered load whose p99 is within the plain path's p99 at 70% "
+                         f"of its capacity"),
             "path": "served",
             "boxes": n_boxes,
             "requests": n_req,
@@ -281,19 +343,28 @@ def run_served(args):
         "served": {"capacity_mq_per_s": sat_served["mq_per_s"], "at_equal_p99": best,
                    "sweep": sweep},
         "p99_target_ms": target,
-        "bit_exact_check": {"requests": len(reqs_v), "counts_and_hashes_equal": same},
+        "p99_target_met": met,
+        "served_vs_plain_at_equal_p99": round(best["mq_per_s"] / op_plain["mq_per_s"], 3),
+        "bit_exact_check": {"requests": len(reqs_v), "counts_and_hashes_equal": same,
+                            "mismatches": detail},
         "build": build,
+        "tuned_plans": tuned,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_batch<write> (batched write pass)",
+            "kernel": "k_batch<write> (batched write pass), every launch of the command",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "algorithmic_bytes": int(wbytes),
+            "algorithmic_bytes_per_launch": int(wbytes),
             "launches": int(wn),
-            "kernel_ms": round(wms, 3),
+            "avg_launch_ms": round(wavg, 4),
+            "note": ("up to 3 batches share the GPU, so launches overlap and their durations "
+                     "include the other slots' work; 'standalone' has one batch in flight"),
+            "standalone": {"launches": int(solo_n), "avg_launch_ms": round(solo_ms / max(solo_n, 1), 4),
+                           "algorithmic_bytes_per_launch": int(solo_bytes / max(solo_n, 1)),
+                           "achieved": round(solo_gbs, 2), "frac": round(solo_gbs / HBM_PEAK_GBS, 4)},
         },
     }
     if not args.no_cpu_baseline:
@@ -325,6 +396,8 @@ def run_c3(args):
     ix.Init(R.Config(max_geometries=args.boxes))
     build = build_rate(R, ix, boxes, torch)
     log(f"build: {build}")
+    tuned = ix.Autotune(pred, queries) if not args.no_tune else None
+    log(f"autotune: {tuned}")
     cnt = R.SharedValue()
     cnt.set(0)
     ix.Query(pred, queries, cnt)
@@ -382,6 +455,7 @@ def run_c3(args):
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                        "p99": round(float(np.percentile(step_ms, 99)), 4)},
         "build": build,
+        "tuned_plan": tuned,
         "phases_ms_per_step": {"count_scan": round(prof["query_count"][1] / max(prof["query_count"][0], 1), 4),
                                "write": round(trav_avg, 4)},
         "roofline": {"bound": "hbm", "kernel": "k_query_wave<intersects, write>",
@@ -526,11 +600,12 @@ def main():
     ap.add_argument("--rounds", type=int, default=20)
     ap.add_argument("--boxes", type=int, default=10_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000)
-    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--max-request", type=int, default=1_000_000)
-    ap.add_argument("--arena-pairs", type=int, default=3_000_000_000)
+    ap.add_argument("--arena-pairs", type=int, default=6_000_000_000)
     ap.add_argument("--verify", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default plans)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
@@ -543,6 +618,9 @@ def main():
     out = {"served": run_served, "c3": run_c3, "c4": run_c4,
            "copy": run_copy_calibration}[args.workload](args)
     print(json.dumps(out), flush=True)
+    check = out.get("bit_exact_check")
+    if check and not check["counts_and_hashes_equal"]:
+        sys.exit(3)  # results differ: the line above says where
 
 
 if __name__ == "__main__":

@@ -204,8 +204,9 @@ int rts_index_get_stats(rts_index* index, rts_index_stats* out);
  * `queries` under every candidate plan -- leaf size, one query per lane or
  * per wavefront, Morton-ordered or input-ordered queries -- with the given
  * handler. With `apply` set, it stores the fastest plan in the index for this
- * (predicate, query kind); later queries of that kind use it for every
- * setting the config leaves on auto. Results are identical under every plan;
+ * (predicate, query kind); later queries of that kind, and the batches of
+ * rts_serve_run, use it for every setting their config leaves on auto. The
+ * plan survives rts_index_clear. Results are identical under every plan;
  * only the time differs. `queries` is rts_point* when `points` is 1, else
  * rts_envelope*, in device or pinned host memory. Synchronous. */
 typedef struct rts_tune_config {

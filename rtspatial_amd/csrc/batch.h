@@ -10,6 +10,7 @@ namespace rts {
 constexpr uint32_t kReqPoints = 1u;       // queries are points (float2)
 constexpr uint32_t kReqPredShift = 1;     // bits 1-2: rts_predicate
 constexpr uint32_t kReqWaveTiles = 8u;    // one query per wavefront (else 64 per tile)
+constexpr uint32_t kReqLeafShift = 4;     // bits 4-6: log2(leaf_size) + 1, 0 = the tree's
 
 struct BatchRequest {
   const void* queries;  // device (or pinned host) queries of the request

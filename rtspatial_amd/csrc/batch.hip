@@ -47,8 +47,6 @@ __device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, uint32_t n_til
       const uint32_t begin = x * per;
       const uint32_t end = begin + per < n_tiles ? begin + per : n_tiles;
       if (begin >= end) continue;
-      if (__hip_atomic_load(&counters[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= end - begin)
-        continue;
       const uint32_t t = atomicAdd(&counters[x], 1u);
       if (t < end - begin) {
         got = begin + t;
@@ -91,7 +89,8 @@ __device__ void run_wave_tile(const TreeView& t, const BatchRequest& r, uint32_t
 template <bool WRITE>
 __global__ void __launch_bounds__(kThreads) k_batch(const TreeView* tv, const BatchArgs a) {
   __shared__ WaveLds s_lds[kWaves];
-  const TreeView t = *tv;
+  const TreeView t0 = *tv;
+  TreeView t = t0;  // leaf_size may differ per request
   const uint32_t w = threadIdx.x / kWave;
   const BatchHeader h = *a.header;
   if (h.n_tiles == 0 || h.n_requests == 0 || t.n == 0) return;
@@ -107,6 +106,8 @@ __global__ void __launch_bounds__(kThreads) k_batch(const TreeView* tv, const Ba
     const uint32_t local = tile - r.tbase;
     const bool points = r.flags & kReqPoints;
     const int pred = (r.flags >> kReqPredShift) & 3;
+    const uint32_t lb = (r.flags >> kReqLeafShift) & 7u;
+    t.leaf_size = lb ? 1u << (lb - 1) : t0.leaf_size;
     if (r.flags & kReqWaveTiles) {
       if (local >= r.n) continue;
       if (points) {

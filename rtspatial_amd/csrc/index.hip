@@ -261,6 +261,8 @@ TreeView tree_view(const rts_index* ix) {
   return t;
 }
 
+int device_error_status(rts_index* ix) { return check_device_error(ix); }
+
 int ensure_built(rts_index* ix, hipStream_t stream) {
   poll_cost(ix);
   if (ix->stale) return build_tree(ix, stream);
@@ -315,6 +317,8 @@ int rts_index_create(const rts_config* cfg, rts_index** out) {
   ix->error = reinterpret_cast<uint32_t*>(ix->small + 64);
   ix->root_live = reinterpret_cast<uint32_t*>(ix->small + 80);
   RTS_HIP(hipMemset(ix->small, 0, 256));
+  // Complete before any stream (including non-blocking ones) uses the index.
+  RTS_HIP(hipDeviceSynchronize());
   RTS_HIP(hipHostMalloc(reinterpret_cast<void**>(&ix->h_cost), 4 * sizeof(double)));
   RTS_HIP(hipEventCreateWithFlags(&ix->ev_cost_build, hipEventDisableTiming));
   RTS_HIP(hipEventCreateWithFlags(&ix->ev_cost_refit, hipEventDisableTiming));
@@ -362,8 +366,6 @@ int rts_index_clear(rts_index* ix) {
   ix->n_tree = 0;
   ix->stale = true;
   ix->refit_needed = false;
-  for (auto& row : ix->tuned)
-    for (auto& t : row) t = rts_index::TunedPlan{};
   ix->cost_build_pending = ix->cost_refit_pending = false;
   ix->sah_at_build = ix->sah_current = 0;
   return RTS_OK;
@@ -426,14 +428,19 @@ struct QueryScratch {
   uint64_t* offsets = nullptr;      //                exclusive scan of counts
   uint64_t* total = nullptr;        //                [0] total, [1] queue base
   void* scan_tmp = nullptr;
+  uint32_t* tickets = nullptr;      // wave traversal: per-XCD query tickets
 };
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
-int alloc_scratch(uint64_t nq, bool sort, bool two_pass, hipStream_t stream, QueryScratch* qs,
-                  size_t* order_off, size_t* sort_off) {
+int alloc_scratch(uint64_t nq, bool sort, bool wave, bool two_pass, hipStream_t stream,
+                  QueryScratch* qs, size_t* order_off, size_t* sort_off) {
   size_t off = 0;
-  size_t o_keys = 0, o_sort = 0, o_counts = 0, o_offsets = 0, o_total = 0, o_scan = 0;
+  size_t o_keys = 0, o_sort = 0, o_counts = 0, o_offsets = 0, o_total = 0, o_scan = 0, o_tk = 0;
+  if (wave) {
+    o_tk = off;
+    off = align_up(off + kWaveTicketWords * sizeof(uint32_t));
+  }
   if (sort) {
     o_keys = off;
     off = align_up(off + 4 * nq * sizeof(uint32_t));
@@ -453,6 +460,7 @@ int alloc_scratch(uint64_t nq, bool sort, bool two_pass, hipStream_t stream, Que
   if (off == 0) return RTS_OK;
   RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&qs->block), off, stream));
   char* b = qs->block;
+  if (wave) qs->tickets = reinterpret_cast<uint32_t*>(b + o_tk);
   if (two_pass) {
     qs->counts = reinterpret_cast<uint32_t*>(b + o_counts);
     qs->offsets = reinterpret_cast<uint64_t*>(b + o_offsets);
@@ -475,6 +483,7 @@ int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, 
   uint32_t* vals_b = keys_b + words;
   SortScratch ss;
   sort_scratch_carve(sort_base, nq, &ss);
+  ss.error = ix->error;  // a look-back that gives up is reported by the index's error word
   if (points) {
     RTS_HIP(launch_morton_points(static_cast<const float2*>(queries), (uint32_t)nq, ix->scene, keys,
                                  stream));
@@ -500,8 +509,9 @@ struct Plan {
 // rts_index_autotune fills the settings left on auto; otherwise: one query
 // per wavefront where queries have many results (envelope intersects /
 // within), one per lane otherwise (points, envelope contains), and Morton
-// ordering only for per-lane traversal of large batches (neighbouring lanes
-// then share their descent; wave traversal gains nothing from it).
+// ordering of batches of 64K queries or more (neighbouring lanes share their
+// descent; neighbouring waves, which take consecutive queries on one XCD,
+// share that XCD's L2).
 Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
   const rts_index::TunedPlan& t = ix->tuned[pred][points ? 1 : 0];
   Plan p;
@@ -516,7 +526,7 @@ Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
   else if (t.valid)
     p.sort = t.sort;
   else
-    p.sort = p.traversal == RTS_TRAVERSAL_THREAD && nq >= 65536;
+    p.sort = nq >= 65536;
   p.leaf_size = t.valid ? t.leaf_size : ix->cfg.leaf_size;
   return p;
 }
@@ -537,7 +547,8 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
                             : static_cast<const void*>(static_cast<const float4*>(queries) + off);
     QueryScratch qs;
     size_t o_keys = 0, o_sort = 0;
-    int rc = alloc_scratch(m, sort, two_pass, stream, &qs, &o_keys, &o_sort);
+    int rc = alloc_scratch(m, sort, trav == RTS_TRAVERSAL_WAVE, two_pass, stream, &qs, &o_keys,
+                           &o_sort);
     if (rc) return rc;
     if (sort) {
       PhaseToken ph = phase_begin(RTS_PHASE_QUERY_ORDER, stream);
@@ -562,12 +573,12 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
       RTS_HIP(launch_query(ql, stream));
       phase_end(ph, stream);
     } else if (!two_pass) {
-      WaveArgs wa{nullptr, nullptr, nullptr};
+      WaveArgs wa{nullptr, nullptr, nullptr, qs.tickets};
       PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
       RTS_HIP(launch_query_wave(ql, wa, kPhaseCountSink, stream));
       phase_end(ph, stream);
     } else {
-      WaveArgs wa{qs.counts, qs.offsets, qs.total + 1};
+      WaveArgs wa{qs.counts, qs.offsets, qs.total + 1, qs.tickets};
       PhaseToken ph = phase_begin(RTS_PHASE_QUERY_COUNT, stream);
       RTS_HIP(launch_query_wave(ql, wa, kPhaseCount, stream));
       RTS_HIP(launch_exclusive_scan(qs.counts, m, qs.offsets, qs.total, qs.scan_tmp, stream));

@@ -60,7 +60,9 @@ struct rts_index {
   uint64_t builds = 0;
   uint64_t refits = 0;
 
-  // Plans stored by rts_index_autotune, per [predicate][points].
+  // Plans stored by rts_index_autotune, per [predicate][points]; kept until
+  // the next tuning of that kind (Clear keeps them: they describe the
+  // workload, not the stored boxes).
   struct TunedPlan {
     bool valid = false;
     uint32_t traversal = 0;
@@ -73,4 +75,7 @@ struct rts_index {
 namespace rts {
 TreeView tree_view(const rts_index* ix);
 int ensure_built(rts_index* ix, hipStream_t stream);
+// Reads the device error word (synchronous): RTS_ERR_DEVICE with a message if
+// a kernel of this index gave up a bounded wait or walk.
+int device_error_status(rts_index* ix);
 }  // namespace rts

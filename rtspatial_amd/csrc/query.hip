@@ -196,57 +196,107 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
 // holds each query's pairs contiguously, in query order.
 enum { kWaveCount = 0, kWaveCountSink = 1, kWaveWrite = 2 };
 
-// Workgroups are dealt to the 8 XCDs round-robin; remapping the launch index
-// gives each XCD (and its L2) one contiguous slice of the query order.
+// Persistent waves take queries from one ticket counter per XCD. The query
+// order is cut into 8 contiguous slices and workgroups are dealt to the XCDs
+// round-robin, so wave w first drains slice w % 8 -- neighbouring (Morton-
+// ordered) queries then share one XCD's L2 -- and steals from the other
+// slices once its own is empty; costs are heavy-tailed, so no static split
+// balances them.
 constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_slice_block(uint32_t b, uint32_t grid) {
-  const uint32_t per = grid / kXcds;  // grid is a multiple of kXcds
-  return (b % kXcds) * per + b / kXcds;
+constexpr uint32_t kTicketStride = 16;  // one 64-byte line per counter
+
+// Next query index for this wave, or >= n when every slice is drained.
+// `slice` is the slice to try first; it becomes the one taken from.
+__device__ __forceinline__ uint32_t grab_query(uint32_t* tickets, uint32_t n, uint32_t& slice) {
+  const uint32_t per = (n + kXcds - 1) / kXcds;
+  uint32_t got = 0xFFFFFFFFu, from = slice;
+  if (lane_id() == 0) {
+    for (uint32_t k = 0; k < kXcds; k++) {
+      const uint32_t x = (slice + k) % kXcds;
+      const uint32_t begin = x * per;
+      const uint32_t end = begin + per < n ? begin + per : n;
+      if (begin >= end) continue;
+      // No relaxed load ahead of the atomic: across XCDs it is a second
+      // memory-side round trip on the same hot line (measured ~1.5x slower).
+      const uint32_t v = atomicAdd(tickets + x * kTicketStride, 1u);
+      if (v < end - begin) {
+        got = begin + v;
+        from = x;
+        break;
+      }
+    }
+  }
+  slice = readlane_u32(from, 0);
+  return readlane_u32(got, 0);
 }
 
-// One wavefront per workgroup: query costs are heavy-tailed, and a workgroup
-// holds its LDS until its last wave ends, so waves packed together would keep
-// the LDS of finished queries from new ones.
+// One wavefront per workgroup, so that a wave's LDS is released by itself.
 template <int PRED, bool POINTS, int MODE>
 __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const WaveArgs a) {
   __shared__ WaveLds s_lds;
   const TreeView t = q.tree;
-  const uint32_t lin = xcd_slice_block(blockIdx.x, gridDim.x);
-  if (lin >= q.n || t.n == 0) return;  // wave-uniform
-  const uint32_t qi = q.order ? q.order[lin] : lin;
-  const float4 qb = load_query(q.queries, POINTS, qi);
-  if (MODE == kWaveWrite) {
-    const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
-    wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base, dq->data, dq->capacity,
-                                   a.base[0] + a.offsets[qi], s_lds);
-    return;
-  }
-  const uint64_t all = wave_query<PRED, POINTS, false>(t, qb, 0, nullptr, 0, 0, s_lds);
-  if (lane_id() == 0) {
-    if (MODE == kWaveCount) {
-      a.counts[qi] = (uint32_t)all;
-    } else if (all) {
-      atomicAdd(static_cast<unsigned long long*>(const_cast<void*>(q.sink.arg)),
-                (unsigned long long)all);
+  if (t.n == 0) return;
+  uint32_t slice = blockIdx.x % kXcds;
+  // Every iteration takes a query or finds none left: q.n + 1 iterations
+  // bound the loop.
+  for (uint32_t it = 0; it <= q.n; it++) {
+    const uint32_t lin = grab_query(a.tickets, q.n, slice);
+    if (lin >= q.n) return;
+    const uint32_t qi = q.order ? q.order[lin] : lin;
+    const float4 qb = load_query(q.queries, POINTS, qi);
+    if (MODE == kWaveWrite) {
+      const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
+      wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base, dq->data, dq->capacity,
+                                     a.base[0] + a.offsets[qi], s_lds);
+      continue;
+    }
+    const uint64_t all = wave_query<PRED, POINTS, false>(t, qb, 0, nullptr, 0, 0, s_lds);
+    if (lane_id() == 0) {
+      if (MODE == kWaveCount) {
+        a.counts[qi] = (uint32_t)all;
+      } else if (all) {
+        atomicAdd(static_cast<unsigned long long*>(const_cast<void*>(q.sink.arg)),
+                  (unsigned long long)all);
+      }
     }
   }
 }
 
+// Persistent grid: 32 single-wave workgroups per CU, the most waves a CU
+// holds. Occupancy (VGPRs, LDS) admits fewer at once; the rest start as
+// waves finish and exit as soon as they find every slice drained.
+uint32_t wave_grid_blocks() {
+  static const uint32_t blocks = [] {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return (uint32_t)((cus > 0 ? cus : 256) * 32);
+  }();
+  return blocks;
+}
+
+template <int PRED, bool POINTS, int MODE>
+hipError_t launch_wave_mode(const QueryLaunch& q, const WaveArgs& a, hipStream_t stream) {
+  hipError_t e = launch_fill_u32(a.tickets, 0u, kXcds * kTicketStride, stream);
+  if (e != hipSuccess) return e;
+  const uint32_t resident = wave_grid_blocks();
+  uint32_t blocks = resident < q.n ? resident : q.n;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;  // whole rounds over the XCDs
+  hipLaunchKernelGGL((k_query_wave<PRED, POINTS, MODE>), dim3(blocks), dim3(kWave), 0, stream, q,
+                     a);
+  return hipGetLastError();
+}
+
 template <int PRED, bool POINTS>
 hipError_t launch_wave(const QueryLaunch& q, const WaveArgs& a, int phase, hipStream_t stream) {
-  const uint32_t blocks = (q.n + kXcds - 1) / kXcds * kXcds;  // whole rounds over the XCDs
-  const dim3 grid(blocks), block(kWave);
   switch (phase) {
     case kWaveCount:
-      hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveCount>), grid, block, 0, stream, q, a);
-      break;
+      return launch_wave_mode<PRED, POINTS, kWaveCount>(q, a, stream);
     case kWaveCountSink:
-      hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveCountSink>), grid, block, 0, stream, q, a);
-      break;
+      return launch_wave_mode<PRED, POINTS, kWaveCountSink>(q, a, stream);
     default:
-      hipLaunchKernelGGL((k_query_wave<PRED, POINTS, kWaveWrite>), grid, block, 0, stream, q, a);
+      return launch_wave_mode<PRED, POINTS, kWaveWrite>(q, a, stream);
   }
-  return hipGetLastError();
 }
 
 template <int PRED, bool POINTS>

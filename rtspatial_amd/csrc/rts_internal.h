@@ -150,7 +150,9 @@ struct WaveArgs {
   uint32_t* counts;
   const uint64_t* offsets;
   const uint64_t* base;
+  uint32_t* tickets;  // kWaveTicketWords words, cleared by the launch
 };
+constexpr uint32_t kWaveTicketWords = 8 * 16;
 enum { kPhaseCount = 0, kPhaseCountSink = 1, kPhaseWrite = 2 };
 hipError_t launch_query_wave(const QueryLaunch& q, const WaveArgs& a, int phase,
                              hipStream_t stream);

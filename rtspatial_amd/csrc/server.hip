@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -296,12 +297,27 @@ int copy_out(Engine& E, const Chunk& c, const rts_pair* src, uint64_t cnt, hipSt
   return RTS_OK;
 }
 
+// Tile mode and leaf-range size of a request: the serve config's traversal
+// if it names one, else the plan stored by rts_index_autotune for this
+// (predicate, query kind), else one query per wavefront for envelope
+// intersects / within and 64 per tile otherwise.
 uint32_t tile_flags(const Engine& E, const rts_request& r) {
   uint32_t f = (r.points ? kReqPoints : 0u) | ((uint32_t)(r.predicate & 3) << kReqPredShift);
+  const rts_index::TunedPlan& tp = E.ix->tuned[r.predicate & 3][r.points ? 1 : 0];
   const uint32_t trav = E.cfg.traversal;
-  const bool wave = trav == RTS_TRAVERSAL_WAVE ||
-                    (trav != RTS_TRAVERSAL_THREAD && !r.points && r.predicate != RTS_CONTAINS);
+  bool wave;
+  if (trav == RTS_TRAVERSAL_WAVE || trav == RTS_TRAVERSAL_THREAD)
+    wave = trav == RTS_TRAVERSAL_WAVE;
+  else if (tp.valid)
+    wave = tp.traversal == RTS_TRAVERSAL_WAVE;
+  else
+    wave = !r.points && r.predicate != RTS_CONTAINS;
   if (wave) f |= kReqWaveTiles;
+  if (tp.valid && tp.leaf_size >= 1 && tp.leaf_size <= 64) {
+    uint32_t lg = 0;
+    while ((2u << lg) <= tp.leaf_size) lg++;
+    f |= (lg + 1) << kReqLeafShift;
+  }
   return f;
 }
 
@@ -592,9 +608,13 @@ int run_plain(Engine& E) {
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_spans), 2 * kRing * sizeof(uint64_t)));
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_hash), kRing * sizeof(unsigned long long)));
   for (auto& e : evs) SRV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // Set up on the loop's own stream: it does not wait for the null stream,
+  // so a null-stream memset could still be pending when the first query
+  // reads the counter.
   const rts_dev_queue q{arena, E.cfg.arena_pairs, d_counter};
-  SRV_HIP(hipMemcpy(d_queue, &q, sizeof(q), hipMemcpyHostToDevice));
-  SRV_HIP(hipMemset(d_counter, 0, sizeof(unsigned long long)));
+  SRV_HIP(hipMemcpyAsync(d_queue, &q, sizeof(q), hipMemcpyHostToDevice, stream));
+  SRV_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(d_counter), 0u, 2, stream));
+  SRV_HIP(hipStreamSynchronize(stream));
 
   std::deque<PlainChunk> inflight;
   uint64_t next_up = 0, ring = 0;
@@ -766,6 +786,10 @@ int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request*
   if (E.upd_stream) {
     (void)hipStreamSynchronize(E.upd_stream);
     (void)hipStreamDestroy(E.upd_stream);
+  }
+  if (!rc) {
+    std::lock_guard<std::mutex> lk(ix->mu);
+    rc = device_error_status(ix);
   }
   if (E.upd_event) (void)hipEventDestroy(E.upd_event);
   if (stats) {
