@@ -811,9 +811,15 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   best = default_ms;
   Plan winner = base;
   uint32_t plans = 1;
-  for (uint32_t k = 1; k <= 6; ++k) {
-    if (!(cfg.leaf_sizes & (1u << k))) continue;
-    for (int trav : {RTS_TRAVERSAL_THREAD, RTS_TRAVERSAL_WAVE}) {
+  // Traversal modes in the outer loop: a mode whose first plan is already 4x
+  // slower than the best is not tried with the other leaf sizes or orders
+  // (per-lane traversal of queries with thousands of results each, say).
+  const int other =
+      base.traversal == RTS_TRAVERSAL_WAVE ? (int)RTS_TRAVERSAL_THREAD : (int)RTS_TRAVERSAL_WAVE;
+  for (int trav : {base.traversal, other}) {
+    bool first = true;
+    for (uint32_t k = 1; k <= 6; ++k) {
+      if (!(cfg.leaf_sizes & (1u << k))) continue;
       for (bool sort : {false, true}) {
         Plan p{trav, sort, 1u << k};
         if (p.traversal == base.traversal && p.sort == base.sort && p.leaf_size == base.leaf_size)
@@ -826,8 +832,11 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
           best = ms;
           winner = p;
         }
+        if (first && ms > 4.0 * best) goto next_mode;
+        first = false;
       }
     }
+  next_mode:;
   }
   rc = check_device_error(ix);
   if (rc) return rc;

@@ -409,7 +409,8 @@ def test_autotune_keeps_results_exact(oracle, kind):
     want = oracle.RTree(b).pairs(int(pred), q)
     dq = dev(q)
     r = ix.Autotune(pred, dq, sample=4096, reps=2)
-    assert r["sample"] == 4096 and r["plans"] == 16
+    # 16 plans, fewer when a traversal mode is dropped after its first plan
+    assert r["sample"] == 4096 and 9 <= r["plans"] <= 16
     assert r["leaf_size"] in (4, 8, 16, 32) and r["traversal"] in (THREAD, WAVE)
     assert r["sort_queries"] in (1, 2) and 0 < r["best_ms"] <= r["default_ms"]
     sample_want = oracle.RTree(b).count(int(pred), q[:: q.shape[0] // 4096][:4096])
