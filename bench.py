@@ -269,7 +269,7 @@ def run_served(args):
     op_plain, _, _ = one("plain", lam0)
     target = op_plain["p99_ms"]
     sweep, best = [], None
-    for m in (1.0, 1.3, 1.6, 1.9, 2.2, 2.5):
+    for m in (1.0, 1.5, 2.0, 2.3, 2.6, 2.9):
         s, _, _ = one("batched", lam0 * m)
         sweep.append(s)
         if s["p99_ms"] <= target and (best is None or s["mq_per_s"] > best["mq_per_s"]):
@@ -371,7 +371,7 @@ def run_served(args):
         "tuned_plans": tuned,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_batch<write> (batched write pass), every launch of the command",
+            "kernel": "batched write pass (k_batch<true, class> kernels of one batch), every batch of the command",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -22,10 +22,32 @@ struct BatchRequest {
 };
 static_assert(sizeof(BatchRequest) == 32, "32-byte request descriptor");
 
+// Tile classes: one kernel each, so that each holds only its traversal
+// (a kernel with every traversal inlined needs ~100 VGPRs and runs 4 waves
+// per SIMD). A batch lists its requests class by class.
+constexpr int kClassWaveIntersects = 0;
+constexpr int kClassWaveWithin = 1;
+constexpr int kClassWaveContains = 2;
+constexpr int kClassWavePoints = 3;
+constexpr int kClassLane = 4;  // 64 queries per tile, one per lane, any predicate
+constexpr int kClasses = 5;
+
+__host__ __device__ inline int tile_class(uint32_t flags) {
+  if (!(flags & kReqWaveTiles)) return kClassLane;
+  if (flags & kReqPoints) return kClassWavePoints;
+  switch ((flags >> kReqPredShift) & 3) {
+    case RTS_INTERSECTS: return kClassWaveIntersects;
+    case RTS_WITHIN: return kClassWaveWithin;
+    default: return kClassWaveContains;
+  }
+}
+
 struct BatchHeader {
   uint32_t n_requests;
   uint32_t n_tiles;
   uint64_t n_queries;
+  uint32_t class_end[kClasses];  // class c owns tiles [class_end[c-1], class_end[c])
+  uint32_t pad;
 };
 
 struct BatchArgs {
@@ -55,7 +77,6 @@ hipError_t launch_hash_ranges(const rts_pair* arena, uint64_t cap, const uint64_
 hipError_t launch_rebase_hash_span(rts_pair* arena, uint64_t cap, const uint64_t* span,
                                    uint32_t add, int do_hash, unsigned long long* hash,
                                    hipStream_t stream);
-uint32_t batch_persistent_blocks();
 size_t batch_seg_sums_words();
 
 }  // namespace rts

@@ -37,19 +37,21 @@ __device__ __forceinline__ uint32_t find_request(const BatchRequest* reqs, uint3
   return lo;
 }
 
-// Next tile for this wave, or >= n_tiles when all are taken.
-__device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, uint32_t n_tiles, uint32_t xcd) {
-  const uint32_t per = (n_tiles + kXcds - 1) / kXcds;
+// Next tile of [lo, hi) for this wave, or >= hi when all are taken.
+__device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, uint32_t lo, uint32_t hi,
+                                              uint32_t xcd) {
+  const uint32_t n = hi - lo;
+  const uint32_t per = (n + kXcds - 1) / kXcds;
   uint32_t got = 0xFFFFFFFFu;
   if (lane_id() == 0) {
     for (uint32_t k = 0; k < kXcds; k++) {
       const uint32_t x = (xcd + k) % kXcds;
       const uint32_t begin = x * per;
-      const uint32_t end = begin + per < n_tiles ? begin + per : n_tiles;
+      const uint32_t end = begin + per < n ? begin + per : n;
       if (begin >= end) continue;
       const uint32_t t = atomicAdd(&counters[x], 1u);
       if (t < end - begin) {
-        got = begin + t;
+        got = lo + begin + t;
         break;
       }
     }
@@ -85,43 +87,43 @@ __device__ void run_wave_tile(const TreeView& t, const BatchRequest& r, uint32_t
 }
 
 // The tree view is read from device memory so that a captured graph stays
-// valid when the index is rebuilt between batches.
-template <bool WRITE>
+// valid when the index is rebuilt between batches. The kernel of class CLS
+// takes that class's tiles (see tile_class).
+template <bool WRITE, int CLS>
 __global__ void __launch_bounds__(kThreads) k_batch(const TreeView* tv, const BatchArgs a) {
-  __shared__ WaveLds s_lds[kWaves];
+  constexpr bool kWaveCls = CLS != kClassLane;
+  __shared__ WaveLds s_lds[kWaveCls ? kWaves : 1];
   const TreeView t0 = *tv;
   TreeView t = t0;  // leaf_size may differ per request
   const uint32_t w = threadIdx.x / kWave;
   const BatchHeader h = *a.header;
-  if (h.n_tiles == 0 || h.n_requests == 0 || t.n == 0) return;
-  uint32_t* counters = a.tile_counters + (WRITE ? kXcds : 0);
+  const uint32_t lo = CLS == 0 ? 0u : h.class_end[CLS - 1];
+  const uint32_t hi = h.class_end[CLS];
+  if (lo >= hi || h.n_requests == 0 || t.n == 0) return;
+  uint32_t* counters = a.tile_counters + ((WRITE ? kClasses : 0) + CLS) * kXcds;
   const uint32_t xcd = blockIdx.x % kXcds;
-  // Every iteration consumes a tile (or finds none left), so n_tiles + 1
+  // Every iteration consumes a tile (or finds none left), so hi - lo + 1
   // iterations bound the loop.
-  for (uint32_t it = 0; it <= h.n_tiles; it++) {
-    const uint32_t tile = grab_tile(counters, h.n_tiles, xcd);
-    if (tile >= h.n_tiles) break;
+  for (uint32_t it = 0; it <= hi - lo; it++) {
+    const uint32_t tile = grab_tile(counters, lo, hi, xcd);
+    if (tile >= hi) break;
     const uint32_t ri = find_request(a.requests, h.n_requests, tile);
     BatchRequest r = a.requests[ri];
     const uint32_t local = tile - r.tbase;
-    const bool points = r.flags & kReqPoints;
-    const int pred = (r.flags >> kReqPredShift) & 3;
     const uint32_t lb = (r.flags >> kReqLeafShift) & 7u;
     t.leaf_size = lb ? 1u << (lb - 1) : t0.leaf_size;
-    if (r.flags & kReqWaveTiles) {
-      if (local >= r.n) continue;
-      if (points) {
-        run_wave_tile<RTS_CONTAINS, true, WRITE>(t, r, local, a, s_lds[w]);
-      } else if (pred == RTS_INTERSECTS) {
-        run_wave_tile<RTS_INTERSECTS, false, WRITE>(t, r, local, a, s_lds[w]);
-      } else if (pred == RTS_WITHIN) {
-        run_wave_tile<RTS_WITHIN, false, WRITE>(t, r, local, a, s_lds[w]);
-      } else {
-        run_wave_tile<RTS_CONTAINS, false, WRITE>(t, r, local, a, s_lds[w]);
-      }
+    if (CLS == kClassWaveIntersects) {
+      if (local < r.n) run_wave_tile<RTS_INTERSECTS, false, WRITE>(t, r, local, a, s_lds[w]);
+    } else if (CLS == kClassWaveWithin) {
+      if (local < r.n) run_wave_tile<RTS_WITHIN, false, WRITE>(t, r, local, a, s_lds[w]);
+    } else if (CLS == kClassWaveContains) {
+      if (local < r.n) run_wave_tile<RTS_CONTAINS, false, WRITE>(t, r, local, a, s_lds[w]);
+    } else if (CLS == kClassWavePoints) {
+      if (local < r.n) run_wave_tile<RTS_CONTAINS, true, WRITE>(t, r, local, a, s_lds[w]);
     } else {
       const uint32_t q0 = local * kWave;
-      if (points) {
+      const int pred = (r.flags >> kReqPredShift) & 3;
+      if (r.flags & kReqPoints) {
         run_lane_tile<RTS_CONTAINS, true, WRITE>(t, r, q0, a);
       } else if (pred == RTS_INTERSECTS) {
         run_lane_tile<RTS_INTERSECTS, false, WRITE>(t, r, q0, a);
@@ -219,38 +221,52 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 
 }  // namespace
 
-uint32_t batch_persistent_blocks() {
-  static uint32_t blocks = 0;
-  if (blocks == 0) {
-    int dev = 0, cus = 0, per_cu = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<true>, kThreads, 0);
-    if (cus <= 0) cus = 256;
-    if (per_cu <= 0) per_cu = 1;
-    blocks = (uint32_t)(cus * per_cu);
-    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
-  }
-  return blocks;
+// Resident workgroups of a persistent kernel over the device (occupancy).
+template <typename K>
+uint32_t persistent_blocks(K kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0);
+  if (cus <= 0) cus = 256;
+  if (per_cu <= 0) per_cu = 1;
+  const uint32_t blocks = (uint32_t)(cus * per_cu);
+  return (blocks + kXcds - 1) / kXcds * kXcds;
+}
+
+template <bool WRITE, int CLS>
+void launch_batch_kernel(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
+  static const uint32_t blocks = persistent_blocks(k_batch<WRITE, CLS>);
+  hipLaunchKernelGGL((k_batch<WRITE, CLS>), dim3(blocks), dim3(kThreads), 0, stream, t, a);
+}
+
+// Every class kernel in turn; a class without tiles exits at once.
+template <bool WRITE>
+void launch_batch_pass(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
+  launch_batch_kernel<WRITE, kClassWaveIntersects>(t, a, stream);
+  launch_batch_kernel<WRITE, kClassWaveWithin>(t, a, stream);
+  launch_batch_kernel<WRITE, kClassWaveContains>(t, a, stream);
+  launch_batch_kernel<WRITE, kClassWavePoints>(t, a, stream);
+  launch_batch_kernel<WRITE, kClassLane>(t, a, stream);
 }
 
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
-  const uint32_t blocks = batch_persistent_blocks();
   // Phase events are only recorded outside stream capture (profiling off
   // or graphs off); a captured sequence is timed as a whole by its caller.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(stream, &cap);
   const bool timed = cap == hipStreamCaptureStatusNone;
   PhaseToken ph = timed ? phase_begin(RTS_PHASE_BATCH_COUNT, stream) : PhaseToken{-1, nullptr};
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.tile_counters, 2 * kXcds);
-  hipLaunchKernelGGL(k_batch<false>, dim3(blocks), dim3(kThreads), 0, stream, t, a);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.tile_counters,
+                     2 * kClasses * kXcds);
+  launch_batch_pass<false>(t, a, stream);
   hipLaunchKernelGGL(k_seg_sums, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_apply, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_request_ranges, dim3(16), dim3(kThreads), 0, stream, a);
   phase_end(ph, stream);
   ph = timed ? phase_begin(RTS_PHASE_BATCH_WRITE, stream) : PhaseToken{-1, nullptr};
-  hipLaunchKernelGGL(k_batch<true>, dim3(blocks), dim3(kThreads), 0, stream, t, a);
+  launch_batch_pass<true>(t, a, stream);
   phase_end(ph, stream);
   return hipGetLastError();
 }

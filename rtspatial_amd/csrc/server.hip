@@ -363,7 +363,7 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_ranges), 2 * maxr * sizeof(uint64_t)));
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_hash), maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_desc, E.desc_bytes));
-  SRV_HIP(hipMalloc(&s.d_counters, 64 * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_counters, 2 * kClasses * 8 * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_counts, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_offsets, maxq * sizeof(uint64_t)));
   SRV_HIP(hipMalloc(&s.d_seg, batch_seg_sums_words() * sizeof(uint64_t)));
@@ -512,12 +512,18 @@ int complete_slot(Engine& E, Slot& s) {
   return RTS_OK;
 }
 
-int launch_slot(Engine& E, Slot& s, const std::vector<Chunk>& chunks) {
+int launch_slot(Engine& E, Slot& s, std::vector<Chunk> chunks) {
+  // Requests class by class: each tile class has its own kernel over its
+  // own tile range.
+  std::stable_sort(chunks.begin(), chunks.end(), [&](const Chunk& x, const Chunk& y) {
+    return tile_class(tile_flags(E, E.reqs[x.req])) < tile_class(tile_flags(E, E.reqs[y.req]));
+  });
   *reinterpret_cast<TreeView*>(s.h_desc + kViewOff) = tree_view(E.ix);
   BatchHeader* h = reinterpret_cast<BatchHeader*>(s.h_desc + kHeaderOff);
   BatchRequest* br = reinterpret_cast<BatchRequest*>(s.h_desc + kReqsOff);
   uint64_t qb = 0;
   uint32_t tb = 0;
+  uint32_t class_end[kClasses] = {0, 0, 0, 0, 0};
   for (size_t i = 0; i < chunks.size(); i++) {
     const Chunk& c = chunks[i];
     const rts_request& r = E.reqs[c.req];
@@ -530,9 +536,14 @@ int launch_slot(Engine& E, Slot& s, const std::vector<Chunk>& chunks) {
     br[i].qid_base = (uint32_t)c.q0;  // pairs report query indices within the request
     qb += c.n;
     tb += (br[i].flags & kReqWaveTiles) ? (uint32_t)c.n : (uint32_t)((c.n + kWave - 1) / kWave);
+    class_end[tile_class(br[i].flags)] = tb;
   }
+  for (int c = 1; c < kClasses; c++)  // classes without requests end where the previous ends
+    if (class_end[c] < class_end[c - 1]) class_end[c] = class_end[c - 1];
   h->n_requests = (uint32_t)chunks.size();
   h->n_tiles = tb;
+  for (int c = 0; c < kClasses; c++) h->class_end[c] = class_end[c];
+  h->pad = 0;
   h->n_queries = qb;
   if (E.have_upd_event) SRV_HIP(hipStreamWaitEvent(s.stream, E.upd_event, 0));
   if (E.cfg.use_graphs) {
