@@ -61,12 +61,29 @@ struct BatchArgs {
   uint64_t* ranges;              // per request: {first arena slot, results}
   rts_pair* arena;               // results
   uint64_t arena_cap;            // pairs the arena holds
+  const uint32_t* order;         // wave tiles in Morton order (tile -> tile), or null
+};
+
+// Morton ordering of a batch's wave tiles: keys (class << 30 | Morton code of
+// the query's centre) sorted with the tile index, so that neighbouring tiles
+// of one class run together on one XCD. Buffers hold max_tiles entries.
+struct BatchSort {
+  uint32_t* keys_a;
+  uint32_t* vals_a;
+  uint32_t* keys_b;
+  uint32_t* vals_b;
+  SortScratch ss;
+  const float4* scene;  // the index's scene bounds (device)
+  uint32_t n;           // wave tiles of this batch (0: no ordering)
 };
 
 // Count pass, scan, per-request ranges and write pass of one batch, on
-// `stream` (fixed grids: capturable in a graph and replayable for any batch
-// whose sizes fit the buffers). `t` is a device copy of the tree view.
-hipError_t launch_batch(const TreeView* t, const BatchArgs& a, hipStream_t stream);
+// `stream`. With sort.n = 0 every launch has a fixed grid (capturable in a
+// graph and replayable for any batch whose sizes fit the buffers); otherwise
+// the wave tiles are Morton-ordered first. `t` is a device copy of the tree
+// view.
+hipError_t launch_batch(const TreeView* t, const BatchArgs& a, const BatchSort& sort,
+                        hipStream_t stream);
 // hashes[c] = sum of splitmix64((query_id << 32) | geom_id) over range c of
 // the arena (ranges as written by the batch: {first slot, count}, in order).
 hipError_t launch_hash_ranges(const rts_pair* arena, uint64_t cap, const uint64_t* ranges,

@@ -17,6 +17,7 @@
 // and reads the batch's sizes from device memory, so one captured hipGraph
 // replays any batch.
 #include "batch.h"
+#include "morton.h"
 #include "traverse.h"
 #include "wave.h"
 
@@ -105,8 +106,9 @@ __global__ void __launch_bounds__(kThreads) k_batch(const TreeView* tv, const Ba
   // Every iteration consumes a tile (or finds none left), so hi - lo + 1
   // iterations bound the loop.
   for (uint32_t it = 0; it <= hi - lo; it++) {
-    const uint32_t tile = grab_tile(counters, lo, hi, xcd);
+    uint32_t tile = grab_tile(counters, lo, hi, xcd);
     if (tile >= hi) break;
+    if (kWaveCls && a.order) tile = a.order[tile];  // same class: the keys lead with it
     const uint32_t ri = find_request(a.requests, h.n_requests, tile);
     BatchRequest r = a.requests[ri];
     const uint32_t local = tile - r.tbase;
@@ -240,6 +242,23 @@ void launch_batch_kernel(const TreeView* t, const BatchArgs& a, hipStream_t stre
   hipLaunchKernelGGL((k_batch<WRITE, CLS>), dim3(blocks), dim3(kThreads), 0, stream, t, a);
 }
 
+// Sort keys of the wave tiles [0, n): class in the top 2 bits (the wave
+// classes are 0..3), then the 30-bit Morton code of the query's centre.
+__global__ void __launch_bounds__(kThreads) k_batch_keys(const BatchArgs a, const float4* scene,
+                                                         uint32_t n, uint32_t* keys) {
+  const BatchHeader h = *a.header;
+  const float4 sc = scene[0];
+  for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < n; t += gridDim.x * kThreads) {
+    const uint32_t ri = find_request(a.requests, h.n_requests, t);
+    const BatchRequest r = a.requests[ri];
+    const uint32_t local = t - r.tbase;
+    const bool points = r.flags & kReqPoints;
+    const float4 q = local < r.n ? load_query(r.queries, points, local) : make_float4(0, 0, 0, 0);
+    const uint32_t cls = (uint32_t)tile_class(r.flags) & 3u;
+    keys[t] = (cls << kMortonBits) | morton_of(0.5f * (q.x + q.z), 0.5f * (q.y + q.w), sc);
+  }
+}
+
 // Every class kernel in turn; a class without tiles exits at once.
 template <bool WRITE>
 void launch_batch_pass(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
@@ -250,7 +269,21 @@ void launch_batch_pass(const TreeView* t, const BatchArgs& a, hipStream_t stream
   launch_batch_kernel<WRITE, kClassLane>(t, a, stream);
 }
 
-hipError_t launch_batch(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
+hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSort& sort,
+                        hipStream_t stream) {
+  BatchArgs a = a_in;
+  a.order = nullptr;
+  if (sort.n > 1) {
+    const uint32_t blocks = (sort.n + kThreads - 1) / kThreads < 4096 ? (sort.n + kThreads - 1) / kThreads
+                                                                       : 4096;
+    hipLaunchKernelGGL(k_batch_keys, dim3(blocks), dim3(kThreads), 0, stream, a, sort.scene, sort.n,
+                       sort.keys_a);
+    bool in_b = false;
+    const hipError_t e = radix_sort_pairs(sort.keys_a, sort.vals_a, sort.keys_b, sort.vals_b, sort.n,
+                                          32, true, sort.ss, stream, &in_b);
+    if (e != hipSuccess) return e;
+    a.order = in_b ? sort.vals_b : sort.vals_a;
+  }
   // Phase events are only recorded outside stream capture (profiling off
   // or graphs off); a captured sequence is timed as a whole by its caller.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

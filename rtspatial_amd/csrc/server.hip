@@ -119,6 +119,7 @@ struct Slot {
   uint64_t* d_ranges = nullptr;
   unsigned long long* d_hash = nullptr;
   rts_pair* d_arena = nullptr;
+  uint32_t* d_sort = nullptr;  // Morton ordering of the wave tiles: 4 key/value arrays + scratch
   std::vector<Chunk> chunks;
   bool busy = false;
 };
@@ -333,6 +334,7 @@ BatchArgs slot_args(const Engine& E, const Slot& s) {
   a.ranges = s.d_ranges;
   a.arena = s.d_arena;
   a.arena_cap = E.cfg.arena_pairs;
+  a.order = nullptr;
   return a;
 }
 
@@ -340,7 +342,21 @@ BatchArgs slot_args(const Engine& E, const Slot& s) {
 int record_slot_work(Engine& E, Slot& s, size_t up_bytes, size_t n_req) {
   SRV_HIP(hipMemcpyAsync(s.d_desc, s.h_desc, up_bytes, hipMemcpyHostToDevice, s.stream));
   const BatchArgs a = slot_args(E, s);
-  SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, s.stream));
+  // Morton-order the wave tiles of large batches (not under graph capture,
+  // whose grids must not depend on the batch).
+  const BatchHeader* h = reinterpret_cast<const BatchHeader*>(s.h_desc + kHeaderOff);
+  const uint32_t wave_tiles = h->class_end[kClassWavePoints];
+  BatchSort sort{};
+  const uint64_t maxq = E.cfg.max_batch_queries;
+  sort.keys_a = s.d_sort;
+  sort.vals_a = s.d_sort + maxq;
+  sort.keys_b = s.d_sort + 2 * maxq;
+  sort.vals_b = s.d_sort + 3 * maxq;
+  sort_scratch_carve(s.d_sort + 4 * maxq, maxq, &sort.ss);
+  sort.ss.error = E.ix->error;  // a look-back that gives up is reported after the run
+  sort.scene = E.ix->scene;
+  sort.n = (!E.cfg.use_graphs && wave_tiles >= 65536) ? wave_tiles : 0;
+  SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, sort, s.stream));
   SRV_HIP(hipMemcpyAsync(s.h_ranges, s.d_ranges, 2 * n_req * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, s.stream));
   if (E.cfg.hash_results) {
@@ -371,6 +387,7 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipMalloc(&s.d_ranges, 2 * maxr * sizeof(uint64_t)));
   SRV_HIP(hipMalloc(&s.d_hash, maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_arena, E.cfg.arena_pairs * sizeof(rts_pair)));
+  SRV_HIP(hipMalloc(&s.d_sort, 4 * maxq * sizeof(uint32_t) + sort_scratch_bytes(maxq)));
   std::memset(s.h_desc, 0, E.desc_bytes);
   SRV_TRACE("slot buffers ready (arena %llu pairs)", (unsigned long long)E.cfg.arena_pairs);
   if (E.cfg.use_graphs) {
@@ -401,7 +418,7 @@ void slot_free(Slot& s) {
   if (s.h_hash) (void)hipHostFree(s.h_hash);
   for (void* p : {(void*)s.d_desc, (void*)s.d_counters, (void*)s.d_counts, (void*)s.d_offsets,
                   (void*)s.d_seg, (void*)s.d_total, (void*)s.d_ranges, (void*)s.d_hash,
-                  (void*)s.d_arena})
+                  (void*)s.d_arena, (void*)s.d_sort})
     if (p) (void)hipFree(p);
   if (s.done) (void)hipEventDestroy(s.done);
   if (s.stream) (void)hipStreamDestroy(s.stream);
