@@ -541,7 +541,7 @@ def run_c4(args):
                 ix.Query(R.Predicate.kContains, p, cnt)
                 ev[k][1].record()
                 if inserts:
-                    per.append(cnt.tensor().clone())
+                    per.append(cnt.tensor.clone())
             torch.cuda.synchronize()
             wall = time.perf_counter() - t0
             prof = R.profile_read(True)
