@@ -177,7 +177,8 @@ __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ l
                                                     const uint32_t* __restrict__ parent,
                                                     NodeRec* nodes, float4* __restrict__ root_box,
                                                     uint32_t* __restrict__ root_live,
-                                                    uint32_t* flags, uint32_t* error) {
+                                                    uint32_t* flags, uint32_t* error,
+                                                    uint32_t* boundary) {
   const uint32_t c0 = blockIdx.x * kRefitChunk;
   const uint32_t c1 = (c0 + kRefitChunk < n ? c0 + kRefitChunk : n) - 1;  // inclusive
   for (int r = 0; r < kRefitPerThread; r++) {
@@ -209,11 +210,17 @@ __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ l
       np->box[side] = b;
       if (side) np->live.y = live; else np->live.x = live;
       const uint4 mp = np->meta;
-      const bool local = mp.z >= c0 && mp.w <= c1;
+      if (!(mp.z >= c0 && mp.w <= c1)) {
+        // The node spans chunks: k_refit_top finishes it (and everything
+        // above) after this kernel, so no agent-scope ordering is needed.
+        const uint32_t slot = atomicAdd(&boundary[0], 1u);
+        if (1 + (uint64_t)slot < refit_boundary_words(n)) boundary[1 + slot] = code;
+        else atomicOr(error, 2u);
+        break;
+      }
       // Release our child's bounds; acquire the sibling's if we arrive second.
       const uint32_t old =
-          local ? __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP)
-                : __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old == 0) break;
       const float4 o = np->box[side ^ 1u];
       live += side ? np->live.x : np->live.y;
@@ -224,6 +231,46 @@ __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ l
         break;
       }
       code = parent[p];
+    }
+  }
+}
+
+// The nodes that span chunks, bottom-up, in one workgroup: each boundary
+// code names a child slot that k_refit filled; the second arrival at a node
+// unites both slots and fills the node's slot in its parent.
+constexpr int kRefitTopThreads = 1024;
+
+__global__ void __launch_bounds__(kRefitTopThreads) k_refit_top(const uint32_t* __restrict__ parent,
+                                                              NodeRec* nodes, uint32_t n,
+                                                              float4* __restrict__ root_box,
+                                                              uint32_t* __restrict__ root_live,
+                                                              uint32_t* flags, uint32_t* error,
+                                                              const uint32_t* boundary) {
+  const uint64_t cap = refit_boundary_words(n) - 1;
+  const uint32_t cnt = boundary[0] < cap ? boundary[0] : (uint32_t)cap;
+  for (uint32_t j = threadIdx.x; j < cnt; j += kRefitTopThreads) {
+    uint32_t code = boundary[1 + j];
+    for (int level = 0;; level++) {
+      const uint32_t p = code >> 1;
+      if (level >= kMaxLevels || p >= n - 1) {
+        atomicOr(error, 2u);
+        break;
+      }
+      NodeRec* np = nodes + p;
+      const uint32_t old =
+          __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      const float4 b = unite(np->box[0], np->box[1]);
+      const uint32_t live = np->live.x + np->live.y;
+      if (p == 0) {
+        root_box[0] = b;
+        root_live[0] = live;
+        break;
+      }
+      code = parent[p];
+      NodeRec* pp = nodes + (code >> 1);
+      pp->box[code & 1u] = b;
+      if (code & 1u) pp->live.y = live; else pp->live.x = live;
     }
   }
 }
@@ -371,9 +418,14 @@ hipError_t launch_refit(const RefitArgs& a, hipStream_t stream) {
   if (n > 1)
     hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, a.flags,
                        n - 1);
+  hipError_t e = launch_fill_u32(a.boundary, 0u, 1, stream);
+  if (e != hipSuccess) return e;
   const uint32_t blocks = (n + kRefitChunk - 1) / kRefitChunk;
   hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(kThreads), 0, stream, a.leaf_boxes, n, a.parent,
-                     a.nodes, a.root_box, a.root_live, a.flags, a.error);
+                     a.nodes, a.root_box, a.root_live, a.flags, a.error, a.boundary);
+  if (n > 1)
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(kRefitTopThreads), 0, stream, a.parent, a.nodes,
+                       n, a.root_box, a.root_live, a.flags, a.error, a.boundary);
   return hipGetLastError();
 }
 

@@ -106,7 +106,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   RTS_HIP(dev_alloc(&ix->vals_alt, cap));
   RTS_HIP(dev_alloc(&ix->nodes, cap));
   RTS_HIP(dev_alloc(&ix->parent, 2 * cap));
-  RTS_HIP(dev_alloc(&ix->flags, cap));
+  RTS_HIP(dev_alloc(&ix->flags, cap + refit_boundary_words(cap)));
   RTS_HIP(dev_alloc(&ix->sort_scratch, sort_scratch_bytes(cap)));
   sort_scratch_carve(ix->sort_scratch, cap, &ix->ss);
   ix->n_partials_cap = scene_bounds_partials((uint32_t)cap);
@@ -163,6 +163,7 @@ RefitArgs refit_args(rts_index* ix, uint32_t n) {
   a.root_live = ix->root_live;
   a.flags = ix->flags;
   a.error = ix->error;
+  a.boundary = ix->flags + ix->tcap;  // after the n - 1 node flags
   return a;
 }
 
