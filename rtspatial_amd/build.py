@@ -92,6 +92,24 @@ def build_cpp_tests(verbose=False):
     return outs
 
 
+def build_cli(verbose=False):
+    """Compiles the command-line front end (rtspatial_amd/cli) into
+    rtspatial_amd/bin/, linked against the library next to it."""
+    src = os.path.join(HERE, "cli", "rts_query.cpp")
+    bin_dir = os.path.join(HERE, "bin")
+    os.makedirs(bin_dir, exist_ok=True)
+    out = os.path.join(bin_dir, "rts_query")
+    cmd = [hipcc(), "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", src, "-o",
+           out + ".tmp", f"-L{HERE}", "-lrtspatial_amd", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building rts_query failed:\n{r.stdout}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def _wait(item):
     name, p = item
     out, _ = p.communicate()
@@ -108,3 +126,4 @@ def _wait(item):
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_cpp_tests(verbose="-v" in sys.argv))
+    print(build_cli(verbose="-v" in sys.argv))
