@@ -13,8 +13,10 @@ the plain path's capacity; both capacities and latencies are reported.
 
 --workload c3: one plain Query(kIntersects) of 1M skewed query envelopes over
 the 10M boxes per step, all ~1e10 pairs collected on the device.
+--workload c3f64: the same on a double-coordinate index (SpatialIndex<double, 2>,
+exact double predicates), boxes and queries drawn in double.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served|c3]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served|c3|c3f64|c4]
 """
 import argparse
 import json
@@ -104,6 +106,25 @@ def cpu_baseline(boxes, samples, threads):
                    f"R-tree bulk load {boxes.shape[0] / t_build / 1e6:.3f} Mboxes/s single-threaded; "
                    f"brute-force scan of {qb.shape[0]} queries x {boxes.shape[0]} boxes: "
                    f"{qb.shape[0] / t_bf / 1e6:.6f} Mqueries/s ({n_bf} results)"),
+    }
+
+
+def cpu_baseline_f64(boxes, pred, queries, threads):
+    """The oracle's exact double brute-force join (it has no double R-tree)
+    on a sample of the workload's queries."""
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    n = O.bf_count64(pred, boxes, queries, threads=threads)
+    t = time.perf_counter() - t0
+    return {
+        "value": round(queries.shape[0] / t / 1e6, 6),
+        "unit": "Mqueries/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle brute-force double join: {queries.shape[0]} queries drawn from the "
+                   f"workload x all {boxes.shape[0]} boxes ({n} results, counted) on "
+                   f"{threads} threads"),
     }
 
 
@@ -408,12 +429,15 @@ def run_c3(args):
     from rtspatial_amd import workloads as W
 
     dev = torch.device("cuda:0")
-    log(f"generating workload: {args.boxes} boxes, {args.queries} queries")
-    boxes_np, queries_np = W.config3(args.boxes, args.queries)
+    f64 = args.workload == "c3f64"
+    log(f"generating workload: {args.boxes} boxes, {args.queries} queries, "
+        f"{'double' if f64 else 'float'} coordinates")
+    boxes_np, queries_np = W.config3(args.boxes, args.queries,
+                                     dtype=np.float64 if f64 else np.float32)
     boxes = torch.from_numpy(boxes_np).to(dev)
     queries = torch.from_numpy(queries_np).to(dev)
     pred = R.Predicate.kIntersects
-    ix = R.SpatialIndex()
+    ix = R.SpatialIndex(torch.float64 if f64 else torch.float32)
     ix.Init(R.Config(max_geometries=args.boxes))
     build = build_rate(R, ix, boxes, torch)
     log(f"build: {build}")
@@ -451,10 +475,10 @@ def run_c3(args):
     nq = queries.shape[0]
     trav_n, trav_ms = prof["traverse"]
     trav_avg = trav_ms / max(trav_n, 1)
-    qbytes = query_bytes(nq, 16, n_pairs, args.boxes)
+    qbytes = query_bytes(nq, 32 if f64 else 16, n_pairs, args.boxes)
     achieved = qbytes / (trav_avg / 1e3) / 1e9
-    traffic, traffic_src = profiled_traffic("c3") if args.boxes == 10_000_000 and nq == 1_000_000 \
-        else (None, None)
+    traffic, traffic_src = profiled_traffic(args.workload) \
+        if args.boxes == 10_000_000 and nq == 1_000_000 else (None, None)
     out = {
         "metric": METRIC,
         "value": round(nq / (ms_per_step / 1e3) / 1e6, 4),
@@ -466,11 +490,12 @@ def run_c3(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64" if f64 else "f32",
         "data": "synthetic",
         "config": {
             "workload": (f"range-intersects, skewed: {args.boxes} Gaussian-mixture boxes x {nq} "
-                         f"query envelopes; plain per-batch LBVH, one Query(kIntersects) per step, "
+                         f"query envelopes{' with double coordinates (exact double predicates)' if f64 else ''}; "
+                         f"plain per-batch LBVH, one Query(kIntersects) per step, "
                          f"all pairs collected on the device"),
             "path": "plain", "boxes": args.boxes, "queries": nq, "pairs_per_step": n_pairs,
             "seeds": {"boxes": 3001, "queries": 3002},
@@ -489,8 +514,13 @@ def run_c3(args):
     }
     if not args.no_cpu_baseline:
         rs = np.random.default_rng(99)
-        out["cpu_baseline"] = cpu_baseline(
-            boxes_np, [(int(pred), queries_np[rs.choice(nq, 20000, replace=False)])], cpu_threads())
+        if f64:
+            out["cpu_baseline"] = cpu_baseline_f64(
+                boxes_np, int(pred), queries_np[rs.choice(nq, 1000, replace=False)], cpu_threads())
+        else:
+            out["cpu_baseline"] = cpu_baseline(
+                boxes_np, [(int(pred), queries_np[rs.choice(nq, 20000, replace=False)])],
+                cpu_threads())
     else:
         out["cpu_baseline"] = None
     return out
@@ -620,7 +650,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["served", "c3", "c4", "copy"], default="served")
+    ap.add_argument("--workload", choices=["served", "c3", "c3f64", "c4", "copy"], default="served")
     ap.add_argument("--rounds", type=int, default=20)
     ap.add_argument("--boxes", type=int, default=10_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -639,7 +669,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    out = {"served": run_served, "c3": run_c3, "c4": run_c4,
+    out = {"served": run_served, "c3": run_c3, "c3f64": run_c3, "c4": run_c4,
            "copy": run_copy_calibration}[args.workload](args)
     print(json.dumps(out), flush=True)
     check = out.get("bit_exact_check")

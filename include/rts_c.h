@@ -1,8 +1,8 @@
 /*
  * rts_c.h -- C ABI of the MI355X-native LibRTS spatial index.
  *
- * This is the drop-in boundary for rtspatial::SpatialIndex<float, 2>
- * (reference: include/rtspatial/spatial_index.cuh:62-931). Every entry point
+ * This is the drop-in boundary for rtspatial::SpatialIndex<float, 2> and
+ * SpatialIndex<double, 2> (reference: include/rtspatial/spatial_index.cuh:62-931). Every entry point
  * is extern "C", takes plain pointers and sizes, returns an int status
  * (RTS_OK = 0) and never throws; rts_last_error() gives the message of the
  * calling thread's last failure. The C++ facade in include/rtspatial/ wraps
@@ -55,6 +55,22 @@ typedef struct rts_update {
   uint64_t id;
   rts_envelope box;
 } rts_update;
+
+/* Point<double,2> (geom/point.cuh:44-75): 16 bytes. */
+typedef struct rts_point_d {
+  double x, y;
+} rts_point_d;
+
+/* Envelope<Point<double,2>>: {min, max}, 32 bytes. */
+typedef struct rts_envelope_d {
+  double x0, y0, x1, y1;
+} rts_envelope_d;
+
+/* thrust::pair<size_t, Envelope<Point<double,2>>>: 40 bytes. */
+typedef struct rts_update_d {
+  uint64_t id;
+  rts_envelope_d box;
+} rts_update_d;
 
 /* One result of the collecting handler, in its order
  * (default_handlers/collecting_handlers.h:9-15): geom_id first. */
@@ -115,6 +131,20 @@ typedef enum rts_traversal {
   RTS_TRAVERSAL_WAVE = 2    /* one query per wavefront, lanes share its nodes and leaves */
 } rts_traversal;
 
+/* Coordinate type of an index: SpatialIndex<float,2> or SpatialIndex<double,2>. */
+typedef enum rts_coord {
+  RTS_COORD_F32 = 0,
+  /* Double coordinates, exact double predicates. The tree holds FP32 bounds
+   * rounded outward (the reference's 2-ulp widening, geom/envelope.cuh:131-143,
+   * utils/helpers.h:26-41); a leaf test that FP32 cannot decide reads the
+   * double box. */
+  RTS_COORD_F64 = 1,
+  /* Double coordinates rounded to the nearest float, then FP32 predicates: the
+   * results of the reference's SpatialIndex<double,2>, whose
+   * Point<double,2>::get_coordinate returns float (geom/point.cuh:68). */
+  RTS_COORD_F64_AS_F32 = 2
+} rts_coord;
+
 /* Config (spatial_index.cuh:22-37). OptiX-only fields are accepted and
  * ignored; the sampling fields drive the selectivity probe of the tuner. */
 typedef struct rts_config {
@@ -136,6 +166,8 @@ typedef struct rts_config {
   float sah_rebuild_ratio;       /* 1.5 */
   uint32_t traversal;            /* rts_traversal (AUTO) */
   uint32_t sort_queries;         /* 0 = auto, 1 = Morton-order queries, 2 = keep input order */
+  uint32_t coord;                /* rts_coord (RTS_COORD_F32). A float index takes the rts_index_*
+                                    calls below, a double one the rts_index_*_d calls. */
 } rts_config;
 
 typedef struct rts_index rts_index;
@@ -175,6 +207,21 @@ int rts_index_query_points(rts_index* index, rts_predicate pred, const rts_point
 int rts_index_query_envelopes(rts_index* index, rts_predicate pred, const rts_envelope* queries,
                               uint64_t n, rts_handler handler, void* arg, rts_stream_t stream);
 
+/* The same four calls for an index created with coord RTS_COORD_F64 or
+ * RTS_COORD_F64_AS_F32 (SpatialIndex<double,2>: spatial_index.cuh:142-515
+ * instantiated with COORD_T = double). The float calls on a double index, and
+ * these on a float index, return RTS_ERR_INVALID. rts_index_delete takes ids
+ * and serves both. */
+int rts_index_insert_d(rts_index* index, const rts_envelope_d* boxes, uint64_t n,
+                       rts_stream_t stream);
+int rts_index_update_d(rts_index* index, const rts_update_d* updates, uint64_t n,
+                       rts_stream_t stream);
+int rts_index_query_points_d(rts_index* index, rts_predicate pred, const rts_point_d* queries,
+                             uint64_t n, rts_handler handler, void* arg, rts_stream_t stream);
+int rts_index_query_envelopes_d(rts_index* index, rts_predicate pred,
+                                const rts_envelope_d* queries, uint64_t n, rts_handler handler,
+                                void* arg, rts_stream_t stream);
+
 /* Number of ids handed out so far (live and deleted). */
 uint64_t rts_index_size(const rts_index* index);
 
@@ -202,13 +249,14 @@ int rts_index_get_stats(rts_index* index, rts_index_stats* out);
  * CalculateBestParallelism sampling (spatial_index.cuh:41-57,690-760) picks a
  * ray parallelism. It times the query over an evenly strided sample of
  * `queries` under every candidate plan -- leaf size, one query per lane or
- * per wavefront, Morton-ordered or input-ordered queries -- with the given
- * handler. With `apply` set, it stores the fastest plan in the index for this
+ * per wavefront, and, when the sample is the whole set, Morton-ordered or
+ * input-ordered queries -- with the given handler. With `apply` set, it stores the fastest plan in the index for this
  * (predicate, query kind); later queries of that kind, and the batches of
  * rts_serve_run, use it for every setting their config leaves on auto. The
  * plan survives rts_index_clear. Results are identical under every plan;
  * only the time differs. `queries` is rts_point* when `points` is 1, else
- * rts_envelope*, in device or pinned host memory. Synchronous. */
+ * rts_envelope* (rts_point_d* / rts_envelope_d* for a double index), in
+ * device or pinned host memory. Synchronous. */
 typedef struct rts_tune_config {
   uint32_t sample;      /* queries probed (32768; all of them if fewer) */
   uint32_t leaf_sizes;  /* bit k set: try leaf size 2^k (bits 2..5: 4, 8, 16, 32) */
@@ -219,7 +267,8 @@ typedef struct rts_tune_config {
 typedef struct rts_tune_result {
   uint32_t leaf_size;      /* the winner */
   uint32_t traversal;      /* RTS_TRAVERSAL_THREAD or RTS_TRAVERSAL_WAVE */
-  uint32_t sort_queries;   /* 1 = Morton-ordered, 2 = input order */
+  uint32_t sort_queries;   /* 1 = Morton-ordered, 2 = input order; 0 = not tuned (the
+                              sample was smaller than the query set), the call's size decides */
   uint32_t plans;          /* plans timed */
   uint64_t sample;         /* queries in the sample */
   uint64_t sample_results; /* results of the sample */
@@ -252,11 +301,14 @@ int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* k
  *     on separate streams; whenever a slot is free, every arrived request
  *     (split into chunks when large) is admitted while its predicted result
  *     bytes fit the slot's arena, and the batch runs as one persistent
- *     work-stealing launch sequence (count, scan, write), replayed from a
- *     hipGraph per slot. Chunks whose results overflow the arena are
- *     re-launched.
+ *     work-stealing launch sequence (count, scan, write); with use_graphs the
+ *     sequence is replayed from a hipGraph per slot. Chunks whose results
+ *     overflow the arena are re-launched.
  *   RTS_SERVE_PLAIN: the per-request path: each request (split the same way)
  *     is one rts_index_query_* call on one stream, in arrival order.
+ *
+ * The index must hold float coordinates (RTS_COORD_F32); a double index is
+ * RTS_ERR_INVALID here.
  * ------------------------------------------------------------------------ */
 typedef enum rts_serve_mode {
   RTS_SERVE_BATCHED = 0,

@@ -51,6 +51,7 @@ class Envelope {
 };
 
 static_assert(sizeof(Envelope<Point<float, 2>>) == 16, "Envelope is {min, max}");
+static_assert(sizeof(Envelope<Point<double, 2>>) == 32, "Envelope<double> is {min, max}");
 
 }  // namespace rtspatial
 #endif

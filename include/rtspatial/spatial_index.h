@@ -1,6 +1,6 @@
-// rtspatial::SpatialIndex<float, 2> -- the reference's C++ surface
-// (include/rtspatial/spatial_index.cuh:20-931) as a header-only facade over
-// the C ABI in rts_c.h. Method names, argument meanings and id semantics
+// rtspatial::SpatialIndex<float, 2> and <double, 2> -- the reference's C++
+// surface (include/rtspatial/spatial_index.cuh:20-931) as a header-only facade
+// over the C ABI in rts_c.h. Method names, argument meanings and id semantics
 // follow the reference; streams are hipStream_t; failures throw
 // rtspatial::Exception (the reference throws on CUDA/OptiX errors and aborts
 // on a bad predicate, spatial_index.cuh:387-389).
@@ -52,18 +52,75 @@ struct Config {
   float sah_rebuild_ratio = 1.5f;
   uint32_t traversal = RTS_TRAVERSAL_AUTO;
   uint32_t sort_queries = 0;
+  // SpatialIndex<double, 2> only: RTS_COORD_F64 (exact double predicates) or
+  // RTS_COORD_F64_AS_F32 (coordinates rounded to nearest floats: the
+  // reference's results, whose double points return float coordinates).
+  uint32_t coord = RTS_COORD_F64;
 };
+
+namespace detail {
+// The C entry points of each coordinate type.
+template <typename COORD_T>
+struct CoordAbi;
+template <>
+struct CoordAbi<float> {
+  using c_point = rts_point;
+  using c_envelope = rts_envelope;
+  using c_update = rts_update;
+  static uint32_t coord(uint32_t) { return RTS_COORD_F32; }
+  static int insert(rts_index* i, const c_envelope* b, uint64_t n, hipStream_t s) {
+    return rts_index_insert(i, b, n, s);
+  }
+  static int update(rts_index* i, const c_update* u, uint64_t n, hipStream_t s) {
+    return rts_index_update(i, u, n, s);
+  }
+  static int query(rts_index* i, rts_predicate p, const c_point* q, uint64_t n, rts_handler h,
+                   void* a, hipStream_t s) {
+    return rts_index_query_points(i, p, q, n, h, a, s);
+  }
+  static int query(rts_index* i, rts_predicate p, const c_envelope* q, uint64_t n, rts_handler h,
+                   void* a, hipStream_t s) {
+    return rts_index_query_envelopes(i, p, q, n, h, a, s);
+  }
+};
+template <>
+struct CoordAbi<double> {
+  using c_point = rts_point_d;
+  using c_envelope = rts_envelope_d;
+  using c_update = rts_update_d;
+  static uint32_t coord(uint32_t requested) { return requested; }
+  static int insert(rts_index* i, const c_envelope* b, uint64_t n, hipStream_t s) {
+    return rts_index_insert_d(i, b, n, s);
+  }
+  static int update(rts_index* i, const c_update* u, uint64_t n, hipStream_t s) {
+    return rts_index_update_d(i, u, n, s);
+  }
+  static int query(rts_index* i, rts_predicate p, const c_point* q, uint64_t n, rts_handler h,
+                   void* a, hipStream_t s) {
+    return rts_index_query_points_d(i, p, q, n, h, a, s);
+  }
+  static int query(rts_index* i, rts_predicate p, const c_envelope* q, uint64_t n, rts_handler h,
+                   void* a, hipStream_t s) {
+    return rts_index_query_envelopes_d(i, p, q, n, h, a, s);
+  }
+};
+}  // namespace detail
 
 template <typename COORD_T, int N_DIMS>
 class SpatialIndex;
 
-template <>
-class SpatialIndex<float, 2> {
+template <typename COORD_T>
+class SpatialIndex<COORD_T, 2> {
+  using abi = detail::CoordAbi<COORD_T>;
+
  public:
-  using point_t = Point<float, 2>;
+  using point_t = Point<COORD_T, 2>;
   using envelope_t = Envelope<point_t>;
   using update_t = pair<size_t, envelope_t>;
-  static_assert(sizeof(update_t) == sizeof(rts_update), "pair<size_t, envelope> is 24 bytes");
+  static_assert(sizeof(point_t) == sizeof(typename abi::c_point), "point layout");
+  static_assert(sizeof(envelope_t) == sizeof(typename abi::c_envelope), "envelope layout");
+  static_assert(sizeof(update_t) == sizeof(typename abi::c_update),
+                "pair<size_t, envelope> is rts_update / rts_update_d");
 
   SpatialIndex() = default;
   ~SpatialIndex() {
@@ -97,6 +154,7 @@ class SpatialIndex<float, 2> {
     c.sah_rebuild_ratio = config.sah_rebuild_ratio;
     c.traversal = config.traversal;
     c.sort_queries = config.sort_queries;
+    c.coord = abi::coord(config.coord);
     check(rts_index_create(&c, &ix_), "SpatialIndex::Init");
   }
 
@@ -105,8 +163,8 @@ class SpatialIndex<float, 2> {
 
   // spatial_index.cuh:142-196
   void Insert(ArrayView<envelope_t> envelopes, hipStream_t stream = nullptr) {
-    check(rts_index_insert(handle(), reinterpret_cast<const rts_envelope*>(envelopes.data()),
-                           envelopes.size(), stream),
+    check(abi::insert(handle(), reinterpret_cast<const typename abi::c_envelope*>(envelopes.data()),
+                      envelopes.size(), stream),
           "SpatialIndex::Insert");
   }
 
@@ -120,8 +178,8 @@ class SpatialIndex<float, 2> {
 
   // spatial_index.cuh:280-336
   void Update(const ArrayView<update_t> updates, hipStream_t stream) {
-    check(rts_index_update(handle(), reinterpret_cast<const rts_update*>(updates.data()),
-                           updates.size(), stream),
+    check(abi::update(handle(), reinterpret_cast<const typename abi::c_update*>(updates.data()),
+                      updates.size(), stream),
           "SpatialIndex::Update");
   }
 
@@ -137,16 +195,16 @@ class SpatialIndex<float, 2> {
   // The same with the handler named at run time.
   void Query(Predicate p, ArrayView<point_t> queries, rts_handler handler, void* arg,
              hipStream_t stream) {
-    check(rts_index_query_points(handle(), static_cast<rts_predicate>(p),
-                                 reinterpret_cast<const rts_point*>(queries.data()), queries.size(),
-                                 handler, arg, stream),
+    check(abi::query(handle(), static_cast<rts_predicate>(p),
+                     reinterpret_cast<const typename abi::c_point*>(queries.data()), queries.size(),
+                     handler, arg, stream),
           "SpatialIndex::Query(points)");
   }
   void Query(Predicate p, ArrayView<envelope_t> queries, rts_handler handler, void* arg,
              hipStream_t stream) {
-    check(rts_index_query_envelopes(handle(), static_cast<rts_predicate>(p),
-                                    reinterpret_cast<const rts_envelope*>(queries.data()),
-                                    queries.size(), handler, arg, stream),
+    check(abi::query(handle(), static_cast<rts_predicate>(p),
+                     reinterpret_cast<const typename abi::c_envelope*>(queries.data()),
+                     queries.size(), handler, arg, stream),
           "SpatialIndex::Query(envelopes)");
   }
 

@@ -10,6 +10,8 @@
 //   * the reference tests' input generators  test/test_commons.h:22-67,
 //                                            test/test_envelope_queries.h:121-134
 //   * a brute-force join and an STR R-tree join over those predicates
+//   * the same predicates and brute-force join in double precision
+//     (SpatialIndex<double, 2>), exact
 //   * the reference's ray-slab formulation of intersects
 //                                            include/rtspatial/details/ray_params.h:53-167,
 //                                            src/shaders/shaders_intersects_envelope_query_2d.cu:14-128
@@ -23,6 +25,9 @@
 //   ray formulation          : reproduces 6549771 (test_envelope_queries.h:57,90) and
 //                              differs from the exact join by exactly one pair.
 //   R-tree                   : equality with brute force on seeded inputs.
+//   double predicates/join   : known answer 5 (test_point_queries.h:92-120);
+//                              equality with the float join on float-exact inputs;
+//                              tiny cases whose answer flips under float rounding.
 //   generators               : the known answers above are only reached with
 //                              the exact libstdc++ mt19937 streams.
 //   hash/count summaries     : equality with summaries recomputed from full pair sets.
@@ -77,6 +82,27 @@ inline bool eval(int pred, int kind, const Box& g, const float* q) {
     case P_CONTAINS: return box_contains_box(g, qb);  // reference kContains: g contains q
     default: return box_contains_box(qb, g);          // within: q contains g
   }
+}
+
+// The same closed forms in double precision, for SpatialIndex<double, 2>
+// (envelope.cuh:29-69 with COORD_T = double). The reference's double Point
+// returns float coordinates to these predicates (point.cuh:68); this oracle
+// evaluates them exactly, which is what a double index promises. The
+// reference's rounded behaviour is the float oracle on rounded inputs.
+struct Box64 {
+  double x0, y0, x1, y1;
+};
+inline bool eval64(int pred, int kind, const Box64& g, const double* q) {
+  if (kind == Q_POINT) return q[0] >= g.x0 && q[0] <= g.x1 && q[1] >= g.y0 && q[1] <= g.y1;
+  switch (pred) {
+    case P_INTERSECTS: return q[0] <= g.x1 && q[2] >= g.x0 && q[1] <= g.y1 && q[3] >= g.y0;
+    case P_CONTAINS: return q[0] >= g.x0 && q[2] <= g.x1 && q[1] >= g.y0 && q[3] <= g.y1;
+    default: return g.x0 >= q[0] && g.x1 <= q[2] && g.y0 >= q[1] && g.y1 <= q[3];
+  }
+}
+inline bool eval_t(int pred, int kind, const Box& g, const float* q) { return eval(pred, kind, g, q); }
+inline bool eval_t(int pred, int kind, const Box64& g, const double* q) {
+  return eval64(pred, kind, g, q);
 }
 
 inline uint64_t splitmix64(uint64_t x) {
@@ -388,24 +414,29 @@ int orc_eval(int pred, int kind, const float* box, const float* query) {
 
 uint64_t orc_splitmix64(uint64_t x) { return splitmix64(x); }
 
+}  // extern "C"
+
+namespace {
+
 // ---------------------------------------------------------- brute force ----
 // Counts pairs; per_query (nullable) receives each query's count, hsum/hxor
 // (nullable) each query's sum / xor of splitmix64((q << 32) | g).
-uint64_t orc_bf_summary(int pred, int kind, const float* boxes, uint64_t nb, const uint8_t* alive,
-                        const float* queries, uint64_t nq, int threads, uint64_t* per_query,
-                        uint64_t* hsum, uint64_t* hxor) {
-  const Box* b = reinterpret_cast<const Box*>(boxes);
+template <typename BoxT, typename T>
+uint64_t bf_summary(int pred, int kind, const T* boxes, uint64_t nb, const uint8_t* alive,
+                    const T* queries, uint64_t nq, int threads, uint64_t* per_query,
+                    uint64_t* hsum, uint64_t* hxor) {
+  const BoxT* b = reinterpret_cast<const BoxT*>(boxes);
   int qs = kind == Q_POINT ? 2 : 4;
   int nt = resolve_threads(threads);
   std::vector<uint64_t> part(nt, 0);
   parallel_dynamic(nq, nt, 64, [&](uint64_t qb, uint64_t qe, int t) {
     uint64_t tot = 0;
     for (uint64_t q = qb; q < qe; q++) {
-      const float* qq = queries + q * qs;
+      const T* qq = queries + q * qs;
       uint64_t c = 0, s = 0, x = 0;
       for (uint64_t g = 0; g < nb; g++) {
         if (alive && !alive[g]) continue;
-        if (eval(pred, kind, b[g], qq)) {
+        if (eval_t(pred, kind, b[g], qq)) {
           c++;
           if (hsum) {
             uint64_t h = splitmix64((q << 32) | g);
@@ -428,19 +459,20 @@ uint64_t orc_bf_summary(int pred, int kind, const float* boxes, uint64_t nb, con
 
 // Fills out[2*i] = query_id, out[2*i+1] = box_id sorted by (query, box).
 // Returns the total number of pairs; writes at most `cap` pairs.
-uint64_t orc_bf_pairs(int pred, int kind, const float* boxes, uint64_t nb, const uint8_t* alive,
-                      const float* queries, uint64_t nq, int threads, uint32_t* out, uint64_t cap) {
-  const Box* b = reinterpret_cast<const Box*>(boxes);
+template <typename BoxT, typename T>
+uint64_t bf_pairs(int pred, int kind, const T* boxes, uint64_t nb, const uint8_t* alive,
+                  const T* queries, uint64_t nq, int threads, uint32_t* out, uint64_t cap) {
+  const BoxT* b = reinterpret_cast<const BoxT*>(boxes);
   int qs = kind == Q_POINT ? 2 : 4;
   int nt = resolve_threads(threads);
   std::vector<std::vector<uint32_t>> local(nt);
   parallel_ranges(nq, nt, [&](uint64_t qb, uint64_t qe, int t) {
     auto& v = local[t];
     for (uint64_t q = qb; q < qe; q++) {
-      const float* qq = queries + q * qs;
+      const T* qq = queries + q * qs;
       for (uint64_t g = 0; g < nb; g++) {
         if (alive && !alive[g]) continue;
-        if (eval(pred, kind, b[g], qq)) {
+        if (eval_t(pred, kind, b[g], qq)) {
           v.push_back((uint32_t)q);
           v.push_back((uint32_t)g);
         }
@@ -457,6 +489,37 @@ uint64_t orc_bf_pairs(int pred, int kind, const float* boxes, uint64_t nb, const
     }
   }
   return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t orc_bf_summary(int pred, int kind, const float* boxes, uint64_t nb, const uint8_t* alive,
+                        const float* queries, uint64_t nq, int threads, uint64_t* per_query,
+                        uint64_t* hsum, uint64_t* hxor) {
+  return bf_summary<Box>(pred, kind, boxes, nb, alive, queries, nq, threads, per_query, hsum,
+                         hxor);
+}
+uint64_t orc_bf_pairs(int pred, int kind, const float* boxes, uint64_t nb, const uint8_t* alive,
+                      const float* queries, uint64_t nq, int threads, uint32_t* out, uint64_t cap) {
+  return bf_pairs<Box>(pred, kind, boxes, nb, alive, queries, nq, threads, out, cap);
+}
+// Double coordinates, exact double predicates.
+uint64_t orc_bf_summary_f64(int pred, int kind, const double* boxes, uint64_t nb,
+                            const uint8_t* alive, const double* queries, uint64_t nq, int threads,
+                            uint64_t* per_query, uint64_t* hsum, uint64_t* hxor) {
+  return bf_summary<Box64>(pred, kind, boxes, nb, alive, queries, nq, threads, per_query, hsum,
+                           hxor);
+}
+uint64_t orc_bf_pairs_f64(int pred, int kind, const double* boxes, uint64_t nb,
+                          const uint8_t* alive, const double* queries, uint64_t nq, int threads,
+                          uint32_t* out, uint64_t cap) {
+  return bf_pairs<Box64>(pred, kind, boxes, nb, alive, queries, nq, threads, out, cap);
+}
+int orc_eval_f64(int pred, int kind, const double* box, const double* query) {
+  Box64 g{box[0], box[1], box[2], box[3]};
+  return eval64(pred, kind, g, query) ? 1 : 0;
 }
 
 // -------------------------------------------------------------- R-tree ----

@@ -10,6 +10,9 @@ Array conventions (numpy):
   alive    uint8   [n] or None (deleted boxes are 0)
   pairs    uint32  [k, 2]  (query_id, box_id), sorted by (query_id, box_id)
 
+Double-coordinate indexes (SpatialIndex<double, 2>) are checked with the
+*64 functions: float64 arrays, exact double predicates.
+
 Predicates: INTERSECTS = 0, CONTAINS = 1 (data box contains the query,
 spatial_index.cuh:414-441 with shaders_contains_envelope_query_2d.cu:22),
 WITHIN = 2 (data box inside the query envelope).
@@ -45,6 +48,12 @@ def lib():
         L.orc_bf_summary.restype = u64
         L.orc_bf_pairs.argtypes = [i32, i32, vp, u64, vp, vp, u64, i32, vp, u64]
         L.orc_bf_pairs.restype = u64
+        L.orc_bf_summary_f64.argtypes = [i32, i32, vp, u64, vp, vp, u64, i32, vp, vp, vp]
+        L.orc_bf_summary_f64.restype = u64
+        L.orc_bf_pairs_f64.argtypes = [i32, i32, vp, u64, vp, vp, u64, i32, vp, u64]
+        L.orc_bf_pairs_f64.restype = u64
+        L.orc_eval_f64.argtypes = [i32, i32, vp, vp]
+        L.orc_eval_f64.restype = i32
         L.orc_rtree_build.argtypes = [vp, u64, vp, i32]
         L.orc_rtree_build.restype = vp
         L.orc_rtree_free.argtypes = [vp]
@@ -144,6 +153,51 @@ def bf_pairs(pred, boxes, queries, alive=None, threads=0):
     out = np.empty((max(total, 1), 2), np.uint32)
     got = lib().orc_bf_pairs(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
                              _p(queries), queries.shape[0], threads, _p(out), total)
+    assert got == total
+    return out[:total]
+
+
+# ------------------------------------------------- double coordinates --
+# SpatialIndex<double, 2>: boxes float64 [n, 4], points float64 [n, 2], the
+# predicates evaluated exactly in double precision.
+def _f64(a, cols):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.ndim == 2 and a.shape[1] == cols, a.shape
+    return a
+
+
+def evaluate64(pred, box, query):
+    b = np.ascontiguousarray(box, np.float64)
+    q = np.ascontiguousarray(query, np.float64)
+    return bool(lib().orc_eval_f64(pred, POINT if q.size == 2 else ENVELOPE, _p(b), _p(q)))
+
+
+def bf_summary64(pred, boxes, queries, alive=None, threads=0, want_hash=True):
+    boxes = _f64(boxes, 4)
+    queries = np.ascontiguousarray(queries, dtype=np.float64)
+    nq = queries.shape[0]
+    per_q = np.zeros(nq, np.uint64)
+    hs = np.zeros(nq, np.uint64) if want_hash else None
+    hx = np.zeros(nq, np.uint64) if want_hash else None
+    al = _alive(alive, boxes.shape[0])
+    total = lib().orc_bf_summary_f64(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
+                                     _p(queries), nq, threads, _p(per_q), _p(hs), _p(hx))
+    return int(total), per_q, hs, hx
+
+
+def bf_count64(pred, boxes, queries, alive=None, threads=0):
+    return bf_summary64(pred, boxes, queries, alive, threads, want_hash=False)[0]
+
+
+def bf_pairs64(pred, boxes, queries, alive=None, threads=0):
+    """(query_id, box_id) pairs sorted by (query, box), exact double predicates."""
+    boxes = _f64(boxes, 4)
+    queries = np.ascontiguousarray(queries, dtype=np.float64)
+    al = _alive(alive, boxes.shape[0])
+    total = bf_count64(pred, boxes, queries, alive, threads)
+    out = np.empty((max(total, 1), 2), np.uint32)
+    got = lib().orc_bf_pairs_f64(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
+                                 _p(queries), queries.shape[0], threads, _p(out), total)
     assert got == total
     return out[:total]
 

@@ -15,6 +15,7 @@ RTS_INTERSECTS, RTS_CONTAINS, RTS_WITHIN = 0, 1, 2
 RTS_HANDLER_COUNT, RTS_HANDLER_COLLECT = 0, 1
 RTS_REBUILD_SAH, RTS_REBUILD_NEVER, RTS_REBUILD_ALWAYS = 0, 1, 2
 RTS_TRAVERSAL_AUTO, RTS_TRAVERSAL_THREAD, RTS_TRAVERSAL_WAVE = 0, 1, 2
+RTS_COORD_F32, RTS_COORD_F64, RTS_COORD_F64_AS_F32 = 0, 1, 2
 
 
 class rts_config(ctypes.Structure):
@@ -36,6 +37,7 @@ class rts_config(ctypes.Structure):
         ("sah_rebuild_ratio", ctypes.c_float),
         ("traversal", ctypes.c_uint32),
         ("sort_queries", ctypes.c_uint32),
+        ("coord", ctypes.c_uint32),
     ]
 
 
@@ -160,6 +162,10 @@ SIGNATURES = [
     ("rts_index_update", _I32, [_VP, _VP, _U64, _VP]),
     ("rts_index_query_points", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_query_envelopes", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_index_insert_d", _I32, [_VP, _VP, _U64, _VP]),
+    ("rts_index_update_d", _I32, [_VP, _VP, _U64, _VP]),
+    ("rts_index_query_points_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_index_query_envelopes_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_size", _U64, [_VP]),
     ("rts_index_build", _I32, [_VP, _VP]),
     ("rts_index_sync", _I32, [_VP, _VP]),

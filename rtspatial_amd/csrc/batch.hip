@@ -65,7 +65,7 @@ __device__ void run_lane_tile(const TreeView& t, const BatchRequest& r, uint32_t
                               const BatchArgs& a) {
   const uint32_t q = q0 + lane_id();
   if (q >= r.n) return;
-  const float4 qb = load_query(r.queries, POINTS, q);
+  const QueryF32 qb{load_query(r.queries, POINTS, q)};
   const uint64_t g = r.qbase + q;
   if (WRITE) {
     lane_query<PRED, POINTS, true>(t, qb, q + r.qid_base, a.arena, a.arena_cap, a.offsets[g]);
@@ -77,7 +77,7 @@ __device__ void run_lane_tile(const TreeView& t, const BatchRequest& r, uint32_t
 template <int PRED, bool POINTS, bool WRITE>
 __device__ void run_wave_tile(const TreeView& t, const BatchRequest& r, uint32_t q,
                               const BatchArgs& a, WaveLds& L) {
-  const float4 qb = load_query(r.queries, POINTS, q);
+  const QueryF32 qb{load_query(r.queries, POINTS, q)};
   const uint64_t g = r.qbase + q;
   if (WRITE) {
     wave_query<PRED, POINTS, true>(t, qb, q + r.qid_base, a.arena, a.arena_cap, a.offsets[g], L);

@@ -313,14 +313,16 @@ __global__ void __launch_bounds__(kThreads) k_delete(const uint64_t* __restrict_
                                                      float4* __restrict__ boxes, uint64_t n_boxes,
                                                      float4* __restrict__ leaf_boxes,
                                                      const uint32_t* __restrict__ leaf_of_id,
-                                                     uint64_t n_tree) {
+                                                     uint64_t n_tree, Box64* __restrict__ boxes64) {
   const float nan = __builtin_nanf("");
   const float4 dead = make_float4(nan, nan, nan, nan);
+  const double dnan = __builtin_nan("");
   for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * kThreads) {
     uint64_t id = ids[i];
     if (id >= n_boxes) continue;  // out-of-range ids are ignored
     boxes[id] = dead;
+    if (boxes64) boxes64[id] = Box64{dnan, dnan, dnan, dnan};
     if (id < n_tree) leaf_boxes[leaf_of_id[id]] = dead;
   }
 }
@@ -383,24 +385,31 @@ hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids,
   return hipGetLastError();
 }
 
-// Every stride-th element of `src` (8-byte points or 16-byte envelopes) into
-// dst[0, m): the autotuner's query sample.
-template <typename T>
+// Every stride-th element of `src` into dst[0, m): the autotuner's query
+// sample. Elements are 8-byte float points, 16-byte float envelopes or double
+// points, or 32-byte double envelopes (two 16-byte words).
+template <typename T, int W>
 __global__ void __launch_bounds__(kThreads) k_gather_strided(const T* __restrict__ src, uint64_t stride,
                                                              T* __restrict__ dst, uint32_t m) {
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < m; i += gridDim.x * kThreads)
-    dst[i] = src[(uint64_t)i * stride];
+#pragma unroll
+    for (int w = 0; w < W; w++) dst[(uint64_t)i * W + w] = src[(uint64_t)i * stride * W + w];
 }
 
-hipError_t launch_gather_strided(const void* src, bool points, uint64_t stride, void* dst, uint32_t m,
-                                 hipStream_t stream) {
+hipError_t launch_gather_strided(const void* src, uint32_t elem_bytes, uint64_t stride, void* dst,
+                                 uint32_t m, hipStream_t stream) {
   if (m == 0) return hipSuccess;
-  if (points)
-    hipLaunchKernelGGL(k_gather_strided<float2>, dim3(grid_for(m)), dim3(kThreads), 0, stream,
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL((k_gather_strided<float2, 1>), dim3(grid_for(m)), dim3(kThreads), 0, stream,
                        static_cast<const float2*>(src), stride, static_cast<float2*>(dst), m);
-  else
-    hipLaunchKernelGGL(k_gather_strided<float4>, dim3(grid_for(m)), dim3(kThreads), 0, stream,
+  else if (elem_bytes == 16)
+    hipLaunchKernelGGL((k_gather_strided<float4, 1>), dim3(grid_for(m)), dim3(kThreads), 0, stream,
                        static_cast<const float4*>(src), stride, static_cast<float4*>(dst), m);
+  else if (elem_bytes == 32)
+    hipLaunchKernelGGL((k_gather_strided<float4, 2>), dim3(grid_for(m)), dim3(kThreads), 0, stream,
+                       static_cast<const float4*>(src), stride, static_cast<float4*>(dst), m);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -442,10 +451,10 @@ hipError_t launch_tree_cost(const NodeRec* nodes, const float4* root_box, uint32
 
 hipError_t launch_delete(const uint64_t* ids, uint64_t n_ids, float4* boxes, uint64_t n_boxes,
                          float4* leaf_boxes, const uint32_t* leaf_of_id, uint64_t n_tree,
-                         hipStream_t stream) {
+                         Box64* boxes64, hipStream_t stream) {
   if (n_ids == 0) return hipSuccess;
   hipLaunchKernelGGL(k_delete, dim3(grid_for(n_ids)), dim3(kThreads), 0, stream, ids, n_ids, boxes,
-                     n_boxes, leaf_boxes, leaf_of_id, n_tree);
+                     n_boxes, leaf_boxes, leaf_of_id, n_tree, boxes64);
   return hipGetLastError();
 }
 

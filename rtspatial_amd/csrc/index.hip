@@ -67,19 +67,26 @@ int ensure_id_capacity(rts_index* ix, uint64_t need, hipStream_t stream) {
   cap = std::max<uint64_t>(cap, 1024);
   float4* nb = nullptr;
   uint32_t* nl = nullptr;
+  Box64* nd = nullptr;
   RTS_HIP(dev_alloc(&nb, cap));
   RTS_HIP(dev_alloc(&nl, cap));
+  if (ix->f64()) RTS_HIP(dev_alloc(&nd, cap));
   if (ix->n) {
     RTS_HIP(hipMemcpyAsync(nb, ix->boxes, ix->n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
     RTS_HIP(hipMemcpyAsync(nl, ix->leaf_of_id, ix->n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
                            stream));
+    if (nd)
+      RTS_HIP(hipMemcpyAsync(nd, ix->boxes64, ix->n * sizeof(Box64), hipMemcpyDeviceToDevice,
+                             stream));
   }
   // The old buffers may still be read by work queued on any stream.
   RTS_HIP(hipDeviceSynchronize());
   dev_free(ix->boxes);
   dev_free(ix->leaf_of_id);
+  dev_free(ix->boxes64);
   ix->boxes = nb;
   ix->leaf_of_id = nl;
+  ix->boxes64 = nd;
   ix->cap = cap;
   return RTS_OK;
 }
@@ -259,6 +266,7 @@ TreeView tree_view(const rts_index* ix) {
   t.n = (uint32_t)ix->n_tree;
   t.leaf_size = ix->cfg.leaf_size;
   t.error = ix->error;
+  t.boxes64 = ix->cfg.coord == RTS_COORD_F64 ? ix->boxes64 : nullptr;
   return t;
 }
 
@@ -292,6 +300,7 @@ void rts_config_default(rts_config* c) {
   c->sah_rebuild_ratio = 1.5f;
   c->traversal = RTS_TRAVERSAL_AUTO;
   c->sort_queries = 0;
+  c->coord = RTS_COORD_F32;
 }
 
 const char* rts_last_error(void) { return g_last_error.c_str(); }
@@ -310,6 +319,11 @@ int rts_index_create(const rts_config* cfg, rts_index** out) {
   if (cfg) ix->cfg = *cfg; else rts_config_default(&ix->cfg);
   if (ix->cfg.leaf_size == 0) ix->cfg.leaf_size = 8;
   if (ix->cfg.sah_rebuild_ratio <= 1.0f) ix->cfg.sah_rebuild_ratio = 1.5f;
+  if (ix->cfg.coord > RTS_COORD_F64_AS_F32) {
+    delete ix;
+    set_error("rts_index_create: unknown coord");
+    return RTS_ERR_INVALID;
+  }
   RTS_HIP(hipGetDevice(&ix->device));
   RTS_HIP(hipMalloc(&ix->small, 256));
   ix->scene = reinterpret_cast<float4*>(ix->small);
@@ -339,6 +353,7 @@ int rts_index_destroy(rts_index* ix) {
   if (!ix) return RTS_OK;
   (void)hipDeviceSynchronize();
   dev_free(ix->boxes);
+  dev_free(ix->boxes64);
   dev_free(ix->leaf_of_id);
   dev_free(ix->leaf_boxes);
   dev_free(ix->leaf_ids);
@@ -374,6 +389,7 @@ int rts_index_clear(rts_index* ix) {
 
 int rts_index_insert(rts_index* ix, const rts_envelope* boxes, uint64_t n, rts_stream_t stream) {
   RTS_CHECK_ARG(ix, "rts_index_insert: null index");
+  RTS_CHECK_ARG(!ix->f64(), "rts_index_insert: a double index takes rts_index_insert_d");
   if (n == 0) return RTS_OK;  // spatial_index.cuh:144-146
   RTS_CHECK_ARG(boxes, "rts_index_insert: null boxes");
   std::lock_guard<std::mutex> lk(ix->mu);
@@ -382,6 +398,26 @@ int rts_index_insert(rts_index* ix, const rts_envelope* boxes, uint64_t n, rts_s
   int rc = ensure_id_capacity(ix, ix->n + n, stream);
   if (rc) return rc;
   RTS_HIP(hipMemcpyAsync(ix->boxes + ix->n, boxes, n * sizeof(float4), hipMemcpyDefault, stream));
+  ix->n += n;
+  ix->stale = true;
+  return RTS_OK;
+}
+
+int rts_index_insert_d(rts_index* ix, const rts_envelope_d* boxes, uint64_t n,
+                       rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_insert_d: null index");
+  RTS_CHECK_ARG(ix->f64(), "rts_index_insert_d: a float index takes rts_index_insert");
+  if (n == 0) return RTS_OK;
+  RTS_CHECK_ARG(boxes, "rts_index_insert_d: null boxes");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_CHECK_ARG(ix->n + n <= kMaxBoxes, "rts_index_insert_d: more than 2^30-1 boxes");
+  RTS_HIP(hipSetDevice(ix->device));
+  int rc = ensure_id_capacity(ix, ix->n + n, stream);
+  if (rc) return rc;
+  RTS_HIP(hipMemcpyAsync(ix->boxes64 + ix->n, boxes, n * sizeof(Box64), hipMemcpyDefault, stream));
+  RTS_HIP(launch_round_boxes64(ix->boxes64 + ix->n, n, ix->boxes + ix->n,
+                               ix->cfg.coord == RTS_COORD_F64 ? kRoundOutward : kRoundNearest,
+                               stream));
   ix->n += n;
   ix->stale = true;
   return RTS_OK;
@@ -397,13 +433,14 @@ int rts_index_delete(rts_index* ix, const uint64_t* ids, uint64_t n, rts_stream_
   if (ix->n == 0) return RTS_OK;
   PhaseToken ph = phase_begin(RTS_PHASE_MUTATE, stream);
   RTS_HIP(launch_delete(ids, n, ix->boxes, ix->n, ix->leaf_boxes, ix->leaf_of_id,
-                        ix->stale ? 0 : ix->n_tree, stream));
+                        ix->stale ? 0 : ix->n_tree, ix->boxes64, stream));
   phase_end(ph, stream);
   return after_mutation(ix);
 }
 
 int rts_index_update(rts_index* ix, const rts_update* ups, uint64_t n, rts_stream_t stream) {
   RTS_CHECK_ARG(ix, "rts_index_update: null index");
+  RTS_CHECK_ARG(!ix->f64(), "rts_index_update: a double index takes rts_index_update_d");
   if (n == 0) return RTS_OK;
   RTS_CHECK_ARG(ups, "rts_index_update: null updates");
   std::lock_guard<std::mutex> lk(ix->mu);
@@ -413,6 +450,23 @@ int rts_index_update(rts_index* ix, const rts_update* ups, uint64_t n, rts_strea
   PhaseToken ph = phase_begin(RTS_PHASE_MUTATE, stream);
   RTS_HIP(launch_update(ups, n, ix->boxes, ix->n, ix->leaf_boxes, ix->leaf_of_id,
                         ix->stale ? 0 : ix->n_tree, stream));
+  phase_end(ph, stream);
+  return after_mutation(ix);
+}
+
+int rts_index_update_d(rts_index* ix, const rts_update_d* ups, uint64_t n, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_update_d: null index");
+  RTS_CHECK_ARG(ix->f64(), "rts_index_update_d: a float index takes rts_index_update");
+  if (n == 0) return RTS_OK;
+  RTS_CHECK_ARG(ups, "rts_index_update_d: null updates");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  poll_cost(ix);
+  if (ix->n == 0) return RTS_OK;
+  PhaseToken ph = phase_begin(RTS_PHASE_MUTATE, stream);
+  RTS_HIP(launch_update64(ups, n, ix->boxes64, ix->boxes, ix->n, ix->leaf_boxes, ix->leaf_of_id,
+                          ix->stale ? 0 : ix->n_tree,
+                          ix->cfg.coord == RTS_COORD_F64 ? kRoundOutward : kRoundNearest, stream));
   phase_end(ph, stream);
   return after_mutation(ix);
 }
@@ -475,8 +529,9 @@ int alloc_scratch(uint64_t nq, bool sort, bool wave, bool two_pass, hipStream_t 
 
 // Morton-orders the queries for traversal coherence; the permutation lands in
 // the scratch block.
-int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, char* keys_base,
-                  char* sort_base, hipStream_t stream, const uint32_t** order_out) {
+int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, bool f64,
+                  char* keys_base, char* sort_base, hipStream_t stream,
+                  const uint32_t** order_out) {
   const size_t words = (size_t)nq;
   uint32_t* keys = reinterpret_cast<uint32_t*>(keys_base);
   uint32_t* vals = keys + words;
@@ -485,7 +540,9 @@ int order_queries(rts_index* ix, const void* queries, uint64_t nq, bool points, 
   SortScratch ss;
   sort_scratch_carve(sort_base, nq, &ss);
   ss.error = ix->error;  // a look-back that gives up is reported by the index's error word
-  if (points) {
+  if (f64) {
+    RTS_HIP(launch_morton64(queries, points, (uint32_t)nq, ix->scene, keys, stream));
+  } else if (points) {
     RTS_HIP(launch_morton_points(static_cast<const float2*>(queries), (uint32_t)nq, ix->scene, keys,
                                  stream));
   } else {
@@ -524,7 +581,7 @@ Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
     p.traversal = (points || pred == RTS_CONTAINS) ? RTS_TRAVERSAL_THREAD : RTS_TRAVERSAL_WAVE;
   if (ix->cfg.sort_queries == 1 || ix->cfg.sort_queries == 2)
     p.sort = ix->cfg.sort_queries == 1;
-  else if (t.valid)
+  else if (t.valid && t.sort_tuned)
     p.sort = t.sort;
   else
     p.sort = nq >= 65536;
@@ -532,10 +589,13 @@ Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
   return p;
 }
 
+// Bytes of one query: float or double point or envelope.
+size_t query_bytes(bool points, bool f64) { return (points ? 8 : 16) * (f64 ? 2 : 1); }
+
 // Runs one query call under `plan`; the caller holds the index lock and has
-// brought the tree up to date.
+// brought the tree up to date. f64: double queries on an exact double index.
 int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uint64_t nq,
-             bool points, rts_handler handler, void* arg, hipStream_t stream) {
+             bool points, bool f64, rts_handler handler, void* arg, hipStream_t stream) {
   const bool sort = plan.sort;
   const int trav = plan.traversal;
   const bool two_pass = trav == RTS_TRAVERSAL_WAVE && handler == RTS_HANDLER_COLLECT;
@@ -544,8 +604,7 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
   const uint64_t kChunk = 1ull << 26;
   for (uint64_t off = 0; off < nq; off += kChunk) {
     const uint64_t m = std::min(kChunk, nq - off);
-    const void* qp = points ? static_cast<const void*>(static_cast<const float2*>(queries) + off)
-                            : static_cast<const void*>(static_cast<const float4*>(queries) + off);
+    const void* qp = static_cast<const char*>(queries) + off * query_bytes(points, f64);
     QueryScratch qs;
     size_t o_keys = 0, o_sort = 0;
     int rc = alloc_scratch(m, sort, trav == RTS_TRAVERSAL_WAVE, two_pass, stream, &qs, &o_keys,
@@ -553,7 +612,8 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
     if (rc) return rc;
     if (sort) {
       PhaseToken ph = phase_begin(RTS_PHASE_QUERY_ORDER, stream);
-      rc = order_queries(ix, qp, m, points, qs.block + o_keys, qs.block + o_sort, stream, &qs.order);
+      rc = order_queries(ix, qp, m, points, f64, qs.block + o_keys, qs.block + o_sort, stream,
+                         &qs.order);
       phase_end(ph, stream);
       if (rc) return rc;
     }
@@ -566,6 +626,7 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
     ql.order = qs.order;
     ql.pred = pred;
     ql.point_queries = points ? 1 : 0;
+    ql.f64 = f64 ? 1 : 0;
     ql.sink.mode = handler;
     ql.sink.arg = arg;
     ql.traversal = trav;
@@ -606,16 +667,49 @@ int check_query_args(int pred, const void* queries, uint64_t nq, bool points, rt
   return RTS_OK;
 }
 
+// Queries of a double index with coord RTS_COORD_F64_AS_F32, rounded to
+// nearest floats in a stream-ordered buffer that the destructor releases.
+struct RoundedQueries {
+  void* p = nullptr;
+  hipStream_t stream = nullptr;
+  ~RoundedQueries() {
+    if (p) (void)hipFreeAsync(p, stream);
+  }
+};
+
+// The queries as the kernels take them: unchanged, or, on an index that
+// rounds doubles to floats, rounded into `rq`. *f64 tells which.
+int kernel_queries(rts_index* ix, const void* queries, uint64_t nq, bool points,
+                   hipStream_t stream, RoundedQueries* rq, const void** out, bool* f64) {
+  *out = queries;
+  *f64 = ix->cfg.coord == RTS_COORD_F64;
+  if (ix->cfg.coord != RTS_COORD_F64_AS_F32) return RTS_OK;
+  rq->stream = stream;
+  RTS_HIP(hipMallocAsync(&rq->p, nq * query_bytes(points, false), stream));
+  RTS_HIP(launch_round_queries64(queries, points, nq, rq->p, stream));
+  *out = rq->p;
+  return RTS_OK;
+}
+
 int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool points,
-              rts_handler handler, void* arg, hipStream_t stream) {
+              bool double_queries, rts_handler handler, void* arg, hipStream_t stream) {
   int rc = check_query_args(pred, queries, nq, points, handler, arg);
-  if (rc || nq == 0) return rc;
+  if (rc) return rc;
+  RTS_CHECK_ARG(double_queries == ix->f64(),
+                double_queries ? "query: a float index takes rts_index_query_points/envelopes"
+                               : "query: a double index takes rts_index_query_points_d/envelopes_d");
+  if (nq == 0) return RTS_OK;
   std::lock_guard<std::mutex> lk(ix->mu);
   RTS_HIP(hipSetDevice(ix->device));
   if (ix->n == 0) return RTS_OK;  // spatial_index.cuh:354-356
   rc = ensure_built(ix, stream);
   if (rc) return rc;
-  return run_plan(ix, plan_query(ix, pred, points, nq), pred, queries, nq, points, handler, arg,
+  RoundedQueries rq;
+  const void* kq = nullptr;
+  bool f64 = false;
+  rc = kernel_queries(ix, queries, nq, points, stream, &rq, &kq, &f64);
+  if (rc) return rc;
+  return run_plan(ix, plan_query(ix, pred, points, nq), pred, kq, nq, points, f64, handler, arg,
                   stream);
 }
 
@@ -641,13 +735,25 @@ extern "C" {
 int rts_index_query_points(rts_index* ix, rts_predicate pred, const rts_point* q, uint64_t n,
                            rts_handler handler, void* arg, rts_stream_t stream) {
   RTS_CHECK_ARG(ix, "rts_index_query_points: null index");
-  return run_query(ix, pred, q, n, true, handler, arg, stream);
+  return run_query(ix, pred, q, n, true, false, handler, arg, stream);
 }
 
 int rts_index_query_envelopes(rts_index* ix, rts_predicate pred, const rts_envelope* q, uint64_t n,
                               rts_handler handler, void* arg, rts_stream_t stream) {
   RTS_CHECK_ARG(ix, "rts_index_query_envelopes: null index");
-  return run_query(ix, pred, q, n, false, handler, arg, stream);
+  return run_query(ix, pred, q, n, false, false, handler, arg, stream);
+}
+
+int rts_index_query_points_d(rts_index* ix, rts_predicate pred, const rts_point_d* q, uint64_t n,
+                             rts_handler handler, void* arg, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_query_points_d: null index");
+  return run_query(ix, pred, q, n, true, true, handler, arg, stream);
+}
+
+int rts_index_query_envelopes_d(rts_index* ix, rts_predicate pred, const rts_envelope_d* q,
+                                uint64_t n, rts_handler handler, void* arg, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_query_envelopes_d: null index");
+  return run_query(ix, pred, q, n, false, true, handler, arg, stream);
 }
 
 uint64_t rts_index_size(const rts_index* ix) { return ix ? ix->n : 0; }
@@ -740,6 +846,7 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   RTS_CHECK_ARG((cfg.leaf_sizes & 0x7Eu) != 0,
                 "rts_index_autotune: leaf_sizes needs a bit in 1..6 (leaf sizes 2..64)");
   const bool pts = points != 0;
+  const bool f64 = ix->cfg.coord == RTS_COORD_F64;  // kernels take doubles
   int rc = check_query_args(pred, queries, n, pts, handler, ix);
   if (rc || n == 0) return rc;
   std::lock_guard<std::mutex> lk(ix->mu);
@@ -749,7 +856,9 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   if (rc) return rc;
 
   // An evenly strided sample, shrunk until its results fit a 2^30-pair queue.
-  const size_t qbytes = pts ? sizeof(float2) : sizeof(float4);
+  // A double index's sample is taken in doubles; one that rounds doubles to
+  // floats times the rounded sample.
+  const size_t qbytes = query_bytes(pts, ix->f64());
   const uint64_t kMaxPairs = 1ull << 30;
   TuneBuffers tb;
   RTS_HIP(hipMalloc(&tb.small, 64));
@@ -763,9 +872,18 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     if (tb.sample) RTS_HIP(hipFree(tb.sample));
     tb.sample = nullptr;
     RTS_HIP(hipMalloc(&tb.sample, m * qbytes));
-    RTS_HIP(launch_gather_strided(queries, pts, n / m, tb.sample, (uint32_t)m, stream));
+    RTS_HIP(launch_gather_strided(queries, (uint32_t)qbytes, n / m, tb.sample, (uint32_t)m, stream));
+    if (ix->cfg.coord == RTS_COORD_F64_AS_F32) {
+      void* rounded = nullptr;
+      RTS_HIP(hipMalloc(&rounded, m * query_bytes(pts, false)));
+      const hipError_t e = launch_round_queries64(tb.sample, pts, m, rounded, stream);
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(tb.sample);
+      tb.sample = rounded;
+      RTS_HIP(e);
+    }
     RTS_HIP(hipMemsetAsync(counter, 0, sizeof(*counter), stream));
-    rc = run_plan(ix, base, pred, tb.sample, m, pts, RTS_HANDLER_COUNT, counter, stream);
+    rc = run_plan(ix, base, pred, tb.sample, m, pts, f64, RTS_HANDLER_COUNT, counter, stream);
     if (rc) return rc;
     RTS_HIP(hipMemcpyAsync(&total, counter, sizeof(total), hipMemcpyDeviceToHost, stream));
     RTS_HIP(hipStreamSynchronize(stream));
@@ -791,7 +909,7 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     for (uint32_t r = 0; r <= cfg.reps; ++r) {
       RTS_HIP(hipMemsetAsync(counter, 0, sizeof(*counter), stream));
       RTS_HIP(hipEventRecord(tb.ev[0], stream));
-      int e = run_plan(ix, p, pred, tb.sample, m, pts, handler, arg, stream);
+      int e = run_plan(ix, p, pred, tb.sample, m, pts, f64, handler, arg, stream);
       if (e) return e;
       RTS_HIP(hipEventRecord(tb.ev[1], stream));
       RTS_HIP(hipEventSynchronize(tb.ev[1]));
@@ -815,6 +933,10 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   // Traversal modes in the outer loop: a mode whose first plan is already 4x
   // slower than the best is not tried with the other leaf sizes or orders
   // (per-lane traversal of queries with thousands of results each, say).
+  // Query ordering is only tuned when the sample is the whole query set: a
+  // strided sample has none of the set's locality, which is what Morton
+  // ordering exploits, so otherwise the order stays with the call's size rule.
+  const bool whole = m == n;
   const int other =
       base.traversal == RTS_TRAVERSAL_WAVE ? (int)RTS_TRAVERSAL_THREAD : (int)RTS_TRAVERSAL_WAVE;
   for (int trav : {base.traversal, other}) {
@@ -822,6 +944,7 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     for (uint32_t k = 1; k <= 6; ++k) {
       if (!(cfg.leaf_sizes & (1u << k))) continue;
       for (bool sort : {false, true}) {
+        if (!whole && sort != base.sort) continue;
         Plan p{trav, sort, 1u << k};
         if (p.traversal == base.traversal && p.sort == base.sort && p.leaf_size == base.leaf_size)
           continue;
@@ -846,11 +969,12 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     t.valid = true;
     t.traversal = (uint32_t)winner.traversal;
     t.sort = winner.sort;
+    t.sort_tuned = whole;
     t.leaf_size = winner.leaf_size;
   }
   out->leaf_size = winner.leaf_size;
   out->traversal = (uint32_t)winner.traversal;
-  out->sort_queries = winner.sort ? 1 : 2;
+  out->sort_queries = whole ? (winner.sort ? 1 : 2) : 0;
   out->plans = plans;
   out->sample = m;
   out->sample_results = total;

@@ -17,7 +17,8 @@ struct rts_index {
   // sorted position of each id in the current tree.
   uint64_t n = 0;    // ids handed out
   uint64_t cap = 0;  // capacity of the by-id arrays
-  float4* boxes = nullptr;
+  float4* boxes = nullptr;      // a double index: the boxes rounded to floats
+  rts::Box64* boxes64 = nullptr;  // a double index: the boxes as given
   uint32_t* leaf_of_id = nullptr;
 
   // Tree (valid for n_tree leaves unless stale).
@@ -67,9 +68,12 @@ struct rts_index {
     bool valid = false;
     uint32_t traversal = 0;
     bool sort = false;
+    bool sort_tuned = false;  // the sample was the whole query set
     uint32_t leaf_size = 0;
   };
   TunedPlan tuned[3][2];
+
+  bool f64() const { return cfg.coord != RTS_COORD_F32; }
 };
 
 namespace rts {

@@ -1,5 +1,5 @@
-// LBVH traversal kernels: point and envelope queries with exact FP32
-// predicates, replacing the reference's OptiX raygen/intersection programs
+// LBVH traversal kernels: point and envelope queries with exact FP32 (or,
+// on a double index, exact FP64) predicates, replacing the reference's OptiX raygen/intersection programs
 // (src/shaders/shaders_contains_point_query_2d.cu:14-49,
 //  shaders_contains_envelope_query_2d.cu:14-51,
 //  shaders_intersects_envelope_query_2d.cu:14-128).
@@ -42,9 +42,8 @@ __device__ __forceinline__ void flush_stage(const rts_pair* stage, uint32_t fill
 
 // One query per lane. The wave iterates until all of its lanes are done; in
 // each iteration a lane either expands one node or tests one leaf box.
-template <int PRED, bool POINTS, bool COLLECT>
+template <int PRED, bool POINTS, bool COLLECT, bool F64>
 __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) {
-  using P = Pred<PRED, POINTS>;
   __shared__ rts_pair s_stage[COLLECT ? kWaves : 1][COLLECT ? kStageCap : 1];
 
   unsigned long long* counter;
@@ -66,24 +65,17 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
   const uint32_t qi = working ? (q.order ? q.order[i] : i) : 0;
   const uint32_t qid = qi + q.qid_base;
 
-  float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (working) {
-    if (POINTS) {
-      const float2 p = static_cast<const float2*>(q.queries)[qi];
-      qb = make_float4(p.x, p.y, p.x, p.y);
-    } else {
-      qb = static_cast<const float4*>(q.queries)[qi];
-    }
-  }
+  typename QueryOf<F64>::type qb{};
+  if (working) load_q(q.queries, POINTS, qi, qb);
 
   uint32_t stack[kStackDepth];
   int sp = 0;
   uint32_t cur = 1, end = 0;  // leaf range being scanned: [cur, end]
   if (working) {
     const float4 rb = t.root_box[0];
-    if (!P::node(rb, qb)) {
+    if (!node_test<PRED, POINTS>(rb, qb)) {
       working = false;
-    } else if (t.n <= t.leaf_size || t.n == 1 || P::inside(rb, qb)) {
+    } else if (t.n <= t.leaf_size || t.n == 1 || inside_test<PRED, POINTS>(rb, qb)) {
       cur = 0;
       end = t.n - 1;
     } else {
@@ -131,29 +123,31 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
           const float4 rbx = nd->box[1];
           const uint32_t gamma = m.x & ~kLeafFlag;
           // right child first so that the left one is expanded next
-          if (P::node(rbx, qb)) {
+          if (node_test<PRED, POINTS>(rbx, qb)) {
             if (m.y & kLeafFlag) {
               if (sp < kStackDepth) stack[sp++] = m.y;
             } else {
               const uint32_t size = m.w - gamma;
               if (sp < kStackDepth)
-                stack[sp++] = (size <= t.leaf_size || P::inside(rbx, qb)) ? (m.y | kRangeFlag) : m.y;
+                stack[sp++] = (size <= t.leaf_size || inside_test<PRED, POINTS>(rbx, qb))
+                                  ? (m.y | kRangeFlag) : m.y;
             }
           }
-          if (P::node(lb, qb)) {
+          if (node_test<PRED, POINTS>(lb, qb)) {
             if (m.x & kLeafFlag) {
               if (sp < kStackDepth) stack[sp++] = m.x;
             } else {
               const uint32_t size = gamma - m.z + 1;
               if (sp < kStackDepth)
-                stack[sp++] = (size <= t.leaf_size || P::inside(lb, qb)) ? (m.x | kRangeFlag) : m.x;
+                stack[sp++] = (size <= t.leaf_size || inside_test<PRED, POINTS>(lb, qb))
+                                  ? (m.x | kRangeFlag) : m.x;
             }
           }
         }
       } else {
         working = false;
       }
-      if (have && prim < t.n) hit = P::leaf(t.leaf_boxes[prim], qb);
+      if (have && prim < t.n) hit = leaf_test<PRED, POINTS>(t, prim, t.leaf_boxes[prim], qb);
     }
 
     if (COLLECT) {
@@ -231,7 +225,7 @@ __device__ __forceinline__ uint32_t grab_query(uint32_t* tickets, uint32_t n, ui
 }
 
 // One wavefront per workgroup, so that a wave's LDS is released by itself.
-template <int PRED, bool POINTS, int MODE>
+template <int PRED, bool POINTS, int MODE, bool F64>
 __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const WaveArgs a) {
   __shared__ WaveLds s_lds;
   const TreeView t = q.tree;
@@ -243,7 +237,8 @@ __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const
     const uint32_t lin = grab_query(a.tickets, q.n, slice);
     if (lin >= q.n) return;
     const uint32_t qi = q.order ? q.order[lin] : lin;
-    const float4 qb = load_query(q.queries, POINTS, qi);
+    typename QueryOf<F64>::type qb;
+    load_q(q.queries, POINTS, qi, qb);
     if (MODE == kWaveWrite) {
       const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
       wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base, dq->data, dq->capacity,
@@ -275,38 +270,50 @@ uint32_t wave_grid_blocks() {
   return blocks;
 }
 
-template <int PRED, bool POINTS, int MODE>
+template <int PRED, bool POINTS, int MODE, bool F64>
 hipError_t launch_wave_mode(const QueryLaunch& q, const WaveArgs& a, hipStream_t stream) {
   hipError_t e = launch_fill_u32(a.tickets, 0u, kXcds * kTicketStride, stream);
   if (e != hipSuccess) return e;
   const uint32_t resident = wave_grid_blocks();
   uint32_t blocks = resident < q.n ? resident : q.n;
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;  // whole rounds over the XCDs
-  hipLaunchKernelGGL((k_query_wave<PRED, POINTS, MODE>), dim3(blocks), dim3(kWave), 0, stream, q,
-                     a);
+  hipLaunchKernelGGL((k_query_wave<PRED, POINTS, MODE, F64>), dim3(blocks), dim3(kWave), 0, stream,
+                     q, a);
   return hipGetLastError();
 }
 
-template <int PRED, bool POINTS>
-hipError_t launch_wave(const QueryLaunch& q, const WaveArgs& a, int phase, hipStream_t stream) {
+template <int PRED, bool POINTS, bool F64>
+hipError_t launch_wave_f(const QueryLaunch& q, const WaveArgs& a, int phase, hipStream_t stream) {
   switch (phase) {
     case kWaveCount:
-      return launch_wave_mode<PRED, POINTS, kWaveCount>(q, a, stream);
+      return launch_wave_mode<PRED, POINTS, kWaveCount, F64>(q, a, stream);
     case kWaveCountSink:
-      return launch_wave_mode<PRED, POINTS, kWaveCountSink>(q, a, stream);
+      return launch_wave_mode<PRED, POINTS, kWaveCountSink, F64>(q, a, stream);
     default:
-      return launch_wave_mode<PRED, POINTS, kWaveWrite>(q, a, stream);
+      return launch_wave_mode<PRED, POINTS, kWaveWrite, F64>(q, a, stream);
   }
 }
 
 template <int PRED, bool POINTS>
-hipError_t launch_pred(const QueryLaunch& q, hipStream_t stream) {
+hipError_t launch_wave(const QueryLaunch& q, const WaveArgs& a, int phase, hipStream_t stream) {
+  return q.f64 ? launch_wave_f<PRED, POINTS, true>(q, a, phase, stream)
+               : launch_wave_f<PRED, POINTS, false>(q, a, phase, stream);
+}
+
+template <int PRED, bool POINTS, bool F64>
+hipError_t launch_pred_f(const QueryLaunch& q, hipStream_t stream) {
   const dim3 grid((q.n + kThreads - 1) / kThreads), block(kThreads);
   if (q.sink.mode == RTS_HANDLER_COLLECT)
-    hipLaunchKernelGGL((k_query_thread<PRED, POINTS, true>), grid, block, 0, stream, q);
+    hipLaunchKernelGGL((k_query_thread<PRED, POINTS, true, F64>), grid, block, 0, stream, q);
   else
-    hipLaunchKernelGGL((k_query_thread<PRED, POINTS, false>), grid, block, 0, stream, q);
+    hipLaunchKernelGGL((k_query_thread<PRED, POINTS, false, F64>), grid, block, 0, stream, q);
   return hipGetLastError();
+}
+
+template <int PRED, bool POINTS>
+hipError_t launch_pred(const QueryLaunch& q, hipStream_t stream) {
+  return q.f64 ? launch_pred_f<PRED, POINTS, true>(q, stream)
+               : launch_pred_f<PRED, POINTS, false>(q, stream);
 }
 
 }  // namespace

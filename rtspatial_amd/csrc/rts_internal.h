@@ -36,6 +36,12 @@ struct __align__(64) NodeRec {
 };
 static_assert(sizeof(NodeRec) == 64, "node record is one 64-byte line");
 
+// A box with double coordinates (rts_envelope_d): x0, y0, x1, y1.
+struct Box64 {
+  double x0, y0, x1, y1;
+};
+static_assert(sizeof(Box64) == sizeof(rts_envelope_d), "Box64 is rts_envelope_d");
+
 struct TreeView {
   const float4* leaf_boxes;
   const uint32_t* leaf_ids;
@@ -45,6 +51,7 @@ struct TreeView {
   uint32_t n;                 // leaves
   uint32_t leaf_size;         // ranges with <= leaf_size leaves are scanned directly
   uint32_t* error;            // device word: bit 1 refit walk, bit 2 traversal bound
+  const Box64* boxes64;       // double index: the boxes by id (else null)
 };
 
 // Device error bits (the index's error word).
@@ -105,8 +112,9 @@ hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* sce
                                 uint32_t* keys, hipStream_t stream);
 hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids, uint32_t n,
                                 float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream);
-hipError_t launch_gather_strided(const void* src, bool points, uint64_t stride, void* dst, uint32_t m,
-                                 hipStream_t stream);
+// Every stride-th element of src (elem_bytes 8, 16 or 32) into dst[0, m).
+hipError_t launch_gather_strided(const void* src, uint32_t elem_bytes, uint64_t stride, void* dst,
+                                 uint32_t m, hipStream_t stream);
 hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes,
                          uint32_t* parent, uint32_t* flags, hipStream_t stream);
 struct RefitArgs {
@@ -132,11 +140,27 @@ hipError_t launch_tree_cost(const NodeRec* nodes, const float4* root_box, uint32
                             double* out, hipStream_t stream);
 hipError_t launch_delete(const uint64_t* ids, uint64_t n_ids, float4* boxes, uint64_t n_boxes,
                          float4* leaf_boxes, const uint32_t* leaf_of_id, uint64_t n_tree,
-                         hipStream_t stream);
+                         Box64* boxes64, hipStream_t stream);
 hipError_t launch_update(const rts_update* ups, uint64_t n_ups, float4* boxes, uint64_t n_boxes,
                          float4* leaf_boxes, const uint32_t* leaf_of_id, uint64_t n_tree,
                          hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t stream);
+
+// coord64.hip: double coordinates.
+enum { kRoundOutward = 0, kRoundNearest = 1 };
+// dst[i] = src[i] rounded to float boxes: outward (min down, max up) or to nearest.
+hipError_t launch_round_boxes64(const Box64* src, uint64_t n, float4* dst, int rounding,
+                                hipStream_t stream);
+// Queries rounded to nearest floats (double2 -> float2 or Box64 -> float4).
+hipError_t launch_round_queries64(const void* src, bool points, uint64_t n, void* dst,
+                                  hipStream_t stream);
+// boxes64[id] = box, and the rounded float box by id and in sorted position.
+hipError_t launch_update64(const rts_update_d* ups, uint64_t n_ups, Box64* boxes64, float4* boxes,
+                           uint64_t n_boxes, float4* leaf_boxes, const uint32_t* leaf_of_id,
+                           uint64_t n_tree, int rounding, hipStream_t stream);
+// Morton keys of double points / box centres (rounded to floats first).
+hipError_t launch_morton64(const void* queries, bool points, uint32_t n, const float4* scene,
+                           uint32_t* keys, hipStream_t stream);
 
 // query.hip
 struct QueryLaunch {
@@ -146,7 +170,8 @@ struct QueryLaunch {
   uint32_t qid_base;     // added to the query index when reporting
   const uint32_t* order; // optional permutation: lane i works on queries[order[i]]
   int pred;              // rts_predicate
-  int point_queries;     // 1 = float2 points
+  int point_queries;     // 1 = points
+  int f64;               // 1 = double queries (double2 / Box64), exact double predicates
   SinkArgs sink;
   int traversal;         // rts_traversal (resolved: THREAD or WAVE)
 };

@@ -771,6 +771,10 @@ int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request*
     set_error("rts_serve_run: null argument");
     return RTS_ERR_INVALID;
   }
+  if (ix->cfg.coord != RTS_COORD_F32) {
+    set_error("rts_serve_run: the served path takes float indexes (RTS_COORD_F32)");
+    return RTS_ERR_INVALID;
+  }
   Engine E;
   E.ix = ix;
   if (cfg) E.cfg = *cfg; else rts_serve_config_default(&E.cfg);

@@ -7,12 +7,34 @@
 // subtree lying inside the query whose leaves are all live is emitted (or
 // counted) without testing: refit bounds cover every live box below a node,
 // so each of them matches.
+//
+// Double indexes (QueryF64) run the same walks on the outward-rounded float
+// bounds. Node and "inside" tests compare float bounds with the query rounded
+// in the direction that makes each comparison exact (round.h), so they are the
+// exact tests of the float bounds against the double query. A leaf is decided
+// in FP32 when its float box settles it (necessary test fails, or the box
+// shrunk by one ulp -- which lies inside the double box -- passes the
+// sufficient test); otherwise, when a coordinate lies within one float ulp of
+// the query's, the double box is read and tested exactly.
 #pragma once
 
+#include "round.h"
 #include "rts_internal.h"
 #include "wave.h"
 
 namespace rts {
+
+// A float query: (x0, y0, x1, y1); a point is (x, y, x, y).
+struct QueryF32 {
+  float4 b;
+};
+// A double query: its bounds rounded inward and outward, and its coordinates
+// (x, y or x0, y0, x1, y1) for the exact test.
+struct QueryF64 {
+  float4 in;   // (up x0, up y0, down x1, down y1)
+  float4 out;  // (down x0, down y0, up x1, up y1)
+  const double* q;
+};
 
 template <int PRED, bool POINTS>
 struct Pred;
@@ -46,6 +68,93 @@ struct Pred<RTS_WITHIN, false> {
   __device__ static bool leaf(const float4 b, const float4 q) { return contains_box(q, b); }
 };
 
+// Double queries (envelope.cuh:29-69 with COORD_T = double). `in` decides
+// "float bound <= q.x1" and "float bound >= q.x0" exactly; `out` decides
+// "float bound <= q.x0" and "float bound >= q.x1" exactly.
+__device__ __forceinline__ float4 shrink_ulp(const float4 b) {
+  return make_float4(f32_up(b.x), f32_up(b.y), f32_down(b.z), f32_down(b.w));
+}
+template <int PRED, bool POINTS>
+struct Pred64;
+template <int P>
+struct Pred64<P, true> {
+  __device__ static bool node(const float4 n, const QueryF64& q) { return intersects(n, q.in); }
+  __device__ static bool inside(const float4, const QueryF64&) { return false; }
+  __device__ static bool nec(const float4 b, const QueryF64& q) { return intersects(b, q.in); }
+  __device__ static bool suff(const float4 b, const QueryF64& q) {
+    return intersects(shrink_ulp(b), q.in);
+  }
+  __device__ static bool exact(const Box64& d, const double* q) {
+    return q[0] >= d.x0 && q[0] <= d.x1 && q[1] >= d.y0 && q[1] <= d.y1;
+  }
+};
+template <>
+struct Pred64<RTS_INTERSECTS, false> {
+  __device__ static bool node(const float4 n, const QueryF64& q) { return intersects(n, q.in); }
+  __device__ static bool inside(const float4 n, const QueryF64& q) { return contains_box(q.in, n); }
+  __device__ static bool nec(const float4 b, const QueryF64& q) { return intersects(b, q.in); }
+  __device__ static bool suff(const float4 b, const QueryF64& q) {
+    return intersects(shrink_ulp(b), q.in);
+  }
+  __device__ static bool exact(const Box64& d, const double* q) {
+    return q[0] <= d.x1 && q[2] >= d.x0 && q[1] <= d.y1 && q[3] >= d.y0;
+  }
+};
+template <>
+struct Pred64<RTS_CONTAINS, false> {
+  __device__ static bool node(const float4 n, const QueryF64& q) { return contains_box(n, q.out); }
+  __device__ static bool inside(const float4, const QueryF64&) { return false; }
+  __device__ static bool nec(const float4 b, const QueryF64& q) { return contains_box(b, q.out); }
+  __device__ static bool suff(const float4 b, const QueryF64& q) {
+    return contains_box(shrink_ulp(b), q.out);
+  }
+  __device__ static bool exact(const Box64& d, const double* q) {
+    return q[0] >= d.x0 && q[2] <= d.x1 && q[1] >= d.y0 && q[3] <= d.y1;
+  }
+};
+template <>
+struct Pred64<RTS_WITHIN, false> {
+  __device__ static bool node(const float4 n, const QueryF64& q) { return intersects(n, q.in); }
+  __device__ static bool inside(const float4 n, const QueryF64& q) { return contains_box(q.in, n); }
+  __device__ static bool nec(const float4 b, const QueryF64& q) { return contains_box(q.out, b); }
+  __device__ static bool suff(const float4 b, const QueryF64& q) { return contains_box(q.in, b); }
+  __device__ static bool exact(const Box64& d, const double* q) {
+    return d.x0 >= q[0] && d.x1 <= q[2] && d.y0 >= q[1] && d.y1 <= q[3];
+  }
+};
+
+// The tests the walks call, by query type.
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool node_test(const float4 n, const QueryF32& q) {
+  return Pred<PRED, POINTS>::node(n, q.b);
+}
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool node_test(const float4 n, const QueryF64& q) {
+  return Pred64<PRED, POINTS>::node(n, q);
+}
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool inside_test(const float4 n, const QueryF32& q) {
+  return Pred<PRED, POINTS>::inside(n, q.b);
+}
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool inside_test(const float4 n, const QueryF64& q) {
+  return Pred64<PRED, POINTS>::inside(n, q);
+}
+// Leaf at sorted position `leaf` with float box b.
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool leaf_test(const TreeView&, uint32_t, const float4 b,
+                                          const QueryF32& q) {
+  return Pred<PRED, POINTS>::leaf(b, q.b);
+}
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, const float4 b,
+                                          const QueryF64& q) {
+  using P = Pred64<PRED, POINTS>;
+  if (!P::nec(b, q)) return false;
+  if (P::suff(b, q)) return true;
+  return P::exact(t.boxes64[t.leaf_ids[leaf]], q.q);
+}
+
 __device__ __forceinline__ void lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -57,6 +166,27 @@ __device__ __forceinline__ float4 load_query(const void* queries, bool points, u
     return make_float4(p.x, p.y, p.x, p.y);
   }
   return static_cast<const float4*>(queries)[i];
+}
+
+// Query i of a float (double2 / Box64 when F64) array.
+template <bool F64>
+struct QueryOf {
+  using type = QueryF32;
+};
+template <>
+struct QueryOf<true> {
+  using type = QueryF64;
+};
+__device__ __forceinline__ void load_q(const void* queries, bool points, uint32_t i, QueryF32& q) {
+  q.b = load_query(queries, points, i);
+}
+__device__ __forceinline__ void load_q(const void* queries, bool points, uint32_t i, QueryF64& q) {
+  const double* p = static_cast<const double*>(queries) + (points ? 2ull : 4ull) * i;
+  const double x0 = p[0], y0 = p[1];
+  const double x1 = points ? x0 : p[2], y1 = points ? y0 : p[3];
+  q.in = make_float4(f32_ru(x0), f32_ru(y0), f32_rd(x1), f32_rd(y1));
+  q.out = make_float4(f32_rd(x0), f32_rd(y0), f32_ru(x1), f32_ru(y1));
+  q.q = p;
 }
 
 // ---------------------------------------------------------------- wave ----
@@ -93,11 +223,10 @@ struct WaveLds {
 // Returns the query's number of results (wave-uniform). With WRITE the pairs
 // (geom_id, qid) go to out[obase ...], slots >= cap dropped; without WRITE,
 // subtrees inside the query are counted from their live-leaf counts.
-template <int PRED, bool POINTS, bool WRITE>
-__device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_t qid,
+template <int PRED, bool POINTS, bool WRITE, typename Q>
+__device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qid,
                                rts_pair* out, const uint64_t cap, const uint64_t obase,
                                WaveLds& L) {
-  using P = Pred<PRED, POINTS>;
   constexpr bool kCounting = !WRITE;
   const uint32_t lane = lane_id();
   uint32_t* stack = L.stack;
@@ -108,8 +237,8 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
   uint64_t inside_live = 0;           // per lane: counting mode, subtrees inside the query
 
   const float4 rb = t.root_box[0];
-  if (P::node(rb, qb)) {
-    const bool inside = P::inside(rb, qb);
+  if (node_test<PRED, POINTS>(rb, qb)) {
+    const bool inside = inside_test<PRED, POINTS>(rb, qb);
     if (kCounting && inside) {
       if (lane == 0) inside_live = t.root_live[0];
     } else if (t.n <= t.leaf_size || t.n == 1 || inside) {
@@ -157,8 +286,8 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
         const uint32_t cnt[2] = {gamma - mt.z + 1, mt.w - gamma};
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-          if (!P::node(cb[side], qb)) continue;
-          const bool inside = P::inside(cb[side], qb);
+          if (!node_test<PRED, POINTS>(cb[side], qb)) continue;
+          const bool inside = inside_test<PRED, POINTS>(cb[side], qb);
           if (kCounting && inside) {
             inside_live += live[side];
           } else if ((ref[side] & kLeafFlag) || cnt[side] <= t.leaf_size || degrade || inside) {
@@ -270,7 +399,8 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
 #pragma unroll
         for (int k = 0; k < kPerLane; k++) {
           if (((valid & ~allm) >> k) & 1u)
-            hitmask |= (P::leaf(t.leaf_boxes[leaf[k]], qb) ? 1u : 0u) << k;
+            hitmask |= (leaf_test<PRED, POINTS>(t, leaf[k], t.leaf_boxes[leaf[k]], qb) ? 1u : 0u)
+                       << k;
         }
         // (4) slots: position order is k-major, so the pairs of round k go
         // right after those of rounds < k, in lane order.
@@ -306,18 +436,17 @@ __device__ uint64_t wave_query(const TreeView& t, const float4 qb, const uint32_
 // (points, data-contains-query). Counting mode counts subtrees inside the
 // query from their live-leaf counts; writing mode writes the query's pairs to
 // out[obase ...] in traversal order. Returns the query's number of results.
-template <int PRED, bool POINTS, bool WRITE>
-__device__ uint64_t lane_query(const TreeView& t, const float4 qb, const uint32_t qid,
+template <int PRED, bool POINTS, bool WRITE, typename Q>
+__device__ uint64_t lane_query(const TreeView& t, const Q& qb, const uint32_t qid,
                                rts_pair* out, const uint64_t cap, const uint64_t obase) {
-  using P = Pred<PRED, POINTS>;
   uint32_t stack[kStackDepth];
   int sp = 0;
   uint32_t cur = 1, end = 0;  // leaf range being scanned: [cur, end]
   bool all = false;           // the range needs no test
   uint64_t found = 0;
   const float4 rb = t.root_box[0];
-  if (!P::node(rb, qb)) return 0;
-  const bool root_inside = P::inside(rb, qb);
+  if (!node_test<PRED, POINTS>(rb, qb)) return 0;
+  const bool root_inside = inside_test<PRED, POINTS>(rb, qb);
   if (root_inside && !WRITE) return t.root_live[0];
   if (t.n <= t.leaf_size || t.n == 1 || root_inside) {
     cur = 0;
@@ -334,7 +463,7 @@ __device__ uint64_t lane_query(const TreeView& t, const float4 qb, const uint32_
     }
     if (cur <= end) {
       const uint32_t prim = cur++;
-      if (prim < t.n && (all || P::leaf(t.leaf_boxes[prim], qb))) {
+      if (prim < t.n && (all || leaf_test<PRED, POINTS>(t, prim, t.leaf_boxes[prim], qb))) {
         if (WRITE && obase + found < cap) {
           rts_pair pr;
           pr.geom_id = t.leaf_ids[prim];
@@ -375,8 +504,8 @@ __device__ uint64_t lane_query(const TreeView& t, const float4 qb, const uint32_
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const int side = 1 - k;
-      if (!P::node(cb[side], qb)) continue;
-      const bool inside = P::inside(cb[side], qb);
+      if (!node_test<PRED, POINTS>(cb[side], qb)) continue;
+      const bool inside = inside_test<PRED, POINTS>(cb[side], qb);
       if (!WRITE && inside) {
         found += live[side];
       } else if (ref[side] & kLeafFlag) {

@@ -1,4 +1,4 @@
-"""Python mirror of rtspatial::SpatialIndex<float, 2> over the C ABI.
+"""Python mirror of rtspatial::SpatialIndex<float, 2> and <double, 2> over the C ABI.
 
 Reference: include/rtspatial/spatial_index.cuh:20-931 (Config, SpatialIndex),
 utils/queue.h:60-113 (Queue), utils/shared_value.h:11-58 (SharedValue),
@@ -11,6 +11,13 @@ as the reference's ArrayView of a pinned_vector).
   ids       int64   [n]           size_t ids (Delete)
   updates   (ids int64 [n], envelopes float32 [n, 4])   thrust::pair<size_t, envelope_t>
   results   int32   [k, 2]        (geom_id, query_id), collecting_handlers.h:9-31
+
+SpatialIndex(torch.float64) is SpatialIndex<double, 2>: points, envelopes and
+updates carry float64 coordinates (updates: int64 [n, 5], see pack_updates).
+Its predicates are exact in double precision; Config.coord =
+RTS_COORD_F64_AS_F32 instead rounds coordinates to the nearest float, which
+reproduces the reference's results (its Point<double,2>::get_coordinate
+returns float, geom/point.cuh:68).
 """
 import ctypes
 import dataclasses
@@ -51,6 +58,9 @@ class Config:
     sah_rebuild_ratio: float = 1.5
     traversal: int = N.RTS_TRAVERSAL_AUTO
     sort_queries: int = 0
+    # Double indexes: RTS_COORD_F64 (exact, the default) or RTS_COORD_F64_AS_F32.
+    # The index's dtype decides between float and double.
+    coord: int = N.RTS_COORD_F64
 
     def to_c(self):
         c = N.rts_config()
@@ -169,16 +179,25 @@ class _null:
 
 
 class SpatialIndex:
-    """rtspatial::SpatialIndex<float, 2> (spatial_index.cuh:62-931)."""
+    """rtspatial::SpatialIndex<COORD_T, 2> (spatial_index.cuh:62-931);
+    COORD_T is torch.float32 (default) or torch.float64."""
 
-    def __init__(self):
+    def __init__(self, coord_t=torch.float32):
+        if coord_t not in (torch.float32, torch.float64):
+            raise TypeError("SpatialIndex: coord_t must be torch.float32 or torch.float64")
         self._h = None
+        self.coord_t = coord_t
+        self._d = coord_t == torch.float64
 
     # -- lifecycle ---------------------------------------------------------
     def Init(self, config=None):
         """spatial_index.cuh:83-110."""
         self._destroy()
         cfg = (config or Config()).to_c()
+        if not self._d:
+            cfg.coord = N.RTS_COORD_F32
+        elif cfg.coord not in (N.RTS_COORD_F64, N.RTS_COORD_F64_AS_F32):
+            raise ValueError("Init: a double index takes coord RTS_COORD_F64 or RTS_COORD_F64_AS_F32")
         h = ctypes.c_void_p()
         N.check(N.lib().rts_index_create(ctypes.byref(cfg), ctypes.byref(h)), "Init")
         self._h = h
@@ -207,9 +226,10 @@ class SpatialIndex:
     # -- mutation ------------------------------------------------------------
     def Insert(self, envelopes, stream=None):
         """spatial_index.cuh:142-196: appends boxes with ids size()..size()+n-1."""
-        _check_array(envelopes, 4, torch.float32, "Insert")
-        N.check(N.lib().rts_index_insert(self._handle(), envelopes.data_ptr(), envelopes.shape[0],
-                                         _stream_handle(stream)), "Insert")
+        _check_array(envelopes, 4, self.coord_t, "Insert")
+        fn = N.lib().rts_index_insert_d if self._d else N.lib().rts_index_insert
+        N.check(fn(self._handle(), envelopes.data_ptr(), envelopes.shape[0],
+                   _stream_handle(stream)), "Insert")
 
     def Delete(self, ids, stream=None):
         """spatial_index.cuh:208-265. ids >= size() are ignored."""
@@ -219,19 +239,21 @@ class SpatialIndex:
 
     def Update(self, updates, stream=None):
         """spatial_index.cuh:280-336. `updates` is a packed int64 [n, 3] tensor
-        from pack_updates(), or an (ids, envelopes) tuple."""
+        ([n, 5] for a double index) from pack_updates(), or an (ids,
+        envelopes) tuple."""
         if isinstance(updates, tuple):
-            updates = pack_updates(*updates)
-        _check_array(updates, 3, torch.int64, "Update")
-        N.check(N.lib().rts_index_update(self._handle(), updates.data_ptr(), updates.shape[0],
-                                         _stream_handle(stream)), "Update")
+            updates = pack_updates(*updates, coord_t=self.coord_t)
+        _check_array(updates, 5 if self._d else 3, torch.int64, "Update")
+        fn = N.lib().rts_index_update_d if self._d else N.lib().rts_index_update
+        N.check(fn(self._handle(), updates.data_ptr(), updates.shape[0], _stream_handle(stream)),
+                "Update")
 
     # -- queries -------------------------------------------------------------
     def Query(self, predicate, queries, arg, stream=None):
         """spatial_index.cuh:352-391 (points, [n, 2]) and :407-515 (envelopes,
         [n, 4]). `arg` is a SharedValue (counting handler) or a Queue
         (collecting handler)."""
-        _check_array(queries, None, torch.float32, "Query")
+        _check_array(queries, None, self.coord_t, "Query")
         if queries.dim() != 2 or queries.shape[1] not in (2, 4):
             raise ValueError("Query: queries must be [n, 2] points or [n, 4] envelopes")
         if isinstance(arg, SharedValue):
@@ -240,8 +262,11 @@ class SpatialIndex:
             handler, ptr = N.RTS_HANDLER_COLLECT, arg.DeviceObject()
         else:
             raise TypeError("Query: arg must be a SharedValue or a Queue")
-        fn = (N.lib().rts_index_query_points if queries.shape[1] == 2
-              else N.lib().rts_index_query_envelopes)
+        L = N.lib()
+        if self._d:
+            fn = L.rts_index_query_points_d if queries.shape[1] == 2 else L.rts_index_query_envelopes_d
+        else:
+            fn = L.rts_index_query_points if queries.shape[1] == 2 else L.rts_index_query_envelopes
         N.check(fn(self._handle(), int(predicate), queries.data_ptr(), queries.shape[0], handler,
                    ptr, _stream_handle(stream)), "Query")
 
@@ -261,7 +286,7 @@ class SpatialIndex:
         plan (leaf size, per-lane or per-wavefront traversal, Morton or input
         order) and, with `apply`, keeps the fastest for later queries of this
         predicate and query kind. Returns the choice and the sample timings."""
-        _check_array(queries, None, torch.float32, "Autotune")
+        _check_array(queries, None, self.coord_t, "Autotune")
         if queries.dim() != 2 or queries.shape[1] not in (2, 4):
             raise ValueError("Autotune: queries must be [n, 2] points or [n, 4] envelopes")
         c = N.rts_tune_config()
@@ -284,11 +309,13 @@ class SpatialIndex:
         return {f: getattr(s, f) for f, _ in N.rts_index_stats._fields_}
 
 
-def pack_updates(ids, envelopes):
-    """(ids int64 [n], envelopes float32 [n, 4]) -> int64 [n, 3], the 24-byte
-    rts_update records (thrust::pair<size_t, envelope_t>)."""
+def pack_updates(ids, envelopes, coord_t=torch.float32):
+    """(ids int64 [n], envelopes [n, 4]) -> int64 [n, 3], the 24-byte
+    rts_update records (thrust::pair<size_t, envelope_t>), or, with coord_t
+    float64, int64 [n, 5], the 40-byte rts_update_d records."""
     ids = ids.reshape(-1, 1).to(torch.int64)
-    env = envelopes.to(torch.float32).contiguous().view(torch.int64).reshape(-1, 2)
+    words = 2 if coord_t == torch.float32 else 4
+    env = envelopes.to(coord_t).contiguous().view(torch.int64).reshape(-1, words)
     return torch.cat([ids, env], dim=1).contiguous()
 
 

@@ -47,17 +47,19 @@ def mixture_means(rng, k=16):
     return rng.uniform(0.1, 0.9, size=(k, 2))
 
 
-def mixture_boxes(rng, means, n, side_mu, side_sigma=1.0, sigma=0.02, chunk=1 << 22):
+def mixture_boxes(rng, means, n, side_mu, side_sigma=1.0, sigma=0.02, chunk=1 << 22,
+                  dtype=np.float32):
     """Centres from the Gaussian mixture, clipped to [0, 1), log-normal sides;
-    box = centre -+ side/2 in FP32."""
-    out = np.empty((n, 4), np.float32)
+    box = centre -+ side/2 in `dtype` (float64: the same draws, kept in double,
+    so the coordinates are not floats)."""
+    out = np.empty((n, 4), dtype)
     for s in range(0, n, chunk):
         m = min(chunk, n - s)
         comp = rng.integers(0, means.shape[0], size=m)
         c = means[comp] + rng.normal(0.0, sigma, size=(m, 2))
-        c = np.clip(c, 0.0, float(ONE_BELOW)).astype(np.float32)
-        side = rng.lognormal(side_mu, side_sigma, size=(m, 2)).astype(np.float32)
-        half = (side * np.float32(0.5)).astype(np.float32)
+        c = np.clip(c, 0.0, float(ONE_BELOW)).astype(dtype)
+        side = rng.lognormal(side_mu, side_sigma, size=(m, 2)).astype(dtype)
+        half = (side * dtype(0.5)).astype(dtype)
         out[s:s + m, 0] = c[:, 0] - half[:, 0]
         out[s:s + m, 1] = c[:, 1] - half[:, 1]
         out[s:s + m, 2] = c[:, 0] + half[:, 0]
@@ -77,11 +79,11 @@ def config2(n_boxes=1_000_000, n_queries=1_000_000):
     return b, q
 
 
-def config3(n_boxes=10_000_000, n_queries=1_000_000):
+def config3(n_boxes=10_000_000, n_queries=1_000_000, dtype=np.float32):
     rb = _rng(3, 1)
     means = mixture_means(rb)
-    b = mixture_boxes(rb, means, n_boxes, math.log(1e-4))
-    q = mixture_boxes(_rng(3, 2), means, n_queries, math.log(5e-3))
+    b = mixture_boxes(rb, means, n_boxes, math.log(1e-4), dtype=dtype)
+    q = mixture_boxes(_rng(3, 2), means, n_queries, math.log(5e-3), dtype=dtype)
     return b, q
 
 

@@ -240,6 +240,54 @@ TEST(EnvelopeQueries, fp32_collect_pairs) {
   ASSERT_EQ(got[2].second, 1u);
 }
 
+// test/test_point_queries.h:92-120: the same case on a double index.
+TEST(PointQueries, fp64_contains_point) {
+  using dindex_t = SpatialIndex<double, 2>;
+  using dpoint_t = dindex_t::point_t;
+  using denv_t = dindex_t::envelope_t;
+  pinned_vector<denv_t> boxes;
+  pinned_vector<dpoint_t> points;
+  boxes.push_back(denv_t(dpoint_t(0, 0), dpoint_t(1, 1)));
+  boxes.push_back(denv_t(dpoint_t(0.2, 0.2), dpoint_t(0.8, 0.8)));
+  for (auto p : {dpoint_t(0.1, 0.1), dpoint_t(0.5, 0.5), dpoint_t(0.5, 0.6), dpoint_t(1.0, 1.1),
+                 dpoint_t(2.0, 2.0)})
+    points.push_back(p);
+  Stream s;
+  dindex_t index;
+  index.Init(Config());
+  index.Insert(ArrayView<denv_t>(boxes), s.cuda_stream());
+  counter().set(s.cuda_stream(), 0);
+  index.Query(Predicate::kContains, ArrayView<dpoint_t>(points), counter().data(), s.cuda_stream());
+  ASSERT_EQ(counter().get(s.cuda_stream()), 5ull);
+}
+
+// A point 1e-12 right of a box edge: outside in double, inside once both are
+// rounded to float -- the answer of the reference's double index, whose
+// points return float coordinates (geom/point.cuh:68).
+TEST(PointQueries, fp64_exact_and_rounded) {
+  using dindex_t = SpatialIndex<double, 2>;
+  using dpoint_t = dindex_t::point_t;
+  using denv_t = dindex_t::envelope_t;
+  pinned_vector<denv_t> boxes;
+  pinned_vector<dpoint_t> points;
+  boxes.push_back(denv_t(dpoint_t(0.0, 0.0), dpoint_t(0.3, 0.3)));
+  points.push_back(dpoint_t(0.3 + 1e-12, 0.1));
+  points.push_back(dpoint_t(0.3, 0.1));
+  Stream s;
+  for (uint32_t coord : {(uint32_t)RTS_COORD_F64, (uint32_t)RTS_COORD_F64_AS_F32}) {
+    dindex_t index;
+    Config c;
+    c.coord = coord;
+    index.Init(c);
+    index.Insert(ArrayView<denv_t>(boxes), s.cuda_stream());
+    counter().set(s.cuda_stream(), 0);
+    index.Query(Predicate::kContains, ArrayView<dpoint_t>(points), counter().data(),
+                s.cuda_stream());
+    ASSERT_EQ(counter().get(s.cuda_stream()), coord == RTS_COORD_F64 ? 1ull : 2ull);
+  }
+  ASSERT_TRUE(!boxes[0].Contains(points[0]) && boxes[0].Contains(points[1]));
+}
+
 TEST(PointQueries, bad_predicate_throws) {
   pinned_vector<envelope_t> boxes;
   boxes.push_back(envelope_t(point_t(0, 0), point_t(1, 1)));

@@ -23,9 +23,10 @@ def pinned(a):
 
 
 def make_index(boxes, batches=1, **cfg):
-    ix = R.SpatialIndex()
-    ix.Init(R.Config(**cfg))
+    """An index over `boxes` (float32 or, for a double index, float64)."""
     b = dev(boxes) if isinstance(boxes, np.ndarray) else boxes
+    ix = R.SpatialIndex(torch.float64 if b.dtype == torch.float64 else torch.float32)
+    ix.Init(R.Config(**cfg))
     n = b.shape[0]
     step = (n + batches - 1) // batches if batches > 1 else n
     for s in range(0, n, max(step, 1)):

@@ -54,6 +54,50 @@ def test_struct_layouts():
     assert ctypes.sizeof(N.rts_tune_config) == 16 and ctypes.sizeof(N.rts_tune_result) == 48
 
 
+LAYOUT_C = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "rts_c.h"
+#define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
+#define S(T) printf(#T " %zu\n", sizeof(T));
+int main(void) {
+  S(rts_config) F(rts_config, sort_queries) F(rts_config, coord)
+  S(rts_point_d) S(rts_envelope_d) S(rts_update_d) F(rts_update_d, box)
+  S(rts_update) S(rts_tune_result) S(rts_serve_config) S(rts_request) S(rts_request_result)
+  return 0;
+}
+"""
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    # The ctypes mirror of every struct has the C compiler's size and offsets.
+    import shutil
+    import subprocess
+    from rtspatial_amd import _native as N
+    gcc = shutil.which("gcc")
+    if not gcc:
+        pytest.skip("no gcc")
+    (tmp_path / "l.c").write_text(LAYOUT_C)
+    subprocess.run([gcc, "-std=c99", f"-I{ROOT}/include", str(tmp_path / "l.c"), "-o",
+                    str(tmp_path / "l")], check=True)
+    out = subprocess.run([str(tmp_path / "l")], check=True, stdout=subprocess.PIPE, text=True).stdout
+    got = dict(line.rsplit(" ", 1) for line in out.splitlines())
+    assert int(got["rts_config"]) == ctypes.sizeof(N.rts_config) == 80
+    assert int(got["rts_config.coord"]) == N.rts_config.coord.offset == 76
+    assert int(got["rts_point_d"]) == 16 and int(got["rts_envelope_d"]) == 32
+    assert int(got["rts_update_d"]) == 40 and int(got["rts_update_d.box"]) == 8
+    assert int(got["rts_update"]) == 24
+    for name in ("rts_tune_result", "rts_serve_config", "rts_request", "rts_request_result"):
+        assert int(got[name]) == ctypes.sizeof(getattr(N, name)), name
+
+
+def test_coord_default_is_float():
+    from rtspatial_amd import _native as N
+    c = N.rts_config()
+    N.lib().rts_config_default(ctypes.byref(c))
+    assert c.coord == N.RTS_COORD_F32
+
+
 def test_tune_config_defaults_and_null_args():
     from rtspatial_amd import _native as N
     c = N.rts_tune_config()

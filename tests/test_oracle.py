@@ -234,3 +234,73 @@ def test_wkt_fixtures_consistent(oracle):
     b = d["pairs_gaussian_boxes_uniform_boxes_contains"]
     key = lambda p: np.sort(np.ascontiguousarray(p).view(np.uint64).ravel())
     assert np.array_equal(key(a[:, ::-1]), key(b))
+
+
+# ------------------------------------------------------- double coordinates --
+def test_f64_known_answer(oracle, ka):
+    # test/test_point_queries.h:92-120 (fp64_contains_point) asserts 5 on the
+    # tiny case's coordinates as doubles.
+    c = ka["tiny_points"]
+    boxes = np.array(c["boxes"], np.float64)
+    pts = np.array(c["points"], np.float64)
+    assert oracle.bf_count64(oracle.CONTAINS, boxes, pts) == 5
+    assert oracle.bf_pairs64(oracle.CONTAINS, boxes, pts).tolist() == c["pairs_query_box"]
+
+
+def test_f64_equals_f32_on_float_exact_inputs(oracle, ref100k):
+    # Floats are exact doubles, so on float inputs the double join must equal
+    # the (known-answer-pinned) float join, ties and deletions included.
+    rng = np.random.default_rng(31)
+    b = ref100k[:20000]
+    alive = (rng.random(b.shape[0]) > 0.1).astype(np.uint8)
+    q = oracle.ref_uniform_boxes(300, 0.1, 0.1)
+    q[:20] = b[rng.integers(0, b.shape[0], 20)]  # exact equalities
+    for pred in (oracle.INTERSECTS, oracle.CONTAINS, oracle.WITHIN):
+        assert np.array_equal(oracle.bf_pairs64(pred, b.astype(np.float64), q.astype(np.float64),
+                                                alive=alive),
+                              oracle.bf_pairs(pred, b, q, alive=alive))
+    p = oracle.ref_uniform_points(400)
+    assert np.array_equal(oracle.bf_pairs64(oracle.CONTAINS, b.astype(np.float64), p.astype(np.float64)),
+                          oracle.bf_pairs(oracle.CONTAINS, b, p))
+
+
+# Cases whose answer flips when coordinates are rounded to float: the double
+# oracle must give the exact answer (evaluating in float would give the other).
+# Rounding to nearest is monotone and every predicate is a conjunction of <=
+# comparisons, so rounding can only add pairs: every flip is exactly false.
+F64_FLIPS = [
+    # (pred, box, query)
+    (1, [0.0, 0.0, 0.1, 0.1], [float(np.float32(0.1)), 0.05]),  # point just right of x1
+    (1, [0.1, 0.0, 0.2, 1.0], [float(np.nextafter(0.1, 0)), 0.5]),
+    (1, [0.0, 0.1, 1.0, 0.2], [0.5, float(np.nextafter(0.2, 1))]),  # just above y1
+    (0, [0.0, 0.0, 0.3, 0.3], [float(np.nextafter(0.3, 1)), 0.0, 0.5, 0.5]),  # gap of 1 ulp
+    (0, [0.0, 0.0, 0.5, 0.5], [0.2, -1.0, 0.3, float(np.nextafter(0.0, -1))]),
+    (1, [0.0, 0.0, 1.0, 1.0], [0.5, 0.5, 1.0 + 1e-12, 0.75]),  # sticks out by 1e-12
+    (1, [0.0, 0.2, 1.0, 1.0], [0.5, 0.2 - 1e-14, 0.6, 0.75]),
+    (2, [0.1 - 1e-12, 0.2, 0.3, 0.4], [0.1, 0.1, 0.5, 0.5]),   # starts left of the query
+    (2, [0.2, 0.2, 0.5 + 1e-12, 0.4], [0.1, 0.1, 0.5, 0.5]),   # ends right of it
+]
+
+
+@pytest.mark.parametrize("pred,box,query", F64_FLIPS)
+def test_f64_rounding_flips(oracle, pred, box, query):
+    b, q = np.float64(box), np.float64(query)
+    assert not oracle.evaluate64(pred, b, q)
+    # the same case rounded to float matches
+    assert oracle.evaluate(pred, np.float32(b), np.float32(q))
+
+
+def test_f64_matches_numpy_restatement(oracle):
+    # Vectorised double restatement on random doubles with near-ties.
+    rng = np.random.default_rng(9)
+    g = np.sort(rng.random((600, 2, 2)), axis=1)
+    g = g.transpose(0, 2, 1).reshape(-1, 4)[:, [0, 2, 1, 3]].copy()
+    q = np.sort(rng.random((80, 2, 2)), axis=1)
+    q = q.transpose(0, 2, 1).reshape(-1, 4)[:, [0, 2, 1, 3]].copy()
+    q[:40] = g[:40] + rng.choice([-1e-13, 0.0, 1e-13], size=(40, 4))  # within an ulp of float
+    G, Q = g[None, :, :], q[:, None, :]
+    inter = (Q[..., 0] <= G[..., 2]) & (Q[..., 2] >= G[..., 0]) & (Q[..., 1] <= G[..., 3]) & (Q[..., 3] >= G[..., 1])
+    cont = (Q[..., 0] >= G[..., 0]) & (Q[..., 2] <= G[..., 2]) & (Q[..., 1] >= G[..., 1]) & (Q[..., 3] <= G[..., 3])
+    within = (G[..., 0] >= Q[..., 0]) & (G[..., 2] <= Q[..., 2]) & (G[..., 1] >= Q[..., 1]) & (G[..., 3] <= Q[..., 3])
+    for pred, m in ((oracle.INTERSECTS, inter), (oracle.CONTAINS, cont), (oracle.WITHIN, within)):
+        assert np.array_equal(oracle.bf_pairs64(pred, g, q), np.argwhere(m).astype(np.uint32))
