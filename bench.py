@@ -526,6 +526,144 @@ def run_c3(args):
     return out
 
 
+# -------------------------------------------------------------- handlers --
+def run_handlers(args):
+    """The 10M x 1M intersects join with result handlers compiled outside the
+    library (rtspatial_amd/examples/librts_user_handlers.so, built on the
+    public device headers), against the built-in handlers on the same index."""
+    import ctypes
+
+    import torch
+
+    import rtspatial_amd as R
+    from rtspatial_amd import build as B
+    from rtspatial_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    boxes_np, queries_np = W.config3(args.boxes, args.queries)
+    boxes = torch.from_numpy(boxes_np).to(dev)
+    queries = torch.from_numpy(queries_np).to(dev)
+    nq = queries.shape[0]
+    pred = R.Predicate.kIntersects
+    ix = R.SpatialIndex()
+    ix.Init(R.Config(max_geometries=args.boxes))
+    ix.Insert(boxes)
+    ix.Build()
+    tuned = ix.Autotune(pred, queries) if not args.no_tune else None
+    log(f"autotune: {tuned}")
+    R._native.lib()
+    X = ctypes.CDLL(B.USER_HANDLERS_LIB)
+    vp, u64, i32, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32
+    X.rtsx_summary.argtypes = [vp, i32, vp, u64, i32, i32, u32, vp, vp, vp, vp]
+    X.rtsx_count.argtypes = [vp, i32, vp, u64, i32, i32, u32, vp, vp]
+    X.rtsx_reference_count.argtypes = [vp, i32, vp, u64, i32, i32, vp, vp]
+    X.rtsx_last_error.restype = ctypes.c_char_p
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    count = torch.zeros(nq, dtype=torch.int64, device=dev)
+    hsum = torch.zeros(nq, dtype=torch.int64, device=dev)
+    mn = torch.zeros(nq, dtype=torch.int32, device=dev)
+    counter = torch.zeros(1, dtype=torch.int64, device=dev)
+    cnt = R.SharedValue()
+    cnt.set(0)
+    ix.Query(pred, queries, cnt)
+    n_pairs = cnt.get()
+    queue = R.Queue(n_pairs + 1024)
+
+    def builtin_count():
+        cnt.tensor.zero_()
+        ix.Query(pred, queries, cnt)
+
+    def builtin_collect():
+        queue.Clear()
+        ix.Query(pred, queries, queue)
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError(X.rtsx_last_error().decode())
+
+    def summary():
+        count.zero_()
+        hsum.zero_()
+        mn.fill_(-1)
+        ok(X.rtsx_summary(ix._handle(), int(pred), queries.data_ptr(), nq, 0, 0, 0,
+                          count.data_ptr(), hsum.data_ptr(), mn.data_ptr(), sp))
+
+    def count_done():
+        ok(X.rtsx_count(ix._handle(), int(pred), queries.data_ptr(), nq, 0, 0, 0,
+                        count.data_ptr(), sp))
+
+    def reference_count():
+        counter.zero_()
+        ok(X.rtsx_reference_count(ix._handle(), int(pred), queries.data_ptr(), nq, 0, 0,
+                                  counter.data_ptr(), sp))
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
+        return {"ms": round(ms, 3), "mq_per_s": round(nq / ms / 1e3, 3)}
+
+    res = {}
+    for name, fn in (("builtin_count", builtin_count), ("builtin_collect", builtin_collect),
+                     ("count_done", count_done), ("summary", summary),
+                     ("reference_count", reference_count)):
+        log(f"timing {name}")
+        res[name] = timed(fn)
+        log(f"{name}: {res[name]}")
+    # The handlers' results agree with the built-in ones on the same index.
+    count_done()
+    torch.cuda.synchronize()
+    per_q_done = count.clone()
+    summary()
+    torch.cuda.synchronize()
+    checks = {"summary_total": int(count.sum().item()), "count_done_total": int(per_q_done.sum().item()),
+              "per_query_equal": bool(torch.equal(per_q_done, count))}
+    reference_count()
+    torch.cuda.synchronize()
+    checks["reference_total"] = int(counter.item())
+    checks["builtin_total"] = int(n_pairs)
+    agree = (checks["summary_total"] == checks["count_done_total"] == checks["reference_total"]
+             == n_pairs) and checks["per_query_equal"]
+    ix.Sync()
+    out = {
+        "metric": METRIC,
+        "value": res["summary"]["mq_per_s"],
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["summary"]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"range-intersects, skewed: {args.boxes} Gaussian-mixture boxes x {nq} "
+                         f"query envelopes ({n_pairs} pairs); value = the caller-compiled per-pair "
+                         f"summary handler (per query: count, splitmix64 hash sum, min id)"),
+            "path": "user handlers", "boxes": args.boxes, "queries": nq, "pairs": n_pairs,
+        },
+        "handlers": res,
+        "checks": checks,
+        "results_agree": agree,
+        "tuned_plan": tuned,
+        "cpu_baseline": None,
+    }
+    if not agree:
+        log(f"MISMATCH between handler results: {checks}")
+    return out
+
+
 # -------------------------------------------------------------------- c4 --
 def run_c4(args):
     import torch
@@ -650,7 +788,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["served", "c3", "c3f64", "c4", "copy"], default="served")
+    ap.add_argument("--workload", choices=["served", "c3", "c3f64", "handlers", "c4", "copy"],
+                    default="served")
     ap.add_argument("--rounds", type=int, default=20)
     ap.add_argument("--boxes", type=int, default=10_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -669,12 +808,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    out = {"served": run_served, "c3": run_c3, "c3f64": run_c3, "c4": run_c4,
+    out = {"served": run_served, "c3": run_c3, "c3f64": run_c3, "handlers": run_handlers,
+           "c4": run_c4,
            "copy": run_copy_calibration}[args.workload](args)
     print(json.dumps(out), flush=True)
     check = out.get("bit_exact_check")
     if check and not check["counts_and_hashes_equal"]:
         sys.exit(3)  # results differ: the line above says where
+    if out.get("results_agree") is False:
+        sys.exit(3)
 
 
 if __name__ == "__main__":

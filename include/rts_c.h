@@ -222,6 +222,52 @@ int rts_index_query_envelopes_d(rts_index* index, rts_predicate pred,
                                 const rts_envelope_d* queries, uint64_t n, rts_handler handler,
                                 void* arg, rts_stream_t stream);
 
+/* The index's tree, for kernels a caller compiles around its own result
+ * handler (include/rtspatial/device/query.h). The reference compiles its
+ * query shaders around a handler header of rtspatial_handle_* functions
+ * (default_handlers/counting_handlers.h, test/CMakeLists.txt:8); here the
+ * caller's HIP code instantiates the traversal kernels with its handler and
+ * runs them on this view. rts_index_tree_view brings the tree up to date on `stream` (a pending
+ * build or refit is queued there) and fills `out`; the view stays valid until
+ * the next call that changes the index (insert, delete, update, clear, build).
+ * All pointers are device pointers. */
+typedef struct rts_tree_view {
+  const void* leaf_boxes;    /* float (x0, y0, x1, y1) per leaf, in tree order */
+  const uint32_t* leaf_ids;  /* box id per leaf */
+  const void* nodes;         /* n - 1 node records of 64 bytes */
+  const void* root_box;      /* float (x0, y0, x1, y1) of the whole tree */
+  const uint32_t* root_live; /* live leaves in the tree */
+  uint32_t* error;           /* the index's device error word */
+  const void* boxes64;       /* RTS_COORD_F64: rts_envelope_d per id; otherwise NULL */
+  uint32_t n;                /* leaves (0: nothing to query) */
+  uint32_t leaf_size;        /* leaf-range size of the index's config */
+  uint32_t coord;            /* rts_coord of the index */
+  uint32_t reserved;
+} rts_tree_view;
+
+int rts_index_tree_view(rts_index* index, rts_stream_t stream, rts_tree_view* out);
+
+/* The plan the library runs a query call of this kind and size under: the
+ * config's settings, else the autotuner's stored plan, else the defaults.
+ * Kernels compiled by the caller follow it to run as the library's do. */
+typedef struct rts_query_plan {
+  uint32_t traversal;    /* RTS_TRAVERSAL_THREAD or RTS_TRAVERSAL_WAVE */
+  uint32_t sort_queries; /* 1 = Morton-ordered, 2 = input order */
+  uint32_t leaf_size;    /* leaf-range size */
+  uint32_t reserved;
+} rts_query_plan;
+
+int rts_index_query_plan(rts_index* index, rts_predicate pred, int points, uint64_t n,
+                         rts_query_plan* out);
+
+/* order[i] = index of the query that comes i-th in Morton order of the query
+ * points / envelope centres (over the index's scene bounds; the tree must be
+ * current, e.g. after rts_index_tree_view on the same stream). `queries` are
+ * floats for a float index and doubles for a double one; n < 2^32. Queued on
+ * `stream`. */
+int rts_index_order_queries(rts_index* index, const void* queries, uint64_t n, int points,
+                            uint32_t* order, rts_stream_t stream);
+
 /* Number of ids handed out so far (live and deleted). */
 uint64_t rts_index_size(const rts_index* index);
 

@@ -1,5 +1,8 @@
-// Device-side LBVH traversal cores shared by the per-request query kernels
-// (query.hip) and the persistent batched kernel (batch.hip).
+// Device-side LBVH traversal cores shared by the library's query kernels
+// (one query per lane or per wavefront, per request or in persistent
+// batches) and by kernels a caller compiles around its own result handler
+// (rtspatial/device/query.h). Results leave through an emitter:
+// emit(k, geom_id, query_id) is called for the k-th result of a query.
 //
 // Node tests are necessary conditions of the leaf predicate on exact FP32
 // bounds, so pruning never drops a true pair; every reported pair passed the
@@ -18,9 +21,9 @@
 // the query's, the double box is read and tested exactly.
 #pragma once
 
-#include "round.h"
-#include "rts_internal.h"
-#include "wave.h"
+#include "rtspatial/device/round.h"
+#include "rtspatial/device/tree.h"
+#include "rtspatial/device/wave.h"
 
 namespace rts {
 
@@ -155,6 +158,27 @@ __device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, cons
   return P::exact(t.boxes64[t.leaf_ids[leaf]], q.q);
 }
 
+// Emitters. The result queue: the query's k-th pair goes to slot base + k,
+// dropped at or past `cap`.
+struct QueueEmit {
+  rts_pair* out;
+  uint64_t cap;
+  uint64_t base;
+  __device__ void operator()(uint64_t k, uint32_t geom_id, uint32_t query_id) const {
+    const uint64_t slot = base + k;
+    if (slot < cap) {
+      rts_pair pr;
+      pr.geom_id = geom_id;
+      pr.query_id = query_id;
+      out[slot] = pr;
+    }
+  }
+};
+// Counting walks emit nothing.
+struct NoEmit {
+  __device__ void operator()(uint64_t, uint32_t, uint32_t) const {}
+};
+
 __device__ __forceinline__ void lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -220,13 +244,12 @@ struct WaveLds {
   uint32_t map[kScanStep];
 };
 
-// Returns the query's number of results (wave-uniform). With WRITE the pairs
-// (geom_id, qid) go to out[obase ...], slots >= cap dropped; without WRITE,
-// subtrees inside the query are counted from their live-leaf counts.
-template <int PRED, bool POINTS, bool WRITE, typename Q>
+// Returns the query's number of results (wave-uniform). With WRITE every
+// pair (geom_id, qid) is emitted, numbered from 0 in emission order; without
+// WRITE, subtrees inside the query are counted from their live-leaf counts.
+template <int PRED, bool POINTS, bool WRITE, typename Q, typename Emit>
 __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qid,
-                               rts_pair* out, const uint64_t cap, const uint64_t obase,
-                               WaveLds& L) {
+                               const Emit& emit, WaveLds& L) {
   constexpr bool kCounting = !WRITE;
   const uint32_t lane = lane_id();
   uint32_t* stack = L.stack;
@@ -409,15 +432,7 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
         for (int k = 0; k < kPerLane; k++) {
           const bool h = (hitmask >> k) & 1u;
           const uint64_t mask = ballot(h);
-          if (WRITE && h) {
-            const uint64_t slot = obase + emitted + step_hits + mask_rank(mask);
-            if (slot < cap) {
-              rts_pair pr;
-              pr.geom_id = gid[k];
-              pr.query_id = qid;
-              out[slot] = pr;
-            }
-          }
+          if (WRITE && h) emit(emitted + step_hits + mask_rank(mask), gid[k], qid);
           step_hits += (uint32_t)__popcll(mask);
         }
         emitted += step_hits;
@@ -431,14 +446,50 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
   return emitted + wave_sum_u64(inside_live);
 }
 
+// ------------------------------------------------------------- tickets ----
+// Persistent waves take queries from one ticket counter per XCD. The query
+// order is cut into 8 contiguous slices and workgroups are dealt to the XCDs
+// round-robin, so wave w first drains slice w % 8 -- neighbouring (Morton-
+// ordered) queries then share one XCD's L2 -- and steals from the other
+// slices once its own is empty; costs are heavy-tailed, so no static split
+// balances them. The counters take kXcds * kTicketStride words, zeroed
+// before the launch.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kTicketStride = 16;  // one 64-byte line per counter
+
+// Next query index for this wave, or >= n when every slice is drained.
+// `slice` is the slice to try first; it becomes the one taken from.
+__device__ __forceinline__ uint32_t grab_query(uint32_t* tickets, uint32_t n, uint32_t& slice) {
+  const uint32_t per = (n + kXcds - 1) / kXcds;
+  uint32_t got = 0xFFFFFFFFu, from = slice;
+  if (lane_id() == 0) {
+    for (uint32_t k = 0; k < kXcds; k++) {
+      const uint32_t x = (slice + k) % kXcds;
+      const uint32_t begin = x * per;
+      const uint32_t end = begin + per < n ? begin + per : n;
+      if (begin >= end) continue;
+      // No relaxed load ahead of the atomic: across XCDs it is a second
+      // memory-side round trip on the same hot line (measured ~1.5x slower).
+      const uint32_t v = atomicAdd(tickets + x * kTicketStride, 1u);
+      if (v < end - begin) {
+        got = begin + v;
+        from = x;
+        break;
+      }
+    }
+  }
+  slice = readlane_u32(from, 0);
+  return readlane_u32(got, 0);
+}
+
 // ---------------------------------------------------------------- lane ----
 // One query per lane with a private stack; for queries with few results
 // (points, data-contains-query). Counting mode counts subtrees inside the
-// query from their live-leaf counts; writing mode writes the query's pairs to
-// out[obase ...] in traversal order. Returns the query's number of results.
-template <int PRED, bool POINTS, bool WRITE, typename Q>
+// query from their live-leaf counts; writing mode emits the query's pairs in
+// traversal order. Returns the query's number of results.
+template <int PRED, bool POINTS, bool WRITE, typename Q, typename Emit>
 __device__ uint64_t lane_query(const TreeView& t, const Q& qb, const uint32_t qid,
-                               rts_pair* out, const uint64_t cap, const uint64_t obase) {
+                               const Emit& emit) {
   uint32_t stack[kStackDepth];
   int sp = 0;
   uint32_t cur = 1, end = 0;  // leaf range being scanned: [cur, end]
@@ -464,12 +515,7 @@ __device__ uint64_t lane_query(const TreeView& t, const Q& qb, const uint32_t qi
     if (cur <= end) {
       const uint32_t prim = cur++;
       if (prim < t.n && (all || leaf_test<PRED, POINTS>(t, prim, t.leaf_boxes[prim], qb))) {
-        if (WRITE && obase + found < cap) {
-          rts_pair pr;
-          pr.geom_id = t.leaf_ids[prim];
-          pr.query_id = qid;
-          out[obase + found] = pr;
-        }
+        if (WRITE) emit(found, t.leaf_ids[prim], qid);
         found++;
       }
       continue;

@@ -9,7 +9,11 @@
 // time, as the reference chooses its shader handler: include
 // rtspatial/default_handlers/counting_handlers.h before this header for the
 // counting handler (arg: device unsigned long long*), otherwise arg is a
-// Queue::DeviceObject() (collecting handler).
+// Queue::DeviceObject() (collecting handler). In a file compiled as HIP,
+// QueryWith(pred, queries, handler, stream) takes a handler functor of the
+// caller's (rtspatial/device/query.h), and with RTSPATIAL_USER_HANDLERS
+// defined, Query(..., void* arg, ...) calls the caller's rtspatial_handle_*
+// functions, as a reference build with a custom handler header does.
 #ifndef RTSPATIAL_AMD_SPATIAL_INDEX_H
 #define RTSPATIAL_AMD_SPATIAL_INDEX_H
 
@@ -27,6 +31,9 @@
 #include "rtspatial/utils/array_view.h"
 #include "rtspatial/utils/exception.h"
 #include "rtspatial/utils/queue.h"
+#if defined(__HIP__)
+#include "rtspatial/device/query.h"
+#endif
 
 namespace rtspatial {
 
@@ -185,12 +192,38 @@ class SpatialIndex<COORD_T, 2> {
 
   // spatial_index.cuh:352-391
   void Query(Predicate p, ArrayView<point_t> queries, void* arg, hipStream_t stream = nullptr) {
+#ifdef RTSPATIAL_USER_HANDLERS
+    QueryWithReferenceHandlers(handle(), p, queries.data(), queries.size(), true, kDoubles, arg,
+                               stream);
+#else
     Query(p, queries, static_cast<rts_handler>(RTSPATIAL_HANDLER), arg, stream);
+#endif
   }
   // spatial_index.cuh:407-515
   void Query(Predicate p, ArrayView<envelope_t> queries, void* arg, hipStream_t stream = nullptr) {
+#ifdef RTSPATIAL_USER_HANDLERS
+    QueryWithReferenceHandlers(handle(), p, queries.data(), queries.size(), false, kDoubles, arg,
+                               stream);
+#else
     Query(p, queries, static_cast<rts_handler>(RTSPATIAL_HANDLER), arg, stream);
+#endif
   }
+
+#if defined(__HIP__)
+  // Results to a handler functor of the caller's (rtspatial/device/query.h).
+  template <typename Handler>
+  void QueryWith(Predicate p, ArrayView<point_t> queries, const Handler& h,
+                 hipStream_t stream = nullptr, uint32_t traversal = RTS_TRAVERSAL_AUTO) {
+    rtspatial::QueryWith(handle(), p, queries.data(), queries.size(), true, kDoubles, h, stream,
+                         traversal);
+  }
+  template <typename Handler>
+  void QueryWith(Predicate p, ArrayView<envelope_t> queries, const Handler& h,
+                 hipStream_t stream = nullptr, uint32_t traversal = RTS_TRAVERSAL_AUTO) {
+    rtspatial::QueryWith(handle(), p, queries.data(), queries.size(), false, kDoubles, h, stream,
+                         traversal);
+  }
+#endif
 
   // The same with the handler named at run time.
   void Query(Predicate p, ArrayView<point_t> queries, rts_handler handler, void* arg,
@@ -214,6 +247,7 @@ class SpatialIndex<COORD_T, 2> {
   rts_index* c_handle() { return handle(); }
 
  private:
+  static constexpr bool kDoubles = std::is_same<COORD_T, double>::value;
   rts_index* handle() const {
     if (!ix_) throw Exception("SpatialIndex::Init has not been called");
     return ix_;

@@ -41,6 +41,31 @@ class rts_config(ctypes.Structure):
     ]
 
 
+class rts_tree_view(ctypes.Structure):
+    _fields_ = [
+        ("leaf_boxes", ctypes.c_void_p),
+        ("leaf_ids", ctypes.c_void_p),
+        ("nodes", ctypes.c_void_p),
+        ("root_box", ctypes.c_void_p),
+        ("root_live", ctypes.c_void_p),
+        ("error", ctypes.c_void_p),
+        ("boxes64", ctypes.c_void_p),
+        ("n", ctypes.c_uint32),
+        ("leaf_size", ctypes.c_uint32),
+        ("coord", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class rts_query_plan(ctypes.Structure):
+    _fields_ = [
+        ("traversal", ctypes.c_uint32),
+        ("sort_queries", ctypes.c_uint32),
+        ("leaf_size", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 class rts_dev_queue(ctypes.Structure):
     _fields_ = [
         ("data", ctypes.c_void_p),
@@ -167,6 +192,9 @@ SIGNATURES = [
     ("rts_index_query_points_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_query_envelopes_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_size", _U64, [_VP]),
+    ("rts_index_tree_view", _I32, [_VP, _VP, ctypes.POINTER(rts_tree_view)]),
+    ("rts_index_query_plan", _I32, [_VP, _I32, _I32, _U64, ctypes.POINTER(rts_query_plan)]),
+    ("rts_index_order_queries", _I32, [_VP, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_build", _I32, [_VP, _VP]),
     ("rts_index_sync", _I32, [_VP, _VP]),
     ("rts_index_get_stats", _I32, [_VP, ctypes.POINTER(rts_index_stats)]),

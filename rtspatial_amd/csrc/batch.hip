@@ -18,15 +18,14 @@
 // replays any batch.
 #include "batch.h"
 #include "morton.h"
-#include "traverse.h"
-#include "wave.h"
+#include "rtspatial/device/traverse.h"
+#include "rtspatial/device/wave.h"
 
 namespace rts {
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
-constexpr uint32_t kXcds = 8;
 
 __device__ __forceinline__ uint32_t find_request(const BatchRequest* reqs, uint32_t nreq,
                                                  uint32_t tile) {
@@ -68,9 +67,9 @@ __device__ void run_lane_tile(const TreeView& t, const BatchRequest& r, uint32_t
   const QueryF32 qb{load_query(r.queries, POINTS, q)};
   const uint64_t g = r.qbase + q;
   if (WRITE) {
-    lane_query<PRED, POINTS, true>(t, qb, q + r.qid_base, a.arena, a.arena_cap, a.offsets[g]);
+    lane_query<PRED, POINTS, true>(t, qb, q + r.qid_base, QueueEmit{a.arena, a.arena_cap, a.offsets[g]});
   } else {
-    a.counts[g] = (uint32_t)lane_query<PRED, POINTS, false>(t, qb, q, nullptr, 0, 0);
+    a.counts[g] = (uint32_t)lane_query<PRED, POINTS, false>(t, qb, q, NoEmit{});
   }
 }
 
@@ -80,9 +79,10 @@ __device__ void run_wave_tile(const TreeView& t, const BatchRequest& r, uint32_t
   const QueryF32 qb{load_query(r.queries, POINTS, q)};
   const uint64_t g = r.qbase + q;
   if (WRITE) {
-    wave_query<PRED, POINTS, true>(t, qb, q + r.qid_base, a.arena, a.arena_cap, a.offsets[g], L);
+    wave_query<PRED, POINTS, true>(t, qb, q + r.qid_base, QueueEmit{a.arena, a.arena_cap, a.offsets[g]},
+                                   L);
   } else {
-    const uint64_t c = wave_query<PRED, POINTS, false>(t, qb, q, nullptr, 0, 0, L);
+    const uint64_t c = wave_query<PRED, POINTS, false>(t, qb, q, NoEmit{}, L);
     if (lane_id() == 0) a.counts[g] = (uint32_t)c;
   }
 }

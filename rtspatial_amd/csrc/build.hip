@@ -18,7 +18,7 @@
 
 #include "morton.h"
 #include "rts_internal.h"
-#include "wave.h"
+#include "rtspatial/device/wave.h"
 
 namespace rts {
 namespace {

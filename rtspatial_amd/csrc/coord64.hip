@@ -6,7 +6,7 @@
 // All kernels are streaming passes (32-byte double box in, 16-byte float box
 // out), HBM-bound.
 #include "morton.h"
-#include "round.h"
+#include "rtspatial/device/round.h"
 #include "rts_internal.h"
 
 namespace rts {

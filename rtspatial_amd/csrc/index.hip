@@ -758,6 +758,63 @@ int rts_index_query_envelopes_d(rts_index* ix, rts_predicate pred, const rts_env
 
 uint64_t rts_index_size(const rts_index* ix) { return ix ? ix->n : 0; }
 
+int rts_index_query_plan(rts_index* ix, rts_predicate pred, int points, uint64_t n,
+                         rts_query_plan* out) {
+  RTS_CHECK_ARG(ix, "rts_index_query_plan: null index");
+  RTS_CHECK_ARG(out, "rts_index_query_plan: null plan");
+  RTS_CHECK_ARG(pred >= RTS_INTERSECTS && pred <= RTS_WITHIN, "rts_index_query_plan: unknown predicate");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  const Plan p = plan_query(ix, pred, points != 0, n);
+  out->traversal = (uint32_t)p.traversal;
+  out->sort_queries = p.sort ? 1u : 2u;
+  out->leaf_size = p.leaf_size;
+  out->reserved = 0;
+  return RTS_OK;
+}
+
+int rts_index_order_queries(rts_index* ix, const void* queries, uint64_t n, int points,
+                            uint32_t* order, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_order_queries: null index");
+  RTS_CHECK_ARG(n < (1ull << 32), "rts_index_order_queries: n must be below 2^32");
+  if (n == 0) return RTS_OK;
+  RTS_CHECK_ARG(queries && order, "rts_index_order_queries: null array");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  char* block = nullptr;
+  const size_t keys_bytes = align_up(4 * n * sizeof(uint32_t));
+  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&block), keys_bytes + sort_scratch_bytes(n),
+                         stream));
+  const uint32_t* perm = nullptr;
+  int rc = order_queries(ix, queries, n, points != 0, ix->f64(), block, block + keys_bytes, stream,
+                         &perm);
+  if (!rc) RTS_HIP(hipMemcpyAsync(order, perm, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+  RTS_HIP(hipFreeAsync(block, stream));
+  return rc;
+}
+
+int rts_index_tree_view(rts_index* ix, rts_stream_t stream, rts_tree_view* out) {
+  RTS_CHECK_ARG(ix, "rts_index_tree_view: null index");
+  RTS_CHECK_ARG(out, "rts_index_tree_view: null view");
+  std::memset(out, 0, sizeof(*out));
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  out->coord = ix->cfg.coord;
+  out->leaf_size = ix->cfg.leaf_size;
+  if (ix->n == 0) return RTS_OK;
+  int rc = ensure_built(ix, stream);
+  if (rc) return rc;
+  const TreeView t = tree_view(ix);
+  out->leaf_boxes = t.leaf_boxes;
+  out->leaf_ids = t.leaf_ids;
+  out->nodes = t.nodes;
+  out->root_box = t.root_box;
+  out->root_live = t.root_live;
+  out->error = t.error;
+  out->boxes64 = t.boxes64;
+  out->n = t.n;
+  return RTS_OK;
+}
+
 int rts_index_build(rts_index* ix, rts_stream_t stream) {
   RTS_CHECK_ARG(ix, "rts_index_build: null index");
   std::lock_guard<std::mutex> lk(ix->mu);

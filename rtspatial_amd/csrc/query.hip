@@ -14,8 +14,8 @@
 // device-scope atomic per ~1000 pairs and fully coalesced 8-byte stores, in
 // place of the reference's one atomic per result (utils/queue.h:22-27).
 #include "rts_internal.h"
-#include "traverse.h"
-#include "wave.h"
+#include "rtspatial/device/traverse.h"
+#include "rtspatial/device/wave.h"
 
 namespace rts {
 namespace {
@@ -190,40 +190,6 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
 // holds each query's pairs contiguously, in query order.
 enum { kWaveCount = 0, kWaveCountSink = 1, kWaveWrite = 2 };
 
-// Persistent waves take queries from one ticket counter per XCD. The query
-// order is cut into 8 contiguous slices and workgroups are dealt to the XCDs
-// round-robin, so wave w first drains slice w % 8 -- neighbouring (Morton-
-// ordered) queries then share one XCD's L2 -- and steals from the other
-// slices once its own is empty; costs are heavy-tailed, so no static split
-// balances them.
-constexpr uint32_t kXcds = 8;
-constexpr uint32_t kTicketStride = 16;  // one 64-byte line per counter
-
-// Next query index for this wave, or >= n when every slice is drained.
-// `slice` is the slice to try first; it becomes the one taken from.
-__device__ __forceinline__ uint32_t grab_query(uint32_t* tickets, uint32_t n, uint32_t& slice) {
-  const uint32_t per = (n + kXcds - 1) / kXcds;
-  uint32_t got = 0xFFFFFFFFu, from = slice;
-  if (lane_id() == 0) {
-    for (uint32_t k = 0; k < kXcds; k++) {
-      const uint32_t x = (slice + k) % kXcds;
-      const uint32_t begin = x * per;
-      const uint32_t end = begin + per < n ? begin + per : n;
-      if (begin >= end) continue;
-      // No relaxed load ahead of the atomic: across XCDs it is a second
-      // memory-side round trip on the same hot line (measured ~1.5x slower).
-      const uint32_t v = atomicAdd(tickets + x * kTicketStride, 1u);
-      if (v < end - begin) {
-        got = begin + v;
-        from = x;
-        break;
-      }
-    }
-  }
-  slice = readlane_u32(from, 0);
-  return readlane_u32(got, 0);
-}
-
 // One wavefront per workgroup, so that a wave's LDS is released by itself.
 template <int PRED, bool POINTS, int MODE, bool F64>
 __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const WaveArgs a) {
@@ -241,11 +207,12 @@ __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const
     load_q(q.queries, POINTS, qi, qb);
     if (MODE == kWaveWrite) {
       const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
-      wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base, dq->data, dq->capacity,
-                                     a.base[0] + a.offsets[qi], s_lds);
+      wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base,
+                                     QueueEmit{dq->data, dq->capacity, a.base[0] + a.offsets[qi]},
+                                     s_lds);
       continue;
     }
-    const uint64_t all = wave_query<PRED, POINTS, false>(t, qb, 0, nullptr, 0, 0, s_lds);
+    const uint64_t all = wave_query<PRED, POINTS, false>(t, qb, 0, NoEmit{}, s_lds);
     if (lane_id() == 0) {
       if (MODE == kWaveCount) {
         a.counts[qi] = (uint32_t)all;

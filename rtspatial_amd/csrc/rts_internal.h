@@ -8,76 +8,18 @@
 #include <string>
 
 #include "rts_c.h"
+#include "rtspatial/device/tree.h"
 
 namespace rts {
 
-// ------------------------------------------------------------ constants ----
-constexpr int kWave = 64;                 // CDNA wavefront width
-constexpr uint32_t kLeafFlag = 0x80000000u;   // child ref: leaf (sorted position)
-constexpr uint32_t kRangeFlag = 0x40000000u;  // stack entry: scan the range of an internal node
-constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
 constexpr uint64_t kMaxBoxes = (1ull << 30) - 1;  // ids and sorted positions fit in 30 bits
-constexpr int kMortonBits = 30;           // 15 bits per axis
-constexpr int kStackDepth = 72;           // >= max Karras depth (64) + 2
-
-// --------------------------------------------------------------- layout ----
-// Karras LBVH over n sorted leaves: internal nodes 0..n-2 (root 0), one
-// 64-byte record each, so expanding a node is one cache-line fetch:
-//   box[0], box[1]  bounds of the left / right child
-//   meta            {left ref, right ref, first leaf, last leaf}
-//                   ref = internal index, or sorted leaf position | kLeafFlag
-//   live            {live leaves below the left child, ... the right child}
-// parent[j] = (parent << 1) | side, for internal j in [0, n-1) and leaf k at
-// [n-1+k]. Leaves are the boxes in Morton order: leaf_boxes[k], id leaf_ids[k].
-struct __align__(64) NodeRec {
-  float4 box[2];
-  uint4 meta;
-  uint4 live;
-};
-static_assert(sizeof(NodeRec) == 64, "node record is one 64-byte line");
-
-// A box with double coordinates (rts_envelope_d): x0, y0, x1, y1.
-struct Box64 {
-  double x0, y0, x1, y1;
-};
-static_assert(sizeof(Box64) == sizeof(rts_envelope_d), "Box64 is rts_envelope_d");
-
-struct TreeView {
-  const float4* leaf_boxes;
-  const uint32_t* leaf_ids;
-  const NodeRec* nodes;
-  const float4* root_box;     // device scalar
-  const uint32_t* root_live;  // device scalar: live leaves in the tree
-  uint32_t n;                 // leaves
-  uint32_t leaf_size;         // ranges with <= leaf_size leaves are scanned directly
-  uint32_t* error;            // device word: bit 1 refit walk, bit 2 traversal bound
-  const Box64* boxes64;       // double index: the boxes by id (else null)
-};
-
-// Device error bits (the index's error word).
-constexpr uint32_t kErrSortWait = 1u;
-constexpr uint32_t kErrRefitWalk = 2u;
-constexpr uint32_t kErrTraversal = 4u;
+constexpr int kMortonBits = 30;                   // 15 bits per axis
 
 // Result sink resolved from (handler, arg).
 struct SinkArgs {
   int mode;          // rts_handler
   const void* arg;   // counter or rts_dev_queue device pointer
 };
-
-// ----------------------------------------------------------- predicates ----
-// Exact closed-interval tests in FP32 (reference geom/envelope.cuh:29-69).
-// Boxes are float4 (x0, y0, x1, y1). Every comparison with a NaN is false, so
-// a deleted box (NaN coordinates) never matches.
-__device__ __forceinline__ bool contains_point(const float4 b, float px, float py) {
-  return px >= b.x && px <= b.z && py >= b.y && py <= b.w;
-}
-__device__ __forceinline__ bool contains_box(const float4 a, const float4 o) {
-  return o.x >= a.x && o.z <= a.z && o.y >= a.y && o.w <= a.w;
-}
-__device__ __forceinline__ bool intersects(const float4 a, const float4 o) {
-  return o.x <= a.z && o.z >= a.x && o.y <= a.w && o.w >= a.y;
-}
 
 // ----------------------------------------------------- host-side kernels ----
 // Thin launch wrappers (defined in the .hip files). All are asynchronous on

@@ -4,7 +4,7 @@
 // scan that adds the tile's offset. Also the queue reservation: one thread
 // adds the grand total to the result queue's counter.
 #include "rts_internal.h"
-#include "wave.h"
+#include "rtspatial/device/wave.h"
 
 namespace rts {
 namespace {

@@ -39,8 +39,8 @@
 #include "batch.h"
 #include "index_state.h"
 #include "rts_internal.h"
-#include "traverse.h"
-#include "wave.h"
+#include "rtspatial/device/traverse.h"
+#include "rtspatial/device/wave.h"
 
 namespace rts {
 namespace {

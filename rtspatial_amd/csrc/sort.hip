@@ -13,7 +13,7 @@
 // wait: a look-back that spins too long sets an error word and stops, so the
 // kernel always finishes.
 #include "rts_internal.h"
-#include "wave.h"
+#include "rtspatial/device/wave.h"
 
 namespace rts {
 namespace {

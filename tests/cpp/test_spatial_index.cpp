@@ -12,38 +12,16 @@
 #include <unordered_set>
 
 #include "mini_test.h"
+#include "ref_inputs.h"
 
 using namespace rtspatial;
+using ref_inputs::uniform_boxes;
+using ref_inputs::uniform_points;
 using index_t = SpatialIndex<float, 2>;
 using point_t = index_t::point_t;
 using envelope_t = index_t::envelope_t;
 
 namespace {
-
-// The reference tests' input streams (test/test_commons.h:22-67): std::mt19937
-// seeded with the element count, libstdc++ uniform_real_distribution<float>.
-std::vector<envelope_t> uniform_boxes(size_t n, float max_w, float max_h) {
-  std::mt19937 mt(n);
-  std::uniform_real_distribution<float> min_x(0.0, 1.0), min_y(0.0, 1.0);
-  std::uniform_real_distribution<float> w_dist(0.0, max_w), h_dist(0.0, max_h);
-  std::vector<envelope_t> out(n);
-  for (size_t i = 0; i < n; i++) {
-    float x = min_x(mt), y = min_y(mt), w = w_dist(mt), h = h_dist(mt);
-    out[i] = envelope_t(point_t(x, y), point_t(std::min(x + w, 1.0f), std::min(y + h, 1.0f)));
-  }
-  return out;
-}
-
-std::vector<point_t> uniform_points(size_t n) {
-  std::mt19937 mt(n);
-  std::uniform_real_distribution<float> gx(0.0, 1.0), gy(0.0, 1.0);
-  std::vector<point_t> out(n);
-  for (size_t i = 0; i < n; i++) {
-    float x = gx(mt), y = gy(mt);
-    out[i] = point_t(x, y);
-  }
-  return out;
-}
 
 // The counting handler's device counter; created in main() and released
 // before main() returns (no device memory is freed during static teardown).

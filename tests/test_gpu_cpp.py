@@ -8,15 +8,19 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "tests", "cpp", "bin", "test_spatial_index")
 
 
-def test_cpp_facade_suite():
+@pytest.mark.parametrize("name", ["test_spatial_index", "test_user_handlers"])
+def test_cpp_facade_suite(name):
+    # test_user_handlers is compiled as HIP: the reference's handler functions
+    # and handler functors of its own, instantiated into the traversal kernels.
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    assert os.path.exists(BIN), "build first: python -c 'import __graft_entry__ as g; g.build()'"
-    r = subprocess.run([BIN], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", name)
+    assert os.path.exists(exe), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    print(f"[gpu-test] C++ suite {name}", flush=True)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "0 failed" in r.stdout
