@@ -664,6 +664,75 @@ def run_handlers(args):
     return out
 
 
+# --------------------------------------------------------------- inserts --
+def run_inserts(args):
+    """Insert batches of growing size into the resident 10M-box index, each
+    followed by a query batch: the first query after an Insert pays the
+    rebuild, the second does not. The reference builds one GAS per Insert
+    under an IAS instead (spatial_index.cuh:142-196)."""
+    import torch
+
+    import rtspatial_amd as R
+    from rtspatial_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    rng = W._rng(3, 1)
+    means = W.mixture_means(rng)
+    boxes = torch.from_numpy(W.mixture_boxes(rng, means, args.boxes, math.log(1e-4))).to(dev)
+    extra = torch.from_numpy(W.mixture_boxes(W._rng(3, 7), means, 1_111_000, math.log(1e-4))).to(dev)
+    pts = torch.from_numpy(W.mixture_boxes(W._rng(3, 8), means, 100_000, math.log(1e-4))[:, :2]
+                           .copy()).to(dev)
+    ix = R.SpatialIndex()
+    ix.Init(R.Config(max_geometries=args.boxes + 1_111_000))
+    ix.Insert(boxes)
+    ix.Build()
+    cnt = R.SharedValue()
+
+    def query():
+        cnt.tensor.zero_()
+        ix.Query(R.Predicate.kContains, pts, cnt)
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b)
+
+    query()
+    rounds = []
+    off = 0
+    for batch in (1_000, 10_000, 100_000, 1_000_000):
+        part = extra[off:off + batch]
+        off += batch
+        t_ins = timed(lambda: ix.Insert(part))
+        t_first = timed(query)   # rebuild + query
+        t_second = timed(query)  # query alone
+        rounds.append({"batch": batch, "boxes_after": ix.size(), "insert_ms": round(t_ins, 3),
+                       "first_query_ms": round(t_first, 3), "query_ms": round(t_second, 3),
+                       "rebuild_ms": round(t_first - t_second, 3),
+                       "results": cnt.get()})
+        log(json.dumps(rounds[-1]))
+    ix.Sync()
+    last = rounds[-1]
+    return {
+        "metric": METRIC,
+        "value": round(last["batch"] / ((last["insert_ms"] + last["rebuild_ms"]) / 1e3) / 1e6, 3),
+        "unit": "Mboxes/s inserted (insert + rebuild)",
+        "n_gpus": 1, "steps": len(rounds), "warmup": 0,
+        "ms_per_step": round(last["insert_ms"] + last["first_query_ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"insert batches of 1e3..1e6 Gaussian-mixture boxes into a resident "
+                                f"{args.boxes}-box index, each followed by 100k point queries"),
+                   "path": "inserts", "boxes": args.boxes},
+        "rounds": rounds,
+        "cpu_baseline": None,
+    }
+
+
 # -------------------------------------------------------------------- c4 --
 def run_c4(args):
     import torch
@@ -788,7 +857,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["served", "c3", "c3f64", "handlers", "c4", "copy"],
+    ap.add_argument("--workload",
+                    choices=["served", "c3", "c3f64", "handlers", "inserts", "c4", "copy"],
                     default="served")
     ap.add_argument("--rounds", type=int, default=20)
     ap.add_argument("--boxes", type=int, default=10_000_000)
@@ -809,7 +879,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     out = {"served": run_served, "c3": run_c3, "c3f64": run_c3, "handlers": run_handlers,
-           "c4": run_c4,
+           "inserts": run_inserts, "c4": run_c4,
            "copy": run_copy_calibration}[args.workload](args)
     print(json.dumps(out), flush=True)
     check = out.get("bit_exact_check")

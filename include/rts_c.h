@@ -291,6 +291,17 @@ typedef struct rts_index_stats {
 
 int rts_index_get_stats(rts_index* index, rts_index_stats* out);
 
+/* Device memory the index holds, in bytes (SpatialIndex::PrintMemoryUsage,
+ * spatial_index.cuh:893-903, reports BVH / geometries / AABBs in MB). */
+typedef struct rts_memory_usage {
+  uint64_t geometries;  /* boxes by id (float; a double index adds its double boxes) */
+  uint64_t aabbs;       /* float boxes in tree order (the leaves) */
+  uint64_t bvh;         /* nodes, leaf ids, parents, build keys and scratch */
+  uint64_t total;
+} rts_memory_usage;
+
+int rts_index_memory(rts_index* index, rts_memory_usage* out);
+
 /* Autotuner. No reference counterpart: it stands where the reference's
  * CalculateBestParallelism sampling (spatial_index.cuh:41-57,690-760) picks a
  * ray parallelism. It times the query over an evenly strided sample of

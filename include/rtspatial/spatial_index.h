@@ -20,6 +20,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <iostream>
 #include <string>
 #include <type_traits>
 
@@ -239,6 +240,21 @@ class SpatialIndex<COORD_T, 2> {
                      reinterpret_cast<const typename abi::c_envelope*>(queries.data()),
                      queries.size(), handler, arg, stream),
           "SpatialIndex::Query(envelopes)");
+  }
+
+  // spatial_index.cuh:766-826 merges the per-Insert BVHs; here every box is
+  // in one tree already, and Optimize brings it up to date now.
+  void Optimize(hipStream_t stream = nullptr) { check(rts_index_build(handle(), stream), "Optimize"); }
+
+  // spatial_index.cuh:893-903
+  void PrintMemoryUsage() {
+    rts_memory_usage m;
+    check(rts_index_memory(handle(), &m), "PrintMemoryUsage");
+    const uint64_t mb = 1024 * 1024;
+    std::cout << "BVH " << m.bvh / mb << " MB, "
+              << "Geometries " << m.geometries / mb << " MB, "
+              << "AABBs " << m.aabbs / mb << " MB, "
+              << "Total " << m.total / mb << " MB" << std::endl;
   }
 
   size_t size() const { return ix_ ? rts_index_size(ix_) : 0; }

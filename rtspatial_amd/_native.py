@@ -66,6 +66,15 @@ class rts_query_plan(ctypes.Structure):
     ]
 
 
+class rts_memory_usage(ctypes.Structure):
+    _fields_ = [
+        ("geometries", ctypes.c_uint64),
+        ("aabbs", ctypes.c_uint64),
+        ("bvh", ctypes.c_uint64),
+        ("total", ctypes.c_uint64),
+    ]
+
+
 class rts_dev_queue(ctypes.Structure):
     _fields_ = [
         ("data", ctypes.c_void_p),
@@ -198,6 +207,7 @@ SIGNATURES = [
     ("rts_index_build", _I32, [_VP, _VP]),
     ("rts_index_sync", _I32, [_VP, _VP]),
     ("rts_index_get_stats", _I32, [_VP, ctypes.POINTER(rts_index_stats)]),
+    ("rts_index_memory", _I32, [_VP, ctypes.POINTER(rts_memory_usage)]),
     ("rts_sort_pairs", _I32, [_VP, _VP, _VP, _VP, _U64, _I32, _VP]),
     ("rts_tune_config_default", None, [ctypes.POINTER(rts_tune_config)]),
     ("rts_index_autotune", _I32, [_VP, _I32, _VP, _U64, _I32, _I32, ctypes.POINTER(rts_tune_config),

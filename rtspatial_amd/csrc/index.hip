@@ -845,6 +845,21 @@ int rts_index_get_stats(rts_index* ix, rts_index_stats* out) {
   return RTS_OK;
 }
 
+int rts_index_memory(rts_index* ix, rts_memory_usage* out) {
+  RTS_CHECK_ARG(ix && out, "rts_index_memory: null argument");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  const uint64_t cap = ix->cap, t = ix->tcap;
+  out->geometries = cap * (sizeof(float4) + sizeof(uint32_t)) + (ix->boxes64 ? cap * sizeof(Box64) : 0);
+  out->aabbs = t * sizeof(float4);
+  out->bvh = t ? (t * sizeof(uint32_t) * 4                      // leaf ids, keys, keys/vals alt
+                  + t * sizeof(NodeRec) + 2 * t * sizeof(uint32_t) // nodes, parents
+                  + (t + refit_boundary_words(t)) * sizeof(uint32_t) // flags + boundary list
+                  + sort_scratch_bytes(t) + ix->n_partials_cap * sizeof(float4))
+               : 0;
+  out->total = out->geometries + out->aabbs + out->bvh + 256;
+  return RTS_OK;
+}
+
 int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                    uint32_t* vals_out, uint64_t n, int key_bits, rts_stream_t stream) {
   RTS_CHECK_ARG(keys_out && vals_out && (keys_in || n == 0), "rts_sort_pairs: null pointer");

@@ -303,6 +303,25 @@ class SpatialIndex:
                                            _stream_handle(stream)), "Autotune")
         return {f: getattr(r, f) for f, _ in N.rts_tune_result._fields_}
 
+    def Optimize(self, stream=None):
+        """spatial_index.cuh:766-826 merges the per-Insert BVHs; here every
+        inserted box already lives in one tree, and Optimize brings that tree
+        up to date now (a pending build or refit runs on `stream`)."""
+        self.Build(stream)
+
+    def memory(self):
+        """Device bytes held: geometries, aabbs (leaf boxes), bvh, total."""
+        m = N.rts_memory_usage()
+        N.check(N.lib().rts_index_memory(self._handle(), ctypes.byref(m)), "memory")
+        return {f: getattr(m, f) for f, _ in N.rts_memory_usage._fields_}
+
+    def PrintMemoryUsage(self):
+        """spatial_index.cuh:893-903: one line, in MB."""
+        m = self.memory()
+        mb = {k: v // (1024 * 1024) for k, v in m.items()}
+        print(f"BVH {mb['bvh']} MB, Geometries {mb['geometries']} MB, AABBs {mb['aabbs']} MB, "
+              f"Total {mb['total']} MB", flush=True)
+
     def stats(self):
         s = N.rts_index_stats()
         N.check(N.lib().rts_index_get_stats(self._handle(), ctypes.byref(s)), "stats")

@@ -121,6 +121,11 @@ TEST(EnvelopeQueries, fp32_intersects_envelope_batch) {
   index.Init(Config());
   insert_batches(index, boxes, 10, s);
   ASSERT_EQ(count(index, Predicate::kIntersects, queries, s), 6549770ull);
+  // Optimize (spatial_index.cuh:766-826) changes no result; PrintMemoryUsage
+  // (:893-903) reports the index's memory.
+  index.Optimize(s.cuda_stream());
+  ASSERT_EQ(count(index, Predicate::kIntersects, queries, s), 6549770ull);
+  index.PrintMemoryUsage();
 }
 
 // test/test_envelope_queries.h:93-164: refit after Update == a fresh build

@@ -65,6 +65,7 @@ int main(void) {
   S(rts_point_d) S(rts_envelope_d) S(rts_update_d) F(rts_update_d, box)
   S(rts_update) S(rts_tune_result) S(rts_serve_config) S(rts_request) S(rts_request_result)
   S(rts_tree_view) F(rts_tree_view, n) F(rts_tree_view, coord) S(rts_query_plan)
+  S(rts_memory_usage)
   return 0;
 }
 """
@@ -91,7 +92,7 @@ def test_ctypes_structs_match_the_header(tmp_path):
     assert int(got["rts_tree_view.n"]) == N.rts_tree_view.n.offset == 56
     assert int(got["rts_tree_view.coord"]) == N.rts_tree_view.coord.offset
     for name in ("rts_tune_result", "rts_serve_config", "rts_request", "rts_request_result",
-                 "rts_tree_view", "rts_query_plan"):
+                 "rts_tree_view", "rts_query_plan", "rts_memory_usage"):
         assert int(got[name]) == ctypes.sizeof(getattr(N, name)), name
 
 
