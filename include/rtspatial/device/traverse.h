@@ -238,9 +238,8 @@ struct WaveLds {
   uint32_t stack[kWaveStack];
   uint32_t rfirst[kWaveRanges];
   uint32_t rpre[kWaveRanges + 1];
-  // Scan step map, position p = leaf base + p: first the (entry + 1) of the
-  // ranges that start at p (0 elsewhere), then, for every p, the leaf index
-  // minus the flat index, with kAllMatch of its range.
+  // Scan step map, position p = leaf base + p: the (entry + 1) of the range
+  // that starts at p, 0 elsewhere.
   uint32_t map[kScanStep];
 };
 
@@ -366,10 +365,10 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
       uint32_t* map = L.map;
       for (uint32_t base = 0; base < total; base += kScanStep) {
         const uint32_t lim = total - base < kScanStep ? total - base : kScanStep;
-        // (1) clear this lane's 8 map words, then mark the ranges that start
-        // inside the step (entries after e0, in rpre order).
+        // (1) clear the step's map, then mark the ranges that start inside
+        // the step (entries after e0, in rpre order) with their entry + 1.
 #pragma unroll
-        for (int k = 0; k < kPerLane; k++) map[lane * kPerLane + k] = 0u;
+        for (int k = 0; k < kPerLane; k++) map[k * kWave + lane] = 0u;
         lds_fence();
         for (uint32_t e = e0 + 1 + lane;; e += kWave) {
           const bool act = e < m && rpre[e] < base + lim;
@@ -377,32 +376,23 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
           if (!__any(act)) break;
         }
         lds_fence();
-        // (2) owner (entry + 1) of positions L*8 .. L*8+7: running maximum
-        // within the lane, then across lanes, seeded with e0 + 1.
-        uint32_t own[kPerLane];
-        uint32_t run = 0;
+        // (2) owner (entry + 1) of position k*64 + L: the largest mark at or
+        // before it, seeded with e0 + 1 -- a max-scan of row k, carried from
+        // row to row -- and from it the position's leaf offset.
+        uint32_t carry = e0 + 1;  // wave-uniform
+        uint32_t e_last = e0;     // wave-uniform: entry of the step's last position
+        uint32_t dl[kPerLane];
 #pragma unroll
         for (int k = 0; k < kPerLane; k++) {
-          const uint32_t v = map[lane * kPerLane + k];
-          run = v > run ? v : run;
-          own[k] = run;
-        }
-        uint32_t carry = wave_inclusive_max(run);
-        carry = __shfl_up(carry, 1, 64);
-        carry = lane == 0 ? e0 + 1 : (carry > e0 + 1 ? carry : e0 + 1);
-        uint32_t last = 0;
-#pragma unroll
-        for (int k = 0; k < kPerLane; k++) {
-          const uint32_t e = (own[k] > carry ? own[k] : carry) - 1;
-          last = e;
+          uint32_t run = wave_inclusive_max(map[k * kWave + lane]);
+          run = run > carry ? run : carry;
+          carry = readlane_u32(run, kWave - 1);
+          if ((uint32_t)k == (lim - 1) / kWave) e_last = readlane_u32(run, (lim - 1) % kWave) - 1;
+          const uint32_t e = run - 1;
           const uint32_t f = rfirst[e];
-          own[k] = ((f & ~kAllMatch) - rpre[e] + kDeltaBias) | (f & kAllMatch);
+          dl[k] = ((f & ~kAllMatch) - rpre[e] + kDeltaBias) | (f & kAllMatch);
         }
-        e0 = readlane_u32(last, (lim - 1) / kPerLane);  // entry of the step's last leaf
-        lds_fence();
-#pragma unroll
-        for (int k = 0; k < kPerLane; k++) map[lane * kPerLane + k] = own[k];
-        lds_fence();
+        e0 = e_last;
         // (3) leaves of positions k*64 + L: ids loaded together with the boxes
         // that need a test, so a step waits for one memory round trip.
         uint32_t leaf[kPerLane];
@@ -411,7 +401,7 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
 #pragma unroll
         for (int k = 0; k < kPerLane; k++) {
           const uint32_t pos = k * kWave + lane;
-          const uint32_t d = map[pos < kScanStep ? pos : 0];
+          const uint32_t d = dl[k];
           leaf[k] = base + pos + (d & ~kAllMatch) - kDeltaBias;
           const bool ok = pos < lim && leaf[k] < t.n;
           valid |= (ok ? 1u : 0u) << k;
