@@ -220,7 +220,7 @@ __device__ __forceinline__ void load_q(const void* queries, bool points, uint32_
 // in an LDS list. Once the list holds enough leaves the wave scans it, 512
 // leaves per step, and writes the matches of this query contiguously from
 // slot `obase`.
-constexpr int kWaveStack = 512;     // LDS stack entries per wave
+constexpr int kWaveStack = 256;     // LDS stack entries per wave
 constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
 constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
 constexpr int kPerLane = 8;         // leaves per lane per scan step
@@ -239,8 +239,8 @@ struct WaveLds {
   uint32_t rfirst[kWaveRanges];
   uint32_t rpre[kWaveRanges + 1];
   // Scan step map, position p = leaf base + p: the (entry + 1) of the range
-  // that starts at p, 0 elsewhere.
-  uint32_t map[kScanStep];
+  // that starts at p, 0 elsewhere (at most kWaveRanges + 1, so 16 bits).
+  uint16_t map[kScanStep];
 };
 
 // Returns the query's number of results (wave-uniform). With WRITE every
@@ -362,17 +362,17 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
       // starts are marked, a prefix maximum gives each position its range,
       // and the map then holds each position's leaf offset.
       uint32_t e0 = 0;  // wave-uniform: entry holding flat index `base`
-      uint32_t* map = L.map;
+      uint16_t* map = L.map;
       for (uint32_t base = 0; base < total; base += kScanStep) {
         const uint32_t lim = total - base < kScanStep ? total - base : kScanStep;
         // (1) clear the step's map, then mark the ranges that start inside
         // the step (entries after e0, in rpre order) with their entry + 1.
 #pragma unroll
-        for (int k = 0; k < kPerLane; k++) map[k * kWave + lane] = 0u;
+        for (int k = 0; k < kPerLane; k++) map[k * kWave + lane] = 0;
         lds_fence();
         for (uint32_t e = e0 + 1 + lane;; e += kWave) {
           const bool act = e < m && rpre[e] < base + lim;
-          if (act) map[rpre[e] - base] = e + 1;
+          if (act) map[rpre[e] - base] = (uint16_t)(e + 1);
           if (!__any(act)) break;
         }
         lds_fence();
@@ -384,7 +384,7 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
         uint32_t dl[kPerLane];
 #pragma unroll
         for (int k = 0; k < kPerLane; k++) {
-          uint32_t run = wave_inclusive_max(map[k * kWave + lane]);
+          uint32_t run = wave_inclusive_max((uint32_t)map[k * kWave + lane]);
           run = run > carry ? run : carry;
           carry = readlane_u32(run, kWave - 1);
           if ((uint32_t)k == (lim - 1) / kWave) e_last = readlane_u32(run, (lim - 1) % kWave) - 1;

@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kThreads) k_batch(const TreeView* tv, const Ba
   TreeView t = t0;  // leaf_size may differ per request
   const uint32_t w = threadIdx.x / kWave;
   const BatchHeader h = *a.header;
-  const uint32_t lo = CLS == 0 ? 0u : h.class_end[CLS - 1];
+  const uint32_t lo = CLS == 0 ? 0u : h.class_end[CLS > 0 ? CLS - 1 : 0];
   const uint32_t hi = h.class_end[CLS];
   if (lo >= hi || h.n_requests == 0 || t.n == 0) return;
   uint32_t* counters = a.tile_counters + ((WRITE ? kClasses : 0) + CLS) * kXcds;
