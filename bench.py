@@ -667,9 +667,11 @@ def run_handlers(args):
 # --------------------------------------------------------------- inserts --
 def run_inserts(args):
     """Insert batches of growing size into the resident 10M-box index, each
-    followed by a query batch: the first query after an Insert pays the
-    rebuild, the second does not. The reference builds one GAS per Insert
-    under an IAS instead (spatial_index.cuh:142-196)."""
+    followed by a query batch: the first query after an Insert pays the build
+    (of the small delta tree while the inserts stay within insert_delta_pct of
+    the main tree, else of the whole tree), the second does not. The
+    reference builds one GAS per Insert under an IAS instead
+    (spatial_index.cuh:142-196)."""
     import torch
 
     import rtspatial_amd as R
@@ -716,13 +718,14 @@ def run_inserts(args):
                        "results": cnt.get()})
         log(json.dumps(rounds[-1]))
     ix.Sync()
-    last = rounds[-1]
+    ins = sum(r["batch"] for r in rounds)
+    ms = sum(r["insert_ms"] + max(r["rebuild_ms"], 0.0) for r in rounds)
     return {
         "metric": METRIC,
-        "value": round(last["batch"] / ((last["insert_ms"] + last["rebuild_ms"]) / 1e3) / 1e6, 3),
-        "unit": "Mboxes/s inserted (insert + rebuild)",
+        "value": round(ins / (ms / 1e3) / 1e6, 3),
+        "unit": "Mboxes/s inserted (insert + build, all batches)",
         "n_gpus": 1, "steps": len(rounds), "warmup": 0,
-        "ms_per_step": round(last["insert_ms"] + last["first_query_ms"], 3),
+        "ms_per_step": round(ms / len(rounds), 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": (f"insert batches of 1e3..1e6 Gaussian-mixture boxes into a resident "

@@ -168,6 +168,11 @@ typedef struct rts_config {
   uint32_t sort_queries;         /* 0 = auto, 1 = Morton-order queries, 2 = keep input order */
   uint32_t coord;                /* rts_coord (RTS_COORD_F32). A float index takes the rts_index_*
                                     calls below, a double one the rts_index_*_d calls. */
+  uint32_t insert_delta_pct;     /* boxes inserted since the last full build go to a second,
+                                    small tree (built alone, queried with the main one) while they
+                                    are at most this percentage of the main tree (2); 0: every
+                                    Insert rebuilds the whole tree. RTS_REBUILD_ALWAYS always
+                                    rebuilds. */
 } rts_config;
 
 typedef struct rts_index rts_index;
@@ -227,10 +232,11 @@ int rts_index_query_envelopes_d(rts_index* index, rts_predicate pred,
  * query shaders around a handler header of rtspatial_handle_* functions
  * (default_handlers/counting_handlers.h, test/CMakeLists.txt:8); here the
  * caller's HIP code instantiates the traversal kernels with its handler and
- * runs them on this view. rts_index_tree_view brings the tree up to date on `stream` (a pending
- * build or refit is queued there) and fills `out`; the view stays valid until
- * the next call that changes the index (insert, delete, update, clear, build).
- * All pointers are device pointers. */
+ * runs them on this view. rts_index_tree_view brings the tree up to date on
+ * `stream` (a pending build or refit is queued there, and a small tree of
+ * recent inserts is merged into the main one) and fills `out`; the view stays
+ * valid until the next call that changes the index (insert, delete, update,
+ * clear, build). All pointers are device pointers. */
 typedef struct rts_tree_view {
   const void* leaf_boxes;    /* float (x0, y0, x1, y1) per leaf, in tree order */
   const uint32_t* leaf_ids;  /* box id per leaf */
@@ -286,7 +292,8 @@ typedef struct rts_index_stats {
   uint64_t refits;         /* refits after Update/Delete */
   double sah_at_build;     /* SAH-style cost right after the last build */
   double sah_current;      /* the last measured cost (after the last refit) */
-  int32_t stale;           /* 1 if the next query will rebuild */
+  int32_t stale;           /* 1 if the next query will build (the whole tree or the small one) */
+  uint32_t delta_boxes;    /* boxes in the small tree of recent inserts (0: one tree) */
 } rts_index_stats;
 
 int rts_index_get_stats(rts_index* index, rts_index_stats* out);

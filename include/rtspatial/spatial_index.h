@@ -64,6 +64,9 @@ struct Config {
   // RTS_COORD_F64_AS_F32 (coordinates rounded to nearest floats: the
   // reference's results, whose double points return float coordinates).
   uint32_t coord = RTS_COORD_F64;
+  // Inserts go to a small second tree while they are at most this percentage
+  // of the main tree (0: every Insert rebuilds the whole tree).
+  uint32_t insert_delta_pct = 2;
 };
 
 namespace detail {
@@ -163,6 +166,7 @@ class SpatialIndex<COORD_T, 2> {
     c.traversal = config.traversal;
     c.sort_queries = config.sort_queries;
     c.coord = abi::coord(config.coord);
+    c.insert_delta_pct = config.insert_delta_pct;
     check(rts_index_create(&c, &ix_), "SpatialIndex::Init");
   }
 

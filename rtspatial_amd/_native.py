@@ -38,6 +38,7 @@ class rts_config(ctypes.Structure):
         ("traversal", ctypes.c_uint32),
         ("sort_queries", ctypes.c_uint32),
         ("coord", ctypes.c_uint32),
+        ("insert_delta_pct", ctypes.c_uint32),
     ]
 
 
@@ -114,6 +115,7 @@ class rts_index_stats(ctypes.Structure):
         ("sah_at_build", ctypes.c_double),
         ("sah_current", ctypes.c_double),
         ("stale", ctypes.c_int32),
+        ("delta_boxes", ctypes.c_uint32),
     ]
 
 

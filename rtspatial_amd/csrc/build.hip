@@ -102,14 +102,16 @@ __global__ void __launch_bounds__(kThreads) k_morton_points(const float2* __rest
 
 // ------------------------------------------------------------ gather ----
 __global__ void __launch_bounds__(kThreads) k_gather(const float4* __restrict__ boxes,
-                                                     const uint32_t* __restrict__ ids, uint32_t n,
-                                                     float4* __restrict__ leaf_boxes,
+                                                     uint32_t* __restrict__ ids, uint32_t n,
+                                                     uint32_t off, float4* __restrict__ leaf_boxes,
                                                      uint32_t* __restrict__ leaf_of_id) {
   for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < n; k += gridDim.x * kThreads) {
-    uint32_t id = ids[k];
-    if (id >= n) continue;  // unreachable for a correct sort; keeps reads in bounds
+    const uint32_t local = ids[k];
+    if (local >= n) continue;  // unreachable for a correct sort; keeps reads in bounds
+    const uint32_t id = local + off;
+    if (off) ids[k] = id;
     leaf_boxes[k] = boxes[id];
-    leaf_of_id[id] = k;
+    leaf_of_id[id] = off + k;
   }
 }
 
@@ -377,11 +379,11 @@ hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* sce
   return hipGetLastError();
 }
 
-hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids, uint32_t n,
+hipError_t launch_gather_leaves(const float4* boxes, uint32_t* sorted_ids, uint32_t n, uint32_t off,
                                 float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kThreads), 0, stream, boxes, sorted_ids, n,
-                     leaf_boxes, leaf_of_id);
+                     off, leaf_boxes, leaf_of_id);
   return hipGetLastError();
 }
 

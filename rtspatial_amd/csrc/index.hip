@@ -3,10 +3,13 @@
 // (reference include/rtspatial/spatial_index.cuh:62-931) with the OptiX
 // GAS/IAS machinery replaced by one LBVH over every inserted box.
 //
-// Build policy: Insert appends boxes by id and marks the tree stale; the next
-// query (or rts_index_build) rebuilds it once, however many Insert calls came
-// before. Update and Delete write the boxes in place and mark the tree for a
-// refit, which runs once at the next query (or rts_index_build); under
+// Build policy: Insert appends boxes by id; the next query (or
+// rts_index_build) brings them into a tree once, however many Insert calls
+// came before: while they are few next to the main tree (insert_delta_pct),
+// a small delta tree is built over them alone and queries walk both trees;
+// otherwise the whole tree is rebuilt. Update and Delete write the boxes in
+// place and mark the trees for a refit, which runs once at the next query
+// (or rts_index_build); under
 // RTS_REBUILD_SAH a refit whose tree cost exceeds sah_rebuild_ratio times the
 // cost measured right after the last build marks the tree stale instead of
 // letting it degrade (the reference never rebuilds, spatial_index.cuh:208-336).
@@ -94,7 +97,8 @@ int ensure_id_capacity(rts_index* ix, uint64_t need, hipStream_t stream) {
 int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   if (n <= ix->tcap) return RTS_OK;
   RTS_HIP(hipDeviceSynchronize());
-  uint64_t cap = std::max<uint64_t>(n, ix->tcap + ix->tcap / 2);
+  // Room for a delta tree of up to insert_delta_pct percent on top of n.
+  uint64_t cap = std::max<uint64_t>(n + n * ix->cfg.insert_delta_pct / 100, ix->tcap + ix->tcap / 2);
   cap = std::max<uint64_t>(cap, 1024);
   dev_free(ix->leaf_boxes);
   dev_free(ix->leaf_ids);
@@ -140,13 +144,13 @@ void poll_cost(rts_index* ix) {
 }
 
 int measure_cost(rts_index* ix, bool after_build, hipStream_t stream) {
-  if (ix->n_tree == 0) return RTS_OK;
+  if (ix->n_main == 0) return RTS_OK;
   if (after_build) {
     if (ix->cost_build_pending) RTS_HIP(hipEventSynchronize(ix->ev_cost_build));
   } else if (ix->cost_refit_pending || ix->cost_build_pending) {
     return RTS_OK;  // a measurement is still in flight; skip this one
   }
-  RTS_HIP(launch_tree_cost(ix->nodes, ix->root_box, (uint32_t)ix->n_tree, ix->cost, stream));
+  RTS_HIP(launch_tree_cost(ix->nodes, ix->root_box, (uint32_t)ix->n_main, ix->cost, stream));
   double* dst = after_build ? ix->h_cost : ix->h_cost + 2;
   RTS_HIP(hipMemcpyAsync(dst, ix->cost, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
   if (after_build) {
@@ -160,14 +164,18 @@ int measure_cost(rts_index* ix, bool after_build, hipStream_t stream) {
   return RTS_OK;
 }
 
-RefitArgs refit_args(rts_index* ix, uint32_t n) {
+// Refit of the main tree, or (delta) of the delta tree, whose leaves, nodes
+// and parents start at n_main, n_main and 2 n_main. Both share the flag and
+// boundary words; refits run one after the other on a stream.
+RefitArgs refit_args(rts_index* ix, bool delta) {
+  const uint64_t off = delta ? ix->n_main : 0;
   RefitArgs a;
-  a.leaf_boxes = ix->leaf_boxes;
-  a.n = n;
-  a.parent = ix->parent;
-  a.nodes = ix->nodes;
-  a.root_box = ix->root_box;
-  a.root_live = ix->root_live;
+  a.leaf_boxes = ix->leaf_boxes + off;
+  a.n = (uint32_t)(delta ? ix->n_delta : ix->n_main);
+  a.parent = ix->parent + 2 * off;
+  a.nodes = ix->nodes + off;
+  a.root_box = delta ? ix->delta_root_box : ix->root_box;
+  a.root_live = delta ? ix->delta_root_live : ix->root_live;
   a.flags = ix->flags;
   a.error = ix->error;
   a.boundary = ix->flags + ix->tcap;  // after the n - 1 node flags
@@ -199,9 +207,12 @@ int build_tree(rts_index* ix, hipStream_t stream) {
     std::swap(ix->leaf_ids, ix->vals_alt);
   }
   ph = phase_begin(RTS_PHASE_BUILD_TREE, stream);
-  RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids, n, ix->leaf_boxes, ix->leaf_of_id, stream));
+  RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids, n, 0, ix->leaf_boxes, ix->leaf_of_id,
+                               stream));
   RTS_HIP(launch_karras(ix->keys, n, ix->nodes, ix->parent, ix->flags, stream));
-  RTS_HIP(launch_refit(refit_args(ix, n), stream));
+  ix->n_main = N;
+  ix->n_delta = 0;
+  RTS_HIP(launch_refit(refit_args(ix, false), stream));
   phase_end(ph, stream);
   ix->n_tree = N;
   ix->stale = false;
@@ -210,6 +221,51 @@ int build_tree(rts_index* ix, hipStream_t stream) {
   ix->build_epoch++;
   if (ix->cfg.rebuild_policy == RTS_REBUILD_SAH) return measure_cost(ix, true, stream);
   return RTS_OK;
+}
+
+// Builds the delta tree over ids [n_main, n) alone: its own scene bounds,
+// Morton sort and hierarchy, in the tails of the shared arrays. The main tree
+// is left as it is (refitted first if a mutation left it pending).
+int build_delta(rts_index* ix, hipStream_t stream) {
+  const uint64_t off = ix->n_main;
+  const uint32_t nd = (uint32_t)(ix->n - off);
+  const uint32_t parts = scene_bounds_partials(nd);
+  PhaseToken ph = phase_begin(RTS_PHASE_BUILD_MORTON, stream);
+  RTS_HIP(launch_scene_bounds(ix->boxes + off, nd, ix->partials, parts, ix->delta_scene, stream));
+  RTS_HIP(launch_morton_boxes(ix->boxes + off, nd, ix->delta_scene, ix->keys + off, stream));
+  phase_end(ph, stream);
+  bool in_b = false;
+  ph = phase_begin(RTS_PHASE_BUILD_SORT, stream);
+  RTS_HIP(radix_sort_pairs(ix->keys + off, ix->leaf_ids + off, ix->keys_alt + off,
+                           ix->vals_alt + off, nd, kMortonBits, true, ix->ss, stream, &in_b));
+  phase_end(ph, stream);
+  if (in_b) {  // the main tree's arrays stay where they are: copy the tail back
+    RTS_HIP(hipMemcpyAsync(ix->keys + off, ix->keys_alt + off, nd * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, stream));
+    RTS_HIP(hipMemcpyAsync(ix->leaf_ids + off, ix->vals_alt + off, nd * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, stream));
+  }
+  ph = phase_begin(RTS_PHASE_BUILD_TREE, stream);
+  RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids + off, nd, (uint32_t)off,
+                               ix->leaf_boxes + off, ix->leaf_of_id, stream));
+  RTS_HIP(launch_karras(ix->keys + off, nd, ix->nodes + off, ix->parent + 2 * off, ix->flags,
+                        stream));
+  ix->n_delta = nd;
+  RTS_HIP(launch_refit(refit_args(ix, true), stream));
+  phase_end(ph, stream);
+  ix->n_tree = ix->n;
+  ix->delta_builds++;
+  return RTS_OK;
+}
+
+// Whether the ids inserted since the last full build go to the delta tree:
+// few enough of them, a tree to add them to, and room in the arrays (growing
+// the arrays rebuilds anyway).
+bool delta_ok(const rts_index* ix) {
+  const uint64_t extra = ix->n - ix->n_main;
+  return ix->cfg.insert_delta_pct > 0 && ix->cfg.rebuild_policy != RTS_REBUILD_ALWAYS &&
+         ix->n_main > 0 && ix->n <= ix->tcap &&
+         extra * 100 <= ix->n_main * (uint64_t)ix->cfg.insert_delta_pct;
 }
 
 // Update and Delete only mark the tree: the refit runs once, at the next
@@ -228,7 +284,8 @@ int after_mutation(rts_index* ix) {
 int refit_tree(rts_index* ix, hipStream_t stream) {
   ix->refit_needed = false;
   PhaseToken ph = phase_begin(RTS_PHASE_REFIT, stream);
-  RTS_HIP(launch_refit(refit_args(ix, (uint32_t)ix->n_tree), stream));
+  RTS_HIP(launch_refit(refit_args(ix, false), stream));
+  if (ix->n_delta) RTS_HIP(launch_refit(refit_args(ix, true), stream));
   phase_end(ph, stream);
   ix->refits++;
   if (ix->cfg.rebuild_policy == RTS_REBUILD_SAH) return measure_cost(ix, false, stream);
@@ -263,19 +320,38 @@ TreeView tree_view(const rts_index* ix) {
   t.nodes = ix->nodes;
   t.root_box = ix->root_box;
   t.root_live = ix->root_live;
-  t.n = (uint32_t)ix->n_tree;
+  t.n = (uint32_t)ix->n_main;
   t.leaf_size = ix->cfg.leaf_size;
   t.error = ix->error;
   t.boxes64 = ix->cfg.coord == RTS_COORD_F64 ? ix->boxes64 : nullptr;
   return t;
 }
 
+TreeView delta_view(const rts_index* ix) {
+  TreeView t = tree_view(ix);
+  const uint64_t off = ix->n_main;
+  t.leaf_boxes = ix->leaf_boxes + off;
+  t.leaf_ids = ix->leaf_ids + off;
+  t.nodes = ix->nodes + off;
+  t.root_box = ix->delta_root_box;
+  t.root_live = ix->delta_root_live;
+  t.n = (uint32_t)ix->n_delta;
+  return t;
+}
+
 int device_error_status(rts_index* ix) { return check_device_error(ix); }
 
-int ensure_built(rts_index* ix, hipStream_t stream) {
+int ensure_built(rts_index* ix, hipStream_t stream, bool single) {
   poll_cost(ix);
   if (ix->stale) return build_tree(ix, stream);
-  if (ix->refit_needed) return refit_tree(ix, stream);
+  const bool inserted = ix->n > ix->n_tree;
+  if ((inserted && !delta_ok(ix)) || (single && (ix->n_delta || inserted)))
+    return build_tree(ix, stream);  // a refit pending now is subsumed
+  if (ix->refit_needed) {
+    int rc = refit_tree(ix, stream);
+    if (rc) return rc;
+  }
+  if (inserted) return build_delta(ix, stream);
   return RTS_OK;
 }
 
@@ -301,6 +377,7 @@ void rts_config_default(rts_config* c) {
   c->traversal = RTS_TRAVERSAL_AUTO;
   c->sort_queries = 0;
   c->coord = RTS_COORD_F32;
+  c->insert_delta_pct = 2;
 }
 
 const char* rts_last_error(void) { return g_last_error.c_str(); }
@@ -331,6 +408,9 @@ int rts_index_create(const rts_config* cfg, rts_index** out) {
   ix->cost = reinterpret_cast<double*>(ix->small + 32);
   ix->error = reinterpret_cast<uint32_t*>(ix->small + 64);
   ix->root_live = reinterpret_cast<uint32_t*>(ix->small + 80);
+  ix->delta_scene = reinterpret_cast<float4*>(ix->small + 96);
+  ix->delta_root_box = reinterpret_cast<float4*>(ix->small + 112);
+  ix->delta_root_live = reinterpret_cast<uint32_t*>(ix->small + 128);
   RTS_HIP(hipMemset(ix->small, 0, 256));
   // Complete before any stream (including non-blocking ones) uses the index.
   RTS_HIP(hipDeviceSynchronize());
@@ -380,6 +460,8 @@ int rts_index_clear(rts_index* ix) {
   RTS_HIP(hipDeviceSynchronize());
   ix->n = 0;
   ix->n_tree = 0;
+  ix->n_main = 0;
+  ix->n_delta = 0;
   ix->stale = true;
   ix->refit_needed = false;
   ix->cost_build_pending = ix->cost_refit_pending = false;
@@ -398,8 +480,7 @@ int rts_index_insert(rts_index* ix, const rts_envelope* boxes, uint64_t n, rts_s
   int rc = ensure_id_capacity(ix, ix->n + n, stream);
   if (rc) return rc;
   RTS_HIP(hipMemcpyAsync(ix->boxes + ix->n, boxes, n * sizeof(float4), hipMemcpyDefault, stream));
-  ix->n += n;
-  ix->stale = true;
+  ix->n += n;  // ids [n_tree, n) wait for the next build (delta or full)
   return RTS_OK;
 }
 
@@ -418,8 +499,7 @@ int rts_index_insert_d(rts_index* ix, const rts_envelope_d* boxes, uint64_t n,
   RTS_HIP(launch_round_boxes64(ix->boxes64 + ix->n, n, ix->boxes + ix->n,
                                ix->cfg.coord == RTS_COORD_F64 ? kRoundOutward : kRoundNearest,
                                stream));
-  ix->n += n;
-  ix->stale = true;
+  ix->n += n;  // ids [n_tree, n) wait for the next build (delta or full)
   return RTS_OK;
 }
 
@@ -618,8 +698,6 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
       if (rc) return rc;
     }
     QueryLaunch ql;
-    ql.tree = tree_view(ix);
-    ql.tree.leaf_size = plan.leaf_size;
     ql.queries = qp;
     ql.n = (uint32_t)m;
     ql.qid_base = (uint32_t)off;
@@ -630,25 +708,33 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
     ql.sink.mode = handler;
     ql.sink.arg = arg;
     ql.traversal = trav;
-    if (trav == RTS_TRAVERSAL_THREAD) {
-      PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
-      RTS_HIP(launch_query(ql, stream));
-      phase_end(ph, stream);
-    } else if (!two_pass) {
-      WaveArgs wa{nullptr, nullptr, nullptr, qs.tickets};
-      PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
-      RTS_HIP(launch_query_wave(ql, wa, kPhaseCountSink, stream));
-      phase_end(ph, stream);
-    } else {
-      WaveArgs wa{qs.counts, qs.offsets, qs.total + 1, qs.tickets};
-      PhaseToken ph = phase_begin(RTS_PHASE_QUERY_COUNT, stream);
-      RTS_HIP(launch_query_wave(ql, wa, kPhaseCount, stream));
-      RTS_HIP(launch_exclusive_scan(qs.counts, m, qs.offsets, qs.total, qs.scan_tmp, stream));
-      RTS_HIP(launch_reserve(arg, qs.total, qs.total + 1, stream));
-      phase_end(ph, stream);
-      ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
-      RTS_HIP(launch_query_wave(ql, wa, kPhaseWrite, stream));
-      phase_end(ph, stream);
+    // The main tree, then the delta tree of recent inserts when there is one;
+    // the second walk reuses the scratch once the first is done (one stream).
+    const TreeView trees[2] = {tree_view(ix), delta_view(ix)};
+    for (const TreeView& tv : trees) {
+      if (tv.n == 0) continue;
+      ql.tree = tv;
+      ql.tree.leaf_size = plan.leaf_size;
+      if (trav == RTS_TRAVERSAL_THREAD) {
+        PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
+        RTS_HIP(launch_query(ql, stream));
+        phase_end(ph, stream);
+      } else if (!two_pass) {
+        WaveArgs wa{nullptr, nullptr, nullptr, qs.tickets};
+        PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
+        RTS_HIP(launch_query_wave(ql, wa, kPhaseCountSink, stream));
+        phase_end(ph, stream);
+      } else {
+        WaveArgs wa{qs.counts, qs.offsets, qs.total + 1, qs.tickets};
+        PhaseToken ph = phase_begin(RTS_PHASE_QUERY_COUNT, stream);
+        RTS_HIP(launch_query_wave(ql, wa, kPhaseCount, stream));
+        RTS_HIP(launch_exclusive_scan(qs.counts, m, qs.offsets, qs.total, qs.scan_tmp, stream));
+        RTS_HIP(launch_reserve(arg, qs.total, qs.total + 1, stream));
+        phase_end(ph, stream);
+        ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
+        RTS_HIP(launch_query_wave(ql, wa, kPhaseWrite, stream));
+        phase_end(ph, stream);
+      }
     }
     if (qs.block) RTS_HIP(hipFreeAsync(qs.block, stream));
   }
@@ -801,7 +887,7 @@ int rts_index_tree_view(rts_index* ix, rts_stream_t stream, rts_tree_view* out) 
   out->coord = ix->cfg.coord;
   out->leaf_size = ix->cfg.leaf_size;
   if (ix->n == 0) return RTS_OK;
-  int rc = ensure_built(ix, stream);
+  int rc = ensure_built(ix, stream, true);  // one tree: a delta tree is merged first
   if (rc) return rc;
   const TreeView t = tree_view(ix);
   out->leaf_boxes = t.leaf_boxes;
@@ -837,11 +923,12 @@ int rts_index_get_stats(rts_index* ix, rts_index_stats* out) {
   poll_cost(ix);
   out->n_boxes = ix->n;
   out->n_tree = ix->n_tree;
+  out->delta_boxes = (uint32_t)ix->n_delta;
   out->builds = ix->builds;
   out->refits = ix->refits;
   out->sah_at_build = ix->sah_at_build;
   out->sah_current = ix->sah_current;
-  out->stale = ix->stale ? 1 : 0;
+  out->stale = (ix->stale || ix->n > ix->n_tree) ? 1 : 0;
   return RTS_OK;
 }
 

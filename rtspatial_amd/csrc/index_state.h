@@ -21,9 +21,16 @@ struct rts_index {
   rts::Box64* boxes64 = nullptr;  // a double index: the boxes as given
   uint32_t* leaf_of_id = nullptr;
 
-  // Tree (valid for n_tree leaves unless stale).
+  // Trees (valid for n_tree leaves unless stale). The main tree holds ids
+  // [0, n_main) at sorted positions [0, n_main); a small delta tree of boxes
+  // inserted since the last full build holds ids [n_main, n_tree) at
+  // positions [n_main, n_tree) of the same leaf arrays, its nodes at
+  // nodes[n_main ...] and its parents at parent[2 n_main ...]. Ids in
+  // [n_tree, n) are inserted but in no tree yet.
   uint64_t tcap = 0;
   uint64_t n_tree = 0;
+  uint64_t n_main = 0;
+  uint64_t n_delta = 0;
   bool stale = true;
   bool refit_needed = false;  // boxes changed in place since the last refit/build
   float4* leaf_boxes = nullptr;
@@ -44,6 +51,9 @@ struct rts_index {
   float4* scene = nullptr;
   float4* root_box = nullptr;
   uint32_t* root_live = nullptr;
+  float4* delta_scene = nullptr;
+  float4* delta_root_box = nullptr;
+  uint32_t* delta_root_live = nullptr;
   double* cost = nullptr;
   uint32_t* error = nullptr;  // device error bits (kErr*)
 
@@ -60,6 +70,7 @@ struct rts_index {
 
   uint64_t builds = 0;
   uint64_t refits = 0;
+  uint64_t delta_builds = 0;
 
   // Plans stored by rts_index_autotune, per [predicate][points]; kept until
   // the next tuning of that kind (Clear keeps them: they describe the
@@ -77,8 +88,13 @@ struct rts_index {
 };
 
 namespace rts {
+// The main tree, and the delta tree (n = 0 when there is none).
 TreeView tree_view(const rts_index* ix);
-int ensure_built(rts_index* ix, hipStream_t stream);
+TreeView delta_view(const rts_index* ix);
+// Brings the trees up to date on `stream`: a pending full build, refit or
+// delta build runs. With `single`, a delta tree is merged by a full build,
+// for callers that walk one tree only.
+int ensure_built(rts_index* ix, hipStream_t stream, bool single = false);
 // Reads the device error word (synchronous): RTS_ERR_DEVICE with a message if
 // a kernel of this index gave up a bounded wait or walk.
 int device_error_status(rts_index* ix);

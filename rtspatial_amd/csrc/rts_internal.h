@@ -52,7 +52,12 @@ hipError_t launch_morton_boxes(const float4* boxes, uint32_t n, const float4* sc
                                uint32_t* keys, hipStream_t stream);
 hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* scene,
                                 uint32_t* keys, hipStream_t stream);
-hipError_t launch_gather_leaves(const float4* boxes, const uint32_t* sorted_ids, uint32_t n,
+// Leaves in Morton order: for k < n, with id = sorted_ids[k] + off (sorted_ids
+// holds ids counted from `off` and is rewritten to the ids themselves):
+// leaf_boxes[k] = boxes[id], leaf_of_id[id] = off + k. off > 0 places a delta
+// tree over ids [off, off + n) at positions [off, off + n); the caller passes
+// sorted_ids and leaf_boxes already advanced by off.
+hipError_t launch_gather_leaves(const float4* boxes, uint32_t* sorted_ids, uint32_t n, uint32_t off,
                                 float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream);
 // Every stride-th element of src (elem_bytes 8, 16 or 32) into dst[0, m).
 hipError_t launch_gather_strided(const void* src, uint32_t elem_bytes, uint64_t stride, void* dst,

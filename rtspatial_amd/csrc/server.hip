@@ -483,7 +483,7 @@ int apply_update(Engine& E, const rts_update_batch& u, const std::vector<Slot>& 
   if (rc) return rc;
   {
     std::lock_guard<std::mutex> lk(E.ix->mu);
-    rc = ensure_built(E.ix, E.upd_stream);
+    rc = ensure_built(E.ix, E.upd_stream, true);
   }
   if (rc) return rc;
   SRV_HIP(hipEventRecord(E.upd_event, E.upd_stream));
@@ -810,7 +810,7 @@ int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request*
     rc = RTS_ERR_HIP;
   if (!rc) {
     std::lock_guard<std::mutex> lk(ix->mu);
-    rc = ensure_built(ix, E.upd_stream);
+    rc = ensure_built(ix, E.upd_stream, true);
   }
   if (!rc && hipStreamSynchronize(E.upd_stream) != hipSuccess) rc = RTS_ERR_HIP;
   if (!rc) rc = probe_estimates(E, E.upd_stream);

@@ -61,6 +61,9 @@ class Config:
     # Double indexes: RTS_COORD_F64 (exact, the default) or RTS_COORD_F64_AS_F32.
     # The index's dtype decides between float and double.
     coord: int = N.RTS_COORD_F64
+    # Inserts go to a small second tree while they are at most this percentage
+    # of the main tree (0: every Insert rebuilds the whole tree).
+    insert_delta_pct: int = 2
 
     def to_c(self):
         c = N.rts_config()

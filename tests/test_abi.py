@@ -64,6 +64,7 @@ int main(void) {
   S(rts_config) F(rts_config, sort_queries) F(rts_config, coord)
   S(rts_point_d) S(rts_envelope_d) S(rts_update_d) F(rts_update_d, box)
   S(rts_update) S(rts_tune_result) S(rts_serve_config) S(rts_request) S(rts_request_result)
+  S(rts_index_stats) F(rts_index_stats, delta_boxes) F(rts_config, insert_delta_pct)
   S(rts_tree_view) F(rts_tree_view, n) F(rts_tree_view, coord) S(rts_query_plan)
   S(rts_memory_usage)
   return 0;
@@ -84,12 +85,15 @@ def test_ctypes_structs_match_the_header(tmp_path):
                     str(tmp_path / "l")], check=True)
     out = subprocess.run([str(tmp_path / "l")], check=True, stdout=subprocess.PIPE, text=True).stdout
     got = dict(line.rsplit(" ", 1) for line in out.splitlines())
-    assert int(got["rts_config"]) == ctypes.sizeof(N.rts_config) == 80
+    assert int(got["rts_config"]) == ctypes.sizeof(N.rts_config) == 88
     assert int(got["rts_config.coord"]) == N.rts_config.coord.offset == 76
     assert int(got["rts_point_d"]) == 16 and int(got["rts_envelope_d"]) == 32
     assert int(got["rts_update_d"]) == 40 and int(got["rts_update_d.box"]) == 8
     assert int(got["rts_update"]) == 24
     assert int(got["rts_tree_view.n"]) == N.rts_tree_view.n.offset == 56
+    assert int(got["rts_index_stats"]) == ctypes.sizeof(N.rts_index_stats) == 56
+    assert int(got["rts_index_stats.delta_boxes"]) == N.rts_index_stats.delta_boxes.offset == 52
+    assert int(got["rts_config.insert_delta_pct"]) == N.rts_config.insert_delta_pct.offset == 80
     assert int(got["rts_tree_view.coord"]) == N.rts_tree_view.coord.offset
     for name in ("rts_tune_result", "rts_serve_config", "rts_request", "rts_request_result",
                  "rts_tree_view", "rts_query_plan", "rts_memory_usage"):
@@ -100,7 +104,7 @@ def test_coord_default_is_float():
     from rtspatial_amd import _native as N
     c = N.rts_config()
     N.lib().rts_config_default(ctypes.byref(c))
-    assert c.coord == N.RTS_COORD_F32
+    assert c.coord == N.RTS_COORD_F32 and c.insert_delta_pct == 2
 
 
 def test_tune_config_defaults_and_null_args():
