@@ -282,6 +282,11 @@ uint64_t rts_index_size(const rts_index* index);
  * inside Insert (spatial_index.cuh:161-190). */
 int rts_index_build(rts_index* index, rts_stream_t stream);
 
+/* SpatialIndex::Optimize (spatial_index.cuh:766-826 merges the per-Insert
+ * BVHs): brings the tree up to date and merges a small tree of recent inserts
+ * into the main one, so queries walk one tree. */
+int rts_index_optimize(rts_index* index, rts_stream_t stream);
+
 /* Waits for `stream` and reports any error a kernel of this index flagged. */
 int rts_index_sync(rts_index* index, rts_stream_t stream);
 

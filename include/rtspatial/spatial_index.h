@@ -246,9 +246,11 @@ class SpatialIndex<COORD_T, 2> {
           "SpatialIndex::Query(envelopes)");
   }
 
-  // spatial_index.cuh:766-826 merges the per-Insert BVHs; here every box is
-  // in one tree already, and Optimize brings it up to date now.
-  void Optimize(hipStream_t stream = nullptr) { check(rts_index_build(handle(), stream), "Optimize"); }
+  // spatial_index.cuh:766-826 merges the per-Insert BVHs; here it brings the
+  // tree up to date and merges the small tree of recent inserts into it.
+  void Optimize(hipStream_t stream = nullptr) {
+    check(rts_index_optimize(handle(), stream), "Optimize");
+  }
 
   // spatial_index.cuh:893-903
   void PrintMemoryUsage() {

@@ -207,6 +207,7 @@ SIGNATURES = [
     ("rts_index_query_plan", _I32, [_VP, _I32, _I32, _U64, ctypes.POINTER(rts_query_plan)]),
     ("rts_index_order_queries", _I32, [_VP, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_build", _I32, [_VP, _VP]),
+    ("rts_index_optimize", _I32, [_VP, _VP]),
     ("rts_index_sync", _I32, [_VP, _VP]),
     ("rts_index_get_stats", _I32, [_VP, ctypes.POINTER(rts_index_stats)]),
     ("rts_index_memory", _I32, [_VP, ctypes.POINTER(rts_memory_usage)]),

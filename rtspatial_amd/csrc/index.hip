@@ -908,6 +908,13 @@ int rts_index_build(rts_index* ix, rts_stream_t stream) {
   return ensure_built(ix, stream);
 }
 
+int rts_index_optimize(rts_index* ix, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_optimize: null index");
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  return ensure_built(ix, stream, true);
+}
+
 int rts_index_sync(rts_index* ix, rts_stream_t stream) {
   RTS_CHECK_ARG(ix, "rts_index_sync: null index");
   std::lock_guard<std::mutex> lk(ix->mu);

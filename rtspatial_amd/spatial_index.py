@@ -307,10 +307,10 @@ class SpatialIndex:
         return {f: getattr(r, f) for f, _ in N.rts_tune_result._fields_}
 
     def Optimize(self, stream=None):
-        """spatial_index.cuh:766-826 merges the per-Insert BVHs; here every
-        inserted box already lives in one tree, and Optimize brings that tree
-        up to date now (a pending build or refit runs on `stream`)."""
-        self.Build(stream)
+        """spatial_index.cuh:766-826 merges the per-Insert BVHs; here it brings
+        the tree up to date and merges the small tree of recent inserts into
+        the main one (on `stream`)."""
+        N.check(N.lib().rts_index_optimize(self._handle(), _stream_handle(stream)), "Optimize")
 
     def memory(self):
         """Device bytes held: geometries, aabbs (leaf boxes), bvh, total."""

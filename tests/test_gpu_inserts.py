@@ -104,10 +104,15 @@ def test_delta_tree_merged_for_served_path_and_tree_view(oracle):
     ix.Insert(dev(extra[:10]))
     ix.Build()
     assert ix.stats()["delta_boxes"] == 10
+    ix.Optimize()  # merges, as the reference's Optimize merges its per-Insert BVHs
+    assert ix.stats()["delta_boxes"] == 0
+    ix.Insert(dev(extra[10:20]))
+    ix.Build()
+    assert ix.stats()["delta_boxes"] == 10
     v = R._native.rts_tree_view()
     N = R._native
     N.check(N.lib().rts_index_tree_view(ix._handle(), None, ctypes.byref(v)), "tree view")
-    assert v.n == 101_010 and ix.stats()["delta_boxes"] == 0
+    assert v.n == 101_020 and ix.stats()["delta_boxes"] == 0
     ix.Sync()
 
 
