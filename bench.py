@@ -874,7 +874,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default plans)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
+        print(json.dumps({"error": "bench.py measures one GPU; run with --gpus 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch

@@ -1,6 +1,6 @@
 // CPU oracle for the LibRTS spatial-index hot path.
 //
-// TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/,
+// TEST INFRASTRUCTURE ONLY. This library checks results: only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
 // product (rtspatial_amd) never links or calls it.
 //

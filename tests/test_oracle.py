@@ -1,5 +1,5 @@
 """CPU tests of the oracle: it must reproduce the reference tests' known
-answers and agree with independent checks before it may judge the GPU path."""
+answers and agree with independent checks before it may check the GPU path."""
 import json
 import os
 
