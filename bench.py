@@ -15,8 +15,10 @@ the plain path's capacity; both capacities and latencies are reported.
 the 10M boxes per step, all ~1e10 pairs collected on the device.
 --workload c3f64: the same on a double-coordinate index (SpatialIndex<double, 2>,
 exact double predicates), boxes and queries drawn in double.
+--workload lines: line-segment intersects, the diagonals of the c3 query
+envelopes as segments over the same boxes.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served|c3|c3f64|c4]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served|c3|c3f64|lines|c4]
 """
 import argparse
 import json
@@ -526,6 +528,129 @@ def run_c3(args):
     return out
 
 
+# ------------------------------------------------------------------ lines --
+def run_lines(args):
+    """Line-segment intersects (Query(kIntersects, ArrayView<line_t>),
+    spatial_index.cuh:517-547): the segments are the diagonals of the skewed
+    query envelopes of --workload c3 (min corner to max corner, as the
+    reference's GenerateUniformLines draws them, test_commons.h:69-97), over
+    the same 10M boxes; all pairs collected on the device. Both traversals are
+    timed; `value` is the library's default plan."""
+    import torch
+
+    import rtspatial_amd as R
+    from rtspatial_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    log(f"generating workload: {args.boxes} boxes, {args.queries} segments")
+    boxes_np, qboxes = W.config3(args.boxes, args.queries)
+    segs_np = np.ascontiguousarray(qboxes)  # (x0, y0, x1, y1) read as p1 = min, p2 = max
+    boxes = torch.from_numpy(boxes_np).to(dev)
+    segs = torch.from_numpy(segs_np).to(dev)
+    pred = R.Predicate.kIntersects
+    nq = segs.shape[0]
+    results = {}
+    value_plan = None
+    for trav in (0, R._native.RTS_TRAVERSAL_THREAD, R._native.RTS_TRAVERSAL_WAVE):
+        ix = R.SpatialIndex()
+        ix.Init(R.Config(max_geometries=args.boxes, traversal=trav))
+        build = build_rate(R, ix, boxes, torch)
+        cnt = R.SharedValue()
+        cnt.set(0)
+        ix.QueryLines(pred, segs, cnt)
+        n_pairs = cnt.get()
+        queue = R.Queue(n_pairs + 1024)
+        for _ in range(args.warmup):
+            queue.Clear()
+            ix.QueryLines(pred, segs, queue)
+        torch.cuda.synchronize()
+        R.profile_enable(True)
+        R.profile_read(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            queue.Clear()
+            ix.QueryLines(pred, segs, queue)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        prof = R.profile_read(reset=True)
+        R.profile_enable(False)
+        ix.Sync()
+        assert queue.size() == n_pairs and not queue.overflowed()
+        trav_n, trav_ms = prof["traverse"]
+        name = {0: "default", 1: "thread", 2: "wave"}[trav]
+        results[name] = {"ms_per_step": round(wall * 1e3 / args.steps, 4),
+                         "traverse_ms": round(trav_ms / max(trav_n, 1), 4),
+                         "count_scan_ms": round(prof["query_count"][1] / max(args.steps, 1), 4),
+                         "pairs": n_pairs}
+        log(f"lines {name}: {results[name]}")
+        if trav == 0:
+            value_plan = ix
+            plan_pairs = n_pairs
+            plan_build = build
+        else:
+            del ix
+        del queue
+        torch.cuda.empty_cache()
+    # results against the oracle on a sample of the segments
+    rs = np.random.default_rng(98)
+    sample = segs_np[rs.choice(nq, 200, replace=False)]
+    c = R.SharedValue()
+    c.set(0)
+    value_plan.QueryLines(pred, torch.from_numpy(sample).to(dev), c)
+    got_sample = c.get()
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    want_sample = O.bf_count(int(pred), boxes_np, sample, threads=cpu_threads(), lines=True)
+    t_cpu = time.perf_counter() - t0
+    agree = got_sample == want_sample
+    log(f"sample of 200 segments: gpu {got_sample} oracle {want_sample}")
+    base = results["default"]
+    ms_per_step = base["ms_per_step"]
+    qbytes = query_bytes(nq, 16, plan_pairs, args.boxes)
+    achieved = qbytes / (base["traverse_ms"] / 1e3) / 1e9
+    traffic, traffic_src = profiled_traffic("lines") \
+        if args.boxes == 10_000_000 and nq == 1_000_000 else (None, None)
+    return {
+        "metric": "Mqueries/s, line-segment intersects, 10M-box index (plain LBVH)",
+        "value": round(nq / (ms_per_step / 1e3) / 1e6, 4),
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (exact predicate, f64 filter)",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"line-segment intersects: {args.boxes} Gaussian-mixture boxes x {nq} "
+                         f"segments (the diagonals of the range-intersects query envelopes); "
+                         f"one QueryLines(kIntersects) per step, all pairs collected on the device"),
+            "path": "plain", "boxes": args.boxes, "queries": nq, "pairs_per_step": plan_pairs,
+            "seeds": {"boxes": 3001, "queries": 3002},
+        },
+        "build": plan_build,
+        "traversals": results,
+        "results_agree": bool(agree),
+        "sample_check": {"segments": 200, "gpu_pairs": got_sample, "oracle_pairs": want_sample},
+        "roofline": {"bound": "hbm", "kernel": "k_query_*<segment, write> (the default plan's)",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": int(qbytes)},
+        "cpu_baseline": None if args.no_cpu_baseline else {
+            "value": round(sample.shape[0] / t_cpu / 1e6, 6),
+            "unit": "Mqueries/s",
+            "cores": cpu_threads(),
+            "kind": "port",
+            "sample": (f"oracle brute-force exact segment join: 200 segments drawn from the "
+                       f"workload x all {args.boxes} boxes ({want_sample} results, counted) on "
+                       f"{cpu_threads()} threads"),
+        },
+    }
+
+
 # -------------------------------------------------------------- handlers --
 def run_handlers(args):
     """The 10M x 1M intersects join with result handlers compiled outside the
@@ -861,7 +986,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload",
-                    choices=["served", "c3", "c3f64", "handlers", "inserts", "c4", "copy"],
+                    choices=["served", "c3", "c3f64", "lines", "handlers", "inserts", "c4",
+                             "copy"],
                     default="served")
     ap.add_argument("--rounds", type=int, default=20)
     ap.add_argument("--boxes", type=int, default=10_000_000)
@@ -881,7 +1007,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    out = {"served": run_served, "c3": run_c3, "c3f64": run_c3, "handlers": run_handlers,
+    out = {"served": run_served, "c3": run_c3, "c3f64": run_c3, "lines": run_lines,
+           "handlers": run_handlers,
            "inserts": run_inserts, "c4": run_c4,
            "copy": run_copy_calibration}[args.workload](args)
     print(json.dumps(out), flush=True)

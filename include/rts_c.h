@@ -72,6 +72,16 @@ typedef struct rts_update_d {
   rts_envelope_d box;
 } rts_update_d;
 
+/* Line<Point<float,2>> (geom/line.cuh:10-27): the segment p1-p2, 16 bytes. */
+typedef struct rts_line {
+  rts_point p1, p2;
+} rts_line;
+
+/* Line<Point<double,2>>: 32 bytes. */
+typedef struct rts_line_d {
+  rts_point_d p1, p2;
+} rts_line_d;
+
 /* One result of the collecting handler, in its order
  * (default_handlers/collecting_handlers.h:9-15): geom_id first. */
 typedef struct rts_pair {
@@ -227,6 +237,30 @@ int rts_index_query_envelopes_d(rts_index* index, rts_predicate pred,
                                 const rts_envelope_d* queries, uint64_t n, rts_handler handler,
                                 void* arg, rts_stream_t stream);
 
+/* SpatialIndex::Query(Predicate, ArrayView<line_t>, void*, stream)
+ * (spatial_index.cuh:517-547), line segments against data boxes.
+ *   RTS_INTERSECTS: the closed segment p1-p2 meets the closed data box,
+ *                   decided exactly (any float coordinates; doubles of
+ *                   magnitude 0 or 2^-450 .. 2^500). The reference declares
+ *                   this query but its intersection program reports nothing
+ *                   (shaders_intersects_line_query_2d.cu:14-41; README.md:59
+ *                   lists line queries as not yet implemented).
+ *   RTS_CONTAINS:   no results, as the reference (spatial_index.cuh:524-526).
+ *   RTS_WITHIN:     RTS_ERR_INVALID.
+ * A segment with a NaN coordinate meets nothing. _d: a double index. */
+int rts_index_query_lines(rts_index* index, rts_predicate pred, const rts_line* queries,
+                          uint64_t n, rts_handler handler, void* arg, rts_stream_t stream);
+int rts_index_query_lines_d(rts_index* index, rts_predicate pred, const rts_line_d* queries,
+                            uint64_t n, rts_handler handler, void* arg, rts_stream_t stream);
+
+/* The kind of the queries of rts_index_query_plan / rts_index_order_queries
+ * (their `points` argument). */
+typedef enum rts_query_kind {
+  RTS_QUERY_ENVELOPES = 0,
+  RTS_QUERY_POINTS = 1,
+  RTS_QUERY_LINES = 2
+} rts_query_kind;
+
 /* The index's tree, for kernels a caller compiles around its own result
  * handler (include/rtspatial/device/query.h). The reference compiles its
  * query shaders around a handler header of rtspatial_handle_* functions
@@ -253,8 +287,9 @@ typedef struct rts_tree_view {
 
 int rts_index_tree_view(rts_index* index, rts_stream_t stream, rts_tree_view* out);
 
-/* The plan the library runs a query call of this kind and size under: the
- * config's settings, else the autotuner's stored plan, else the defaults.
+/* The plan the library runs a query call of this kind (`points`: an
+ * rts_query_kind) and size under: the config's settings, else the
+ * autotuner's stored plan (not for segments), else the defaults.
  * Kernels compiled by the caller follow it to run as the library's do. */
 typedef struct rts_query_plan {
   uint32_t traversal;    /* RTS_TRAVERSAL_THREAD or RTS_TRAVERSAL_WAVE */
@@ -267,7 +302,8 @@ int rts_index_query_plan(rts_index* index, rts_predicate pred, int points, uint6
                          rts_query_plan* out);
 
 /* order[i] = index of the query that comes i-th in Morton order of the query
- * points / envelope centres (over the index's scene bounds; the tree must be
+ * points / envelope and segment bounding-box centres (`points`: an
+ * rts_query_kind; over the index's scene bounds; the tree must be
  * current, e.g. after rts_index_tree_view on the same stream). `queries` are
  * floats for a float index and doubles for a double one; n < 2^32. Queued on
  * `stream`. */

@@ -1,4 +1,5 @@
-// Queries whose results go to a handler compiled by the caller.
+// Queries (points, envelopes, line segments) whose results go to a handler
+// compiled by the caller.
 //
 // The reference compiles its query shaders around a handler header: four
 // extern "C" device functions rtspatial_handle_*(geom_id, query_id, arg) that
@@ -83,7 +84,7 @@ __global__ void __launch_bounds__(kWave) k_user_wave(const TreeView t, const voi
     const uint32_t k = grab_query(tickets, n, slice);
     if (k >= n) return;
     const uint32_t i = order ? order[k] : k;
-    typename QueryOf<F64>::type q;
+    query_t<PRED, F64> q;
     load_q(queries, POINTS, i, q);
     uint64_t c;
     if constexpr (WantsPairs<H>::value)
@@ -106,7 +107,7 @@ __global__ void __launch_bounds__(kLaneThreads) k_user_lane(const TreeView t, co
   const uint32_t i = order ? order[k] : k;
   uint64_t c = 0;
   if (t.n > 0) {
-    typename QueryOf<F64>::type q;
+    query_t<PRED, F64> q;
     load_q(queries, POINTS, i, q);
     if constexpr (WantsPairs<H>::value)
       c = lane_query<PRED, POINTS, true>(t, q, i, CallEmit<H>{h});
@@ -167,10 +168,18 @@ void launch(const TreeView& t, const void* q, uint32_t n, const uint32_t* order,
   }
 }
 
+// kind: an rts_query_kind. Segment queries with kContains have no results;
+// they run on an empty tree so that done() still reports each query.
 template <bool F64, typename H>
-void dispatch(const TreeView& t, int pred, bool points, const void* q, uint32_t n,
+void dispatch(const TreeView& t, int pred, int kind, const void* q, uint32_t n,
               const uint32_t* order, bool wave, const H& h, hipStream_t stream) {
-  if (points) return launch<RTS_CONTAINS, true, F64>(t, q, n, order, wave, h, stream);
+  if (kind == RTS_QUERY_POINTS)
+    return launch<RTS_CONTAINS, true, F64>(t, q, n, order, wave, h, stream);
+  if (kind == RTS_QUERY_LINES) {
+    TreeView tt = t;
+    if (pred != RTS_INTERSECTS) tt.n = 0;
+    return launch<kPredLine, false, F64>(tt, q, n, order, wave, h, stream);
+  }
   switch (pred) {
     case RTS_INTERSECTS:
       return launch<RTS_INTERSECTS, false, F64>(t, q, n, order, wave, h, stream);
@@ -184,28 +193,25 @@ void dispatch(const TreeView& t, int pred, bool points, const void* q, uint32_t 
 
 namespace rtspatial {
 
-// Runs `queries` (n points or envelopes, floats for a float index, doubles
-// for a double one, in device or pinned memory) against `index` with handler
-// `h` on `stream`, under the plan the library would run the same call under
-// (rts_index_query_plan: traversal mode, Morton ordering, leaf-range size);
-// `traversal` other than RTS_TRAVERSAL_AUTO overrides its mode. Throws
-// rtspatial::Exception on a bad argument or a failed launch; rts_index_sync
-// reports a kernel's internal error.
+namespace detail {
+
+// QueryWith over one query kind (an rts_query_kind).
 template <typename Handler>
-void QueryWith(rts_index* index, Predicate p, const void* queries, uint64_t n, bool points,
-               bool doubles, const Handler& h, hipStream_t stream = nullptr,
-               uint32_t traversal = RTS_TRAVERSAL_AUTO) {
+void QueryWithKind(rts_index* index, int pred, const void* queries, uint64_t n, int kind,
+                   bool doubles, const Handler& h, hipStream_t stream, uint32_t traversal) {
   static_assert(std::is_trivially_copyable<Handler>::value, "a handler is a kernel argument");
-  const int pred = static_cast<int>(p);
   if (pred < RTS_INTERSECTS || pred > RTS_WITHIN) throw Exception("QueryWith: unknown predicate");
-  if (points && pred != RTS_CONTAINS)
+  if (kind == RTS_QUERY_POINTS && pred != RTS_CONTAINS)
     throw Exception("QueryWith: point queries support only kContains (spatial_index.cuh:385-389)");
+  if (kind == RTS_QUERY_LINES && pred == RTS_WITHIN)
+    throw Exception("QueryWith: segment queries take kIntersects or kContains "
+                    "(spatial_index.cuh:523-545)");
   if (n >= (1ull << 32)) throw Exception("QueryWith: at most 2^32 - 1 queries per call");
   if (n && !queries) throw Exception("QueryWith: null queries");
   rts_tree_view v;
   check(rts_index_tree_view(index, stream, &v), "QueryWith");
   rts_query_plan plan;
-  check(rts_index_query_plan(index, static_cast<rts_predicate>(pred), points ? 1 : 0, n, &plan),
+  check(rts_index_query_plan(index, static_cast<rts_predicate>(pred), kind, n, &plan),
         "QueryWith");
   if (doubles != (v.coord != RTS_COORD_F32))
     throw Exception(doubles ? "QueryWith: double queries on a float index"
@@ -228,16 +234,16 @@ void QueryWith(rts_index* index, Predicate p, const void* queries, uint64_t n, b
     rts::user::check_hip(hipMallocAsync(reinterpret_cast<void**>(&order), n * sizeof(uint32_t),
                                         stream),
                          "hipMallocAsync");
-    const int rc = rts_index_order_queries(index, queries, n, points ? 1 : 0, order, stream);
+    const int rc = rts_index_order_queries(index, queries, n, kind, order, stream);
     if (rc != RTS_OK) {
       (void)hipFreeAsync(order, stream);
       check(rc, "QueryWith");
     }
   }
   if (v.coord == RTS_COORD_F64) {
-    rts::user::dispatch<true>(t, pred, points, queries, (uint32_t)n, order, wave, h, stream);
+    rts::user::dispatch<true>(t, pred, kind, queries, (uint32_t)n, order, wave, h, stream);
   } else if (v.coord == RTS_COORD_F64_AS_F32) {
-    const uint64_t words = n * (points ? 2 : 4);
+    const uint64_t words = n * (kind == RTS_QUERY_POINTS ? 2 : 4);
     float* rounded = nullptr;
     rts::user::check_hip(hipMallocAsync(reinterpret_cast<void**>(&rounded), words * sizeof(float),
                                         stream),
@@ -245,12 +251,41 @@ void QueryWith(rts_index* index, Predicate p, const void* queries, uint64_t n, b
     uint64_t blocks = (words + 255) / 256;
     hipLaunchKernelGGL(rts::user::k_user_round<>, dim3((uint32_t)(blocks < 4096 ? blocks : 4096)),
                        dim3(256), 0, stream, static_cast<const double*>(queries), words, rounded);
-    rts::user::dispatch<false>(t, pred, points, rounded, (uint32_t)n, order, wave, h, stream);
+    rts::user::dispatch<false>(t, pred, kind, rounded, (uint32_t)n, order, wave, h, stream);
     rts::user::check_hip(hipFreeAsync(rounded, stream), "hipFreeAsync");
   } else {
-    rts::user::dispatch<false>(t, pred, points, queries, (uint32_t)n, order, wave, h, stream);
+    rts::user::dispatch<false>(t, pred, kind, queries, (uint32_t)n, order, wave, h, stream);
   }
   if (order) rts::user::check_hip(hipFreeAsync(order, stream), "hipFreeAsync");
+}
+
+}  // namespace detail
+
+// Runs `queries` (n points or envelopes, floats for a float index, doubles
+// for a double one, in device or pinned memory) against `index` with handler
+// `h` on `stream`, under the plan the library would run the same call under
+// (rts_index_query_plan: traversal mode, Morton ordering, leaf-range size);
+// `traversal` other than RTS_TRAVERSAL_AUTO overrides its mode. Throws
+// rtspatial::Exception on a bad argument or a failed launch; rts_index_sync
+// reports a kernel's internal error.
+template <typename Handler>
+void QueryWith(rts_index* index, Predicate p, const void* queries, uint64_t n, bool points,
+               bool doubles, const Handler& h, hipStream_t stream = nullptr,
+               uint32_t traversal = RTS_TRAVERSAL_AUTO) {
+  detail::QueryWithKind(index, static_cast<int>(p), queries, n,
+                        points ? RTS_QUERY_POINTS : RTS_QUERY_ENVELOPES, doubles, h, stream,
+                        traversal);
+}
+
+// The same for n line segments (rts_line / rts_line_d layout): kIntersects
+// calls the handler for every data box a segment meets; kContains has no
+// results (spatial_index.cuh:524-526), so only done() is called, with 0.
+template <typename Handler>
+void QueryLinesWith(rts_index* index, Predicate p, const void* queries, uint64_t n, bool doubles,
+                    const Handler& h, hipStream_t stream = nullptr,
+                    uint32_t traversal = RTS_TRAVERSAL_AUTO) {
+  detail::QueryWithKind(index, static_cast<int>(p), queries, n, RTS_QUERY_LINES, doubles, h,
+                        stream, traversal);
 }
 
 #ifdef RTSPATIAL_USER_HANDLERS
@@ -287,6 +322,16 @@ inline void QueryWithReferenceHandlers(rts_index* index, Predicate p, const void
   if (p == Predicate::kIntersects)
     return QueryWith(index, p, queries, n, false, doubles, RefEnvelopeIntersects{arg}, stream);
   throw Exception("Query: the reference's handler functions have no kWithin form");
+}
+
+// spatial_index.cuh:517-547 with the caller's handler functions: segment
+// intersects results go to rtspatial_handle_envelope_intersects, the function
+// the reference's line program is written to call
+// (shaders_intersects_line_query_2d.cu:36); kContains has no results.
+inline void QueryLinesWithReferenceHandlers(rts_index* index, Predicate p, const void* queries,
+                                            uint64_t n, bool doubles, void* arg,
+                                            hipStream_t stream) {
+  return QueryLinesWith(index, p, queries, n, doubles, RefEnvelopeIntersects{arg}, stream);
 }
 #endif
 

@@ -19,9 +19,19 @@
 // shrunk by one ulp -- which lies inside the double box -- passes the
 // sufficient test); otherwise, when a coordinate lies within one float ulp of
 // the query's, the double box is read and tested exactly.
+//
+// Line-segment queries (QueryLine, predicate kPredLine) run the same walks:
+// a node is entered when the segment may meet its bounds (segment.h, a
+// necessary condition), and a leaf is reported when the closed segment meets
+// the closed box, decided exactly (on a double index: after the necessary
+// test on the outward float box, on the double box).
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include "rtspatial/device/round.h"
+#include "rtspatial/device/segment.h"
 #include "rtspatial/device/tree.h"
 #include "rtspatial/device/wave.h"
 
@@ -37,6 +47,15 @@ struct QueryF64 {
   float4 in;   // (up x0, up y0, down x1, down y1)
   float4 out;  // (down x0, down y0, up x1, up y1)
   const double* q;
+};
+
+// A segment query (x1, y1, x2, y2): its coordinates as doubles for the exact
+// tests and its bounding box in floats for a first test of a node (rounded
+// outward on a double index, so the test stays a necessary condition).
+template <bool F64>
+struct QueryLine {
+  double s[4];
+  float4 bb;
 };
 
 template <int PRED, bool POINTS>
@@ -157,6 +176,24 @@ __device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, cons
   if (P::suff(b, q)) return true;
   return P::exact(t.boxes64[t.leaf_ids[leaf]], q.q);
 }
+// Segments: no subtree lies "inside" a segment.
+template <int PRED, bool POINTS, bool F64>
+__device__ __forceinline__ bool node_test(const float4 n, const QueryLine<F64>& q) {
+  return intersects(n, q.bb) && segment_may_meet_box(q.s, n.x, n.y, n.z, n.w);
+}
+template <int PRED, bool POINTS, bool F64>
+__device__ __forceinline__ bool inside_test(const float4, const QueryLine<F64>&) {
+  return false;
+}
+template <int PRED, bool POINTS, bool F64>
+__device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, const float4 b,
+                                          const QueryLine<F64>& q) {
+  if (!intersects(b, q.bb)) return false;
+  if (!F64) return segment_meets_box(q.s, b.x, b.y, b.z, b.w);
+  if (!segment_may_meet_box(q.s, b.x, b.y, b.z, b.w)) return false;
+  const Box64 d = t.boxes64[t.leaf_ids[leaf]];
+  return segment_meets_box(q.s, d.x0, d.y0, d.x1, d.y1);
+}
 
 // Emitters. The result queue: the query's k-th pair goes to slot base + k,
 // dropped at or past `cap`.
@@ -192,15 +229,21 @@ __device__ __forceinline__ float4 load_query(const void* queries, bool points, u
   return static_cast<const float4*>(queries)[i];
 }
 
-// Query i of a float (double2 / Box64 when F64) array.
-template <bool F64>
+// The query type of a walk: float or double, point / envelope or segment.
+template <bool F64, bool LINE = false>
 struct QueryOf {
   using type = QueryF32;
 };
 template <>
-struct QueryOf<true> {
+struct QueryOf<true, false> {
   using type = QueryF64;
 };
+template <bool F64>
+struct QueryOf<F64, true> {
+  using type = QueryLine<F64>;
+};
+template <int PRED, bool F64>
+using query_t = typename QueryOf<F64, PRED == kPredLine>::type;
 __device__ __forceinline__ void load_q(const void* queries, bool points, uint32_t i, QueryF32& q) {
   q.b = load_query(queries, points, i);
 }
@@ -211,6 +254,26 @@ __device__ __forceinline__ void load_q(const void* queries, bool points, uint32_
   q.in = make_float4(f32_ru(x0), f32_ru(y0), f32_rd(x1), f32_rd(y1));
   q.out = make_float4(f32_rd(x0), f32_rd(y0), f32_ru(x1), f32_ru(y1));
   q.q = p;
+}
+// Segment i of a float (x1, y1, x2, y2) array, or of a double one. A segment
+// with a NaN coordinate gets a NaN bounding box and so meets nothing.
+__device__ __forceinline__ void load_q(const void* queries, bool, uint32_t i, QueryLine<false>& q) {
+  const float4 f = static_cast<const float4*>(queries)[i];
+  q.s[0] = f.x;
+  q.s[1] = f.y;
+  q.s[2] = f.z;
+  q.s[3] = f.w;
+  q.bb = make_float4(fminf(f.x, f.z), fminf(f.y, f.w), fmaxf(f.x, f.z), fmaxf(f.y, f.w));
+  if (!(f.x == f.x && f.y == f.y && f.z == f.z && f.w == f.w))
+    q.bb = make_float4(__builtin_nanf(""), 0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ void load_q(const void* queries, bool, uint32_t i, QueryLine<true>& q) {
+  const double* p = static_cast<const double*>(queries) + 4ull * i;
+  for (int k = 0; k < 4; k++) q.s[k] = p[k];
+  q.bb = make_float4(f32_rd(fmin(q.s[0], q.s[2])), f32_rd(fmin(q.s[1], q.s[3])),
+                     f32_ru(fmax(q.s[0], q.s[2])), f32_ru(fmax(q.s[1], q.s[3])));
+  if (!(q.s[0] == q.s[0] && q.s[1] == q.s[1] && q.s[2] == q.s[2] && q.s[3] == q.s[3]))
+    q.bb = make_float4(__builtin_nanf(""), 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- wave ----
@@ -246,9 +309,18 @@ struct WaveLds {
 // Returns the query's number of results (wave-uniform). With WRITE every
 // pair (geom_id, qid) is emitted, numbered from 0 in emission order; without
 // WRITE, subtrees inside the query are counted from their live-leaf counts.
+// An emitter may also define begin_step(first_k, n), called by the whole wave
+// before the n results numbered first_k.. of one scan step are emitted (an
+// emitter that stages results per wave makes room for them there).
+template <typename E, typename = void>
+struct HasBeginStep : std::false_type {};
+template <typename E>
+struct HasBeginStep<E, std::void_t<decltype(std::declval<E&>().begin_step(0ull, 0u))>>
+    : std::true_type {};
+
 template <int PRED, bool POINTS, bool WRITE, typename Q, typename Emit>
 __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qid,
-                               const Emit& emit, WaveLds& L) {
+                               Emit&& emit, WaveLds& L) {
   constexpr bool kCounting = !WRITE;
   const uint32_t lane = lane_id();
   uint32_t* stack = L.stack;
@@ -417,6 +489,12 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
         }
         // (4) slots: position order is k-major, so the pairs of round k go
         // right after those of rounds < k, in lane order.
+        if constexpr (WRITE && HasBeginStep<Emit>::value) {
+          uint32_t n = 0;
+#pragma unroll
+          for (int k = 0; k < kPerLane; k++) n += (uint32_t)__popcll(ballot((hitmask >> k) & 1u));
+          if (n) emit.begin_step(emitted, n);
+        }
         uint32_t step_hits = 0;  // wave-uniform
 #pragma unroll
         for (int k = 0; k < kPerLane; k++) {

@@ -53,6 +53,10 @@ struct TreeView {
   const Box64* boxes64;       // double index: the boxes by id (else null)
 };
 
+// Internal predicate id of line-segment queries (closed segment meets closed
+// box); not an rts_predicate value.
+constexpr int kPredLine = 3;
+
 // Device error bits (the index's error word).
 constexpr uint32_t kErrSortWait = 1u;
 constexpr uint32_t kErrRefitWalk = 2u;

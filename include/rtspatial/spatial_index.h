@@ -27,6 +27,7 @@
 #include "rts_c.h"
 #include "rtspatial/default_handlers/collecting_handlers.h"
 #include "rtspatial/geom/envelope.h"
+#include "rtspatial/geom/line.h"
 #include "rtspatial/geom/point.h"
 #include "rtspatial/geom/predicates.h"
 #include "rtspatial/utils/array_view.h"
@@ -78,6 +79,7 @@ struct CoordAbi<float> {
   using c_point = rts_point;
   using c_envelope = rts_envelope;
   using c_update = rts_update;
+  using c_line = rts_line;
   static uint32_t coord(uint32_t) { return RTS_COORD_F32; }
   static int insert(rts_index* i, const c_envelope* b, uint64_t n, hipStream_t s) {
     return rts_index_insert(i, b, n, s);
@@ -93,12 +95,17 @@ struct CoordAbi<float> {
                    void* a, hipStream_t s) {
     return rts_index_query_envelopes(i, p, q, n, h, a, s);
   }
+  static int query(rts_index* i, rts_predicate p, const c_line* q, uint64_t n, rts_handler h,
+                   void* a, hipStream_t s) {
+    return rts_index_query_lines(i, p, q, n, h, a, s);
+  }
 };
 template <>
 struct CoordAbi<double> {
   using c_point = rts_point_d;
   using c_envelope = rts_envelope_d;
   using c_update = rts_update_d;
+  using c_line = rts_line_d;
   static uint32_t coord(uint32_t requested) { return requested; }
   static int insert(rts_index* i, const c_envelope* b, uint64_t n, hipStream_t s) {
     return rts_index_insert_d(i, b, n, s);
@@ -114,6 +121,10 @@ struct CoordAbi<double> {
                    void* a, hipStream_t s) {
     return rts_index_query_envelopes_d(i, p, q, n, h, a, s);
   }
+  static int query(rts_index* i, rts_predicate p, const c_line* q, uint64_t n, rts_handler h,
+                   void* a, hipStream_t s) {
+    return rts_index_query_lines_d(i, p, q, n, h, a, s);
+  }
 };
 }  // namespace detail
 
@@ -128,6 +139,8 @@ class SpatialIndex<COORD_T, 2> {
   using point_t = Point<COORD_T, 2>;
   using envelope_t = Envelope<point_t>;
   using update_t = pair<size_t, envelope_t>;
+  using line_t = Line<point_t>;
+  static_assert(sizeof(line_t) == sizeof(typename abi::c_line), "line layout");
   static_assert(sizeof(point_t) == sizeof(typename abi::c_point), "point layout");
   static_assert(sizeof(envelope_t) == sizeof(typename abi::c_envelope), "envelope layout");
   static_assert(sizeof(update_t) == sizeof(typename abi::c_update),
@@ -214,6 +227,17 @@ class SpatialIndex<COORD_T, 2> {
 #endif
   }
 
+  // spatial_index.cuh:517-547: kIntersects reports the data boxes a segment
+  // meets (exact); kContains reports nothing, as the reference's.
+  void Query(Predicate p, ArrayView<line_t> queries, void* arg, hipStream_t stream = nullptr) {
+#ifdef RTSPATIAL_USER_HANDLERS
+    QueryLinesWithReferenceHandlers(handle(), p, queries.data(), queries.size(), kDoubles, arg,
+                                    stream);
+#else
+    Query(p, queries, static_cast<rts_handler>(RTSPATIAL_HANDLER), arg, stream);
+#endif
+  }
+
 #if defined(__HIP__)
   // Results to a handler functor of the caller's (rtspatial/device/query.h).
   template <typename Handler>
@@ -227,6 +251,12 @@ class SpatialIndex<COORD_T, 2> {
                  hipStream_t stream = nullptr, uint32_t traversal = RTS_TRAVERSAL_AUTO) {
     rtspatial::QueryWith(handle(), p, queries.data(), queries.size(), false, kDoubles, h, stream,
                          traversal);
+  }
+  template <typename Handler>
+  void QueryWith(Predicate p, ArrayView<line_t> queries, const Handler& h,
+                 hipStream_t stream = nullptr, uint32_t traversal = RTS_TRAVERSAL_AUTO) {
+    rtspatial::QueryLinesWith(handle(), p, queries.data(), queries.size(), kDoubles, h, stream,
+                              traversal);
   }
 #endif
 
@@ -244,6 +274,13 @@ class SpatialIndex<COORD_T, 2> {
                      reinterpret_cast<const typename abi::c_envelope*>(queries.data()),
                      queries.size(), handler, arg, stream),
           "SpatialIndex::Query(envelopes)");
+  }
+  void Query(Predicate p, ArrayView<line_t> queries, rts_handler handler, void* arg,
+             hipStream_t stream) {
+    check(abi::query(handle(), static_cast<rts_predicate>(p),
+                     reinterpret_cast<const typename abi::c_line*>(queries.data()), queries.size(),
+                     handler, arg, stream),
+          "SpatialIndex::Query(lines)");
   }
 
   // spatial_index.cuh:766-826 merges the per-Insert BVHs; here it brings the

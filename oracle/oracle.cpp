@@ -12,6 +12,8 @@
 //   * a brute-force join and an STR R-tree join over those predicates
 //   * the same predicates and brute-force join in double precision
 //     (SpatialIndex<double, 2>), exact
+//   * line-segment intersects (the query of spatial_index.cuh:517-547,
+//     whose reference program reports nothing), exact
 //   * the reference's ray-slab formulation of intersects
 //                                            include/rtspatial/details/ray_params.h:53-167,
 //                                            src/shaders/shaders_intersects_envelope_query_2d.cu:14-128
@@ -31,15 +33,22 @@
 //   generators               : the known answers above are only reached with
 //                              the exact libstdc++ mt19937 streams.
 //   hash/count summaries     : equality with summaries recomputed from full pair sets.
+//   segment predicate        : equality with the exact rational restatement in
+//                              oracle.py, itself checked against an exact
+//                              Liang-Barsky clip and closed-form cases.
 //
 // Deleted boxes are modelled by an optional `alive` mask (nullptr = all alive).
 // Predicate codes: 0 = intersects, 1 = contains (data box contains the query,
 // the reference's kContains), 2 = within (data box inside the query envelope).
-// Query kinds: 0 = points (x, y), 1 = envelopes (x0, y0, x1, y1).
+// Query kinds: 0 = points (x, y), 1 = envelopes (x0, y0, x1, y1), 2 = line
+// segments (x1, y1, x2, y2): intersects only (contains and within match
+// nothing, as the reference's kContains line query reports nothing,
+// spatial_index.cuh:524-526).
 
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -56,7 +65,148 @@ struct Box {
 };
 
 enum { P_INTERSECTS = 0, P_CONTAINS = 1, P_WITHIN = 2 };
-enum { Q_POINT = 0, Q_ENVELOPE = 1 };
+enum { Q_POINT = 0, Q_ENVELOPE = 1, Q_LINE = 2 };
+
+// ------------------------------------------------------ line segments ----
+// Line<Point<T,2>> queries (geom/line.cuh:10-27) against closed data boxes:
+// the closed segment (x1, y1, x2, y2) meets the closed box iff their bounding
+// boxes overlap and the four box corners are not all strictly on one side of
+// the segment's line. The side of corner c is the sign of
+//   (x2 - x1) (cy - y1) - (y2 - y1) (cx - x1),
+// taken from a double evaluation when it clears a generous error margin and
+// otherwise computed exactly in integers: every double is m 2^k with an
+// integer m, so after scaling by 2^-kmin the differences and products are
+// integer arithmetic (sign-magnitude, 32-bit limbs).
+struct BigInt {
+  bool neg = false;
+  std::vector<uint32_t> mag;  // little-endian, no leading zero limbs
+};
+void big_trim(BigInt& a) {
+  while (!a.mag.empty() && a.mag.back() == 0) a.mag.pop_back();
+  if (a.mag.empty()) a.neg = false;
+}
+int mag_cmp(const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {
+  if (a.size() != b.size()) return a.size() < b.size() ? -1 : 1;
+  for (size_t i = a.size(); i-- > 0;)
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  return 0;
+}
+std::vector<uint32_t> mag_add(const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {
+  std::vector<uint32_t> r(std::max(a.size(), b.size()) + 1, 0);
+  uint64_t carry = 0;
+  for (size_t i = 0; i < r.size(); i++) {
+    uint64_t v = carry + (i < a.size() ? a[i] : 0) + (i < b.size() ? b[i] : 0);
+    r[i] = (uint32_t)v;
+    carry = v >> 32;
+  }
+  return r;
+}
+std::vector<uint32_t> mag_sub(const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {
+  // a >= b
+  std::vector<uint32_t> r(a.size(), 0);
+  int64_t borrow = 0;
+  for (size_t i = 0; i < a.size(); i++) {
+    int64_t v = (int64_t)a[i] - borrow - (int64_t)(i < b.size() ? b[i] : 0);
+    borrow = v < 0;
+    r[i] = (uint32_t)(v + (borrow << 32));
+  }
+  return r;
+}
+BigInt big_add(const BigInt& a, const BigInt& b) {
+  BigInt r;
+  if (a.neg == b.neg) {
+    r.neg = a.neg;
+    r.mag = mag_add(a.mag, b.mag);
+  } else if (mag_cmp(a.mag, b.mag) >= 0) {
+    r.neg = a.neg;
+    r.mag = mag_sub(a.mag, b.mag);
+  } else {
+    r.neg = b.neg;
+    r.mag = mag_sub(b.mag, a.mag);
+  }
+  big_trim(r);
+  return r;
+}
+BigInt big_neg(BigInt a) {
+  if (!a.mag.empty()) a.neg = !a.neg;
+  return a;
+}
+BigInt big_mul(const BigInt& a, const BigInt& b) {
+  BigInt r;
+  r.mag.assign(a.mag.size() + b.mag.size() + 1, 0);
+  for (size_t i = 0; i < a.mag.size(); i++) {
+    uint64_t carry = 0;
+    for (size_t j = 0; j < b.mag.size(); j++) {
+      uint64_t v = (uint64_t)a.mag[i] * b.mag[j] + r.mag[i + j] + carry;
+      r.mag[i + j] = (uint32_t)v;
+      carry = v >> 32;
+    }
+    for (size_t k = i + b.mag.size(); carry; k++) {
+      uint64_t v = (uint64_t)r.mag[k] + carry;
+      r.mag[k] = (uint32_t)v;
+      carry = v >> 32;
+    }
+  }
+  r.neg = a.neg != b.neg;
+  big_trim(r);
+  return r;
+}
+// The integer d * 2^-kmin, for a finite d whose exponent k (d = m 2^k, m an
+// integer of at most 53 bits) is >= kmin.
+BigInt big_from(double d, int kmin) {
+  BigInt r;
+  if (d == 0.0) return r;
+  int e;
+  const double f = std::frexp(std::fabs(d), &e);     // |d| = f 2^e, f in [0.5, 1)
+  uint64_t m = (uint64_t)std::ldexp(f, 53);           // exact
+  const int shift = e - 53 - kmin;
+  r.mag.assign((size_t)(shift / 32) + 3, 0);
+  const int w = shift / 32, b = shift % 32;
+  const unsigned __int128 v = (unsigned __int128)m << b;
+  for (int i = 0; i < 3; i++) r.mag[w + i] = (uint32_t)(v >> (32 * i));
+  r.neg = d < 0.0;
+  big_trim(r);
+  return r;
+}
+int exponent_of(double d) {  // k with d = m 2^k, m a 53-bit integer
+  int e;
+  std::frexp(d, &e);
+  return e - 53;
+}
+// Sign of (x2 - x1)(cy - y1) - (y2 - y1)(cx - x1), exactly.
+int orient_sign(const double* s, double cx, double cy) {
+  const double a = (s[2] - s[0]) * (cy - s[1]), b = (s[3] - s[1]) * (cx - s[0]);
+  const double o = a - b, margin = 1e-9 * (std::fabs(a) + std::fabs(b)) + 1e-290;
+  if (o > margin) return 1;
+  if (o < -margin) return -1;
+  const double v[6] = {s[0], s[1], s[2], s[3], cx, cy};
+  int kmin = INT32_MAX;
+  for (double x : v)
+    if (x != 0.0) kmin = std::min(kmin, exponent_of(x));
+  if (kmin == INT32_MAX) return 0;
+  BigInt x1 = big_from(s[0], kmin), y1 = big_from(s[1], kmin), x2 = big_from(s[2], kmin),
+         y2 = big_from(s[3], kmin), X = big_from(cx, kmin), Y = big_from(cy, kmin);
+  const BigInt r = big_add(big_mul(big_add(x2, big_neg(x1)), big_add(Y, big_neg(y1))),
+                           big_neg(big_mul(big_add(y2, big_neg(y1)), big_add(X, big_neg(x1)))));
+  return r.mag.empty() ? 0 : (r.neg ? -1 : 1);
+}
+// The closed segment s meets the closed box (x0, y0, x1, y1); a NaN
+// coordinate of either meets nothing.
+bool segment_meets(const double* s, double x0, double y0, double x1, double y1) {
+  for (int k = 0; k < 4; k++)
+    if (std::isnan(s[k])) return false;
+  if (!(std::min(s[0], s[2]) <= x1 && std::max(s[0], s[2]) >= x0 &&
+        std::min(s[1], s[3]) <= y1 && std::max(s[1], s[3]) >= y0))
+    return false;
+  const double cx[4] = {x0, x1, x1, x0}, cy[4] = {y0, y0, y1, y1};
+  int pos = 0, neg = 0;
+  for (int k = 0; k < 4; k++) {
+    const int o = orient_sign(s, cx[k], cy[k]);
+    pos += o > 0;
+    neg += o < 0;
+  }
+  return pos < 4 && neg < 4;
+}
 
 // envelope.cuh:29-38 -- Envelope::Contains(point), closed on both ends.
 inline bool box_contains_point(const Box& b, float px, float py) {
@@ -76,6 +226,10 @@ inline bool box_intersects(const Box& a, const Box& o) {
 // evaluates the same closed form for every predicate code on points.
 inline bool eval(int pred, int kind, const Box& g, const float* q) {
   if (kind == Q_POINT) return box_contains_point(g, q[0], q[1]);
+  if (kind == Q_LINE) {
+    const double sd[4] = {q[0], q[1], q[2], q[3]};
+    return pred == P_INTERSECTS && segment_meets(sd, g.x0, g.y0, g.x1, g.y1);
+  }
   Box qb{q[0], q[1], q[2], q[3]};
   switch (pred) {
     case P_INTERSECTS: return box_intersects(g, qb);
@@ -94,6 +248,7 @@ struct Box64 {
 };
 inline bool eval64(int pred, int kind, const Box64& g, const double* q) {
   if (kind == Q_POINT) return q[0] >= g.x0 && q[0] <= g.x1 && q[1] >= g.y0 && q[1] <= g.y1;
+  if (kind == Q_LINE) return pred == P_INTERSECTS && segment_meets(q, g.x0, g.y0, g.x1, g.y1);
   switch (pred) {
     case P_INTERSECTS: return q[0] <= g.x1 && q[2] >= g.x0 && q[1] <= g.y1 && q[3] >= g.y0;
     case P_CONTAINS: return q[0] >= g.x0 && q[2] <= g.x1 && q[1] >= g.y0 && q[3] <= g.y1;

@@ -7,6 +7,8 @@ Array conventions (numpy):
   boxes    float32 [n, 4]  (x0, y0, x1, y1)  -- Envelope<Point<float,2>>,
            include/rtspatial/geom/envelope.cuh:13-109 (min corner then max corner)
   points   float32 [n, 2]  (x, y)            -- Point<float,2>, geom/point.cuh:11-42
+  segments float32 [n, 4]  (x1, y1, x2, y2)  -- Line<Point<float,2>>, geom/line.cuh:10-27;
+           pass lines=True (INTERSECTS only: CONTAINS and WITHIN match nothing)
   alive    uint8   [n] or None (deleted boxes are 0)
   pairs    uint32  [k, 2]  (query_id, box_id), sorted by (query_id, box_id)
 
@@ -23,7 +25,7 @@ import os
 import numpy as np
 
 INTERSECTS, CONTAINS, WITHIN = 0, 1, 2
-POINT, ENVELOPE = 0, 1
+POINT, ENVELOPE, LINE = 0, 1, 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -77,7 +79,10 @@ def _f32(a, cols):
     return a
 
 
-def _kind(queries):
+def _kind(queries, lines=False):
+    if lines:
+        assert queries.shape[1] == 4, queries.shape
+        return LINE
     return POINT if queries.shape[1] == 2 else ENVELOPE
 
 
@@ -115,14 +120,15 @@ def splitmix64(x):
     return lib().orc_splitmix64(int(x))
 
 
-def evaluate(pred, box, query):
+def evaluate(pred, box, query, lines=False):
     b = np.ascontiguousarray(box, np.float32)
     q = np.ascontiguousarray(query, np.float32)
-    return bool(lib().orc_eval(pred, POINT if q.size == 2 else ENVELOPE, _p(b), _p(q)))
+    kind = LINE if lines else (POINT if q.size == 2 else ENVELOPE)
+    return bool(lib().orc_eval(pred, kind, _p(b), _p(q)))
 
 
 # ------------------------------------------------------------ brute force --
-def bf_summary(pred, boxes, queries, alive=None, threads=0, want_hash=True):
+def bf_summary(pred, boxes, queries, alive=None, threads=0, want_hash=True, lines=False):
     """Total count, per-query counts and per-query (sum, xor) of
     splitmix64((query_id << 32) | box_id) over the matching boxes."""
     boxes = _f32(boxes, 4)
@@ -132,26 +138,26 @@ def bf_summary(pred, boxes, queries, alive=None, threads=0, want_hash=True):
     hs = np.zeros(nq, np.uint64) if want_hash else None
     hx = np.zeros(nq, np.uint64) if want_hash else None
     al = _alive(alive, boxes.shape[0])
-    total = lib().orc_bf_summary(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
+    total = lib().orc_bf_summary(pred, _kind(queries, lines), _p(boxes), boxes.shape[0], _p(al),
                                  _p(queries), nq, threads, _p(per_q), _p(hs), _p(hx))
     return int(total), per_q, hs, hx
 
 
-def bf_count(pred, boxes, queries, alive=None, threads=0):
+def bf_count(pred, boxes, queries, alive=None, threads=0, lines=False):
     boxes = _f32(boxes, 4)
     queries = np.ascontiguousarray(queries, dtype=np.float32)
     al = _alive(alive, boxes.shape[0])
-    return int(lib().orc_bf_summary(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
+    return int(lib().orc_bf_summary(pred, _kind(queries, lines), _p(boxes), boxes.shape[0], _p(al),
                                     _p(queries), queries.shape[0], threads, None, None, None))
 
 
-def bf_pairs(pred, boxes, queries, alive=None, threads=0):
+def bf_pairs(pred, boxes, queries, alive=None, threads=0, lines=False):
     boxes = _f32(boxes, 4)
     queries = np.ascontiguousarray(queries, dtype=np.float32)
     al = _alive(alive, boxes.shape[0])
-    total = bf_count(pred, boxes, queries, alive, threads)
+    total = bf_count(pred, boxes, queries, alive, threads, lines)
     out = np.empty((max(total, 1), 2), np.uint32)
-    got = lib().orc_bf_pairs(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
+    got = lib().orc_bf_pairs(pred, _kind(queries, lines), _p(boxes), boxes.shape[0], _p(al),
                              _p(queries), queries.shape[0], threads, _p(out), total)
     assert got == total
     return out[:total]
@@ -166,13 +172,14 @@ def _f64(a, cols):
     return a
 
 
-def evaluate64(pred, box, query):
+def evaluate64(pred, box, query, lines=False):
     b = np.ascontiguousarray(box, np.float64)
     q = np.ascontiguousarray(query, np.float64)
-    return bool(lib().orc_eval_f64(pred, POINT if q.size == 2 else ENVELOPE, _p(b), _p(q)))
+    kind = LINE if lines else (POINT if q.size == 2 else ENVELOPE)
+    return bool(lib().orc_eval_f64(pred, kind, _p(b), _p(q)))
 
 
-def bf_summary64(pred, boxes, queries, alive=None, threads=0, want_hash=True):
+def bf_summary64(pred, boxes, queries, alive=None, threads=0, want_hash=True, lines=False):
     boxes = _f64(boxes, 4)
     queries = np.ascontiguousarray(queries, dtype=np.float64)
     nq = queries.shape[0]
@@ -180,26 +187,68 @@ def bf_summary64(pred, boxes, queries, alive=None, threads=0, want_hash=True):
     hs = np.zeros(nq, np.uint64) if want_hash else None
     hx = np.zeros(nq, np.uint64) if want_hash else None
     al = _alive(alive, boxes.shape[0])
-    total = lib().orc_bf_summary_f64(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
+    total = lib().orc_bf_summary_f64(pred, _kind(queries, lines), _p(boxes), boxes.shape[0], _p(al),
                                      _p(queries), nq, threads, _p(per_q), _p(hs), _p(hx))
     return int(total), per_q, hs, hx
 
 
-def bf_count64(pred, boxes, queries, alive=None, threads=0):
-    return bf_summary64(pred, boxes, queries, alive, threads, want_hash=False)[0]
+def bf_count64(pred, boxes, queries, alive=None, threads=0, lines=False):
+    return bf_summary64(pred, boxes, queries, alive, threads, want_hash=False, lines=lines)[0]
 
 
-def bf_pairs64(pred, boxes, queries, alive=None, threads=0):
+def bf_pairs64(pred, boxes, queries, alive=None, threads=0, lines=False):
     """(query_id, box_id) pairs sorted by (query, box), exact double predicates."""
     boxes = _f64(boxes, 4)
     queries = np.ascontiguousarray(queries, dtype=np.float64)
     al = _alive(alive, boxes.shape[0])
-    total = bf_count64(pred, boxes, queries, alive, threads)
+    total = bf_count64(pred, boxes, queries, alive, threads, lines)
     out = np.empty((max(total, 1), 2), np.uint32)
-    got = lib().orc_bf_pairs_f64(pred, _kind(queries), _p(boxes), boxes.shape[0], _p(al),
+    got = lib().orc_bf_pairs_f64(pred, _kind(queries, lines), _p(boxes), boxes.shape[0], _p(al),
                                  _p(queries), queries.shape[0], threads, _p(out), total)
     assert got == total
     return out[:total]
+
+
+# ------------------------------------------------------- line segments --
+# Line<Point<float,2>> queries (geom/line.cuh:10-27; the reference's line
+# intersection program reports nothing, shaders_intersects_line_query_2d.cu:
+# 14-41). A closed segment (x1, y1, x2, y2) meets a closed box iff their
+# bounding boxes overlap and the box's corners are not all strictly on one
+# side of the segment's line; evaluated here in exact rational arithmetic
+# (finite coordinates; a segment with a NaN coordinate meets nothing).
+def segment_meets_box(seg, box):
+    from fractions import Fraction as F
+    if any(v != v for v in (float(c) for c in seg)):
+        return False  # a segment with a NaN coordinate meets nothing
+    x1, y1, x2, y2 = (F(float(v)) for v in seg)
+    bx0, by0, bx1, by1 = (float(v) for v in box)
+    if not (min(x1, x2) <= bx1 and max(x1, x2) >= bx0 and min(y1, y2) <= by1
+            and max(y1, y2) >= by0):
+        return False  # also for a NaN box: every comparison is false
+    dx, dy = x2 - x1, y2 - y1
+    sides = set()
+    for cx, cy in ((bx0, by0), (bx1, by0), (bx1, by1), (bx0, by1)):
+        o = dx * (F(cy) - y1) - dy * (F(cx) - x1)
+        sides.add((o > 0) - (o < 0))
+    return sides != {1} and sides != {-1}
+
+
+def segment_pairs(boxes, segs, alive=None):
+    """(query_id, box_id) pairs, sorted, of segments meeting boxes: numpy
+    bounding-box prefilter, exact test on the candidates (small inputs)."""
+    b = np.asarray(boxes)
+    sg = np.asarray(segs)
+    ok = np.ones(b.shape[0], bool) if alive is None else np.asarray(alive).astype(bool)
+    out = []
+    for qi, seg in enumerate(sg):
+        sx0, sx1 = min(seg[0], seg[2]), max(seg[0], seg[2])
+        sy0, sy1 = min(seg[1], seg[3]), max(seg[1], seg[3])
+        cand = np.nonzero(ok & (b[:, 0] <= sx1) & (b[:, 2] >= sx0) & (b[:, 1] <= sy1)
+                          & (b[:, 3] >= sy0))[0]
+        for g in cand:
+            if segment_meets_box(seg, b[g]):
+                out.append((qi, g))
+    return np.asarray(out, dtype=np.uint32).reshape(-1, 2)
 
 
 # ----------------------------------------------------------------- R-tree --

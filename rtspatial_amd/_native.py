@@ -16,6 +16,7 @@ RTS_HANDLER_COUNT, RTS_HANDLER_COLLECT = 0, 1
 RTS_REBUILD_SAH, RTS_REBUILD_NEVER, RTS_REBUILD_ALWAYS = 0, 1, 2
 RTS_TRAVERSAL_AUTO, RTS_TRAVERSAL_THREAD, RTS_TRAVERSAL_WAVE = 0, 1, 2
 RTS_COORD_F32, RTS_COORD_F64, RTS_COORD_F64_AS_F32 = 0, 1, 2
+RTS_QUERY_ENVELOPES, RTS_QUERY_POINTS, RTS_QUERY_LINES = 0, 1, 2
 
 
 class rts_config(ctypes.Structure):
@@ -202,6 +203,8 @@ SIGNATURES = [
     ("rts_index_update_d", _I32, [_VP, _VP, _U64, _VP]),
     ("rts_index_query_points_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_query_envelopes_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_index_query_lines", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_index_query_lines_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_size", _U64, [_VP]),
     ("rts_index_tree_view", _I32, [_VP, _VP, ctypes.POINTER(rts_tree_view)]),
     ("rts_index_query_plan", _I32, [_VP, _I32, _I32, _U64, ctypes.POINTER(rts_query_plan)]),

@@ -651,7 +651,10 @@ struct Plan {
 // descent; neighbouring waves, which take consecutive queries on one XCD,
 // share that XCD's L2).
 Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
-  const rts_index::TunedPlan& t = ix->tuned[pred][points ? 1 : 0];
+  // Segment queries (kPredLine) are not tuned: they take the config or the defaults.
+  static const rts_index::TunedPlan kUntuned{};
+  const rts_index::TunedPlan& t =
+      pred == kPredLine ? kUntuned : ix->tuned[pred][points ? 1 : 0];
   Plan p;
   if (ix->cfg.traversal == RTS_TRAVERSAL_THREAD || ix->cfg.traversal == RTS_TRAVERSAL_WAVE)
     p.traversal = (int)ix->cfg.traversal;
@@ -669,7 +672,7 @@ Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
   return p;
 }
 
-// Bytes of one query: float or double point or envelope.
+// Bytes of one query: float or double point, or envelope / segment.
 size_t query_bytes(bool points, bool f64) { return (points ? 8 : 16) * (f64 ? 2 : 1); }
 
 // Runs one query call under `plan`; the caller holds the index lock and has
@@ -678,7 +681,12 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
              bool points, bool f64, rts_handler handler, void* arg, hipStream_t stream) {
   const bool sort = plan.sort;
   const int trav = plan.traversal;
-  const bool two_pass = trav == RTS_TRAVERSAL_WAVE && handler == RTS_HANDLER_COLLECT;
+  // Collecting wave walks count, scan and then write each query's pairs
+  // contiguously; segment walks, whose count pass would cost as much as the
+  // write (no subtree lies inside a segment), collect in one staged pass.
+  const bool staged =
+      trav == RTS_TRAVERSAL_WAVE && handler == RTS_HANDLER_COLLECT && pred == kPredLine;
+  const bool two_pass = trav == RTS_TRAVERSAL_WAVE && handler == RTS_HANDLER_COLLECT && !staged;
   // At most 2^26 queries per launch: one wavefront per query keeps
   // grid x workgroup size below 2^32 work-items.
   const uint64_t kChunk = 1ull << 26;
@@ -722,7 +730,7 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
       } else if (!two_pass) {
         WaveArgs wa{nullptr, nullptr, nullptr, qs.tickets};
         PhaseToken ph = phase_begin(RTS_PHASE_TRAVERSE, stream);
-        RTS_HIP(launch_query_wave(ql, wa, kPhaseCountSink, stream));
+        RTS_HIP(launch_query_wave(ql, wa, staged ? kPhaseWriteStaged : kPhaseCountSink, stream));
         phase_end(ph, stream);
       } else {
         WaveArgs wa{qs.counts, qs.offsets, qs.total + 1, qs.tickets};
@@ -746,7 +754,8 @@ int check_query_args(int pred, const void* queries, uint64_t nq, bool points, rt
   RTS_CHECK_ARG(handler == RTS_HANDLER_COUNT || handler == RTS_HANDLER_COLLECT,
                 "query: unknown handler");
   RTS_CHECK_ARG(arg, "query: null handler argument");
-  RTS_CHECK_ARG(pred >= RTS_INTERSECTS && pred <= RTS_WITHIN, "query: unknown predicate");
+  RTS_CHECK_ARG((pred >= RTS_INTERSECTS && pred <= RTS_WITHIN) || pred == kPredLine,
+                "query: unknown predicate");
   RTS_CHECK_ARG(!points || pred == RTS_CONTAINS,
                 "query: point queries support only RTS_CONTAINS (spatial_index.cuh:385-389)");
   RTS_CHECK_ARG(nq == 0 || queries, "query: null queries");
@@ -799,6 +808,24 @@ int run_query(rts_index* ix, int pred, const void* queries, uint64_t nq, bool po
                   stream);
 }
 
+// Segment queries (spatial_index.cuh:517-547): RTS_INTERSECTS runs the
+// segment walk; RTS_CONTAINS reports nothing, as the reference's (its
+// kContains case does no work, :524-526).
+int run_lines(rts_index* ix, int pred, const void* queries, uint64_t nq, bool doubles,
+              rts_handler handler, void* arg, hipStream_t stream) {
+  RTS_CHECK_ARG(pred == RTS_INTERSECTS || pred == RTS_CONTAINS,
+                "query_lines: segment queries take RTS_INTERSECTS or RTS_CONTAINS "
+                "(spatial_index.cuh:523-545)");
+  if (pred == RTS_INTERSECTS)
+    return run_query(ix, kPredLine, queries, nq, false, doubles, handler, arg, stream);
+  int rc = check_query_args(RTS_CONTAINS, queries, nq, false, handler, arg);
+  if (rc) return rc;
+  RTS_CHECK_ARG(doubles == ix->f64(),
+                doubles ? "query: a float index takes rts_index_query_lines"
+                        : "query: a double index takes rts_index_query_lines_d");
+  return RTS_OK;
+}
+
 // Device buffers of one tuning run, released on every exit path.
 struct TuneBuffers {
   void* sample = nullptr;
@@ -842,6 +869,18 @@ int rts_index_query_envelopes_d(rts_index* ix, rts_predicate pred, const rts_env
   return run_query(ix, pred, q, n, false, true, handler, arg, stream);
 }
 
+int rts_index_query_lines(rts_index* ix, rts_predicate pred, const rts_line* q, uint64_t n,
+                          rts_handler handler, void* arg, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_query_lines: null index");
+  return run_lines(ix, pred, q, n, false, handler, arg, stream);
+}
+
+int rts_index_query_lines_d(rts_index* ix, rts_predicate pred, const rts_line_d* q, uint64_t n,
+                            rts_handler handler, void* arg, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_query_lines_d: null index");
+  return run_lines(ix, pred, q, n, true, handler, arg, stream);
+}
+
 uint64_t rts_index_size(const rts_index* ix) { return ix ? ix->n : 0; }
 
 int rts_index_query_plan(rts_index* ix, rts_predicate pred, int points, uint64_t n,
@@ -849,8 +888,11 @@ int rts_index_query_plan(rts_index* ix, rts_predicate pred, int points, uint64_t
   RTS_CHECK_ARG(ix, "rts_index_query_plan: null index");
   RTS_CHECK_ARG(out, "rts_index_query_plan: null plan");
   RTS_CHECK_ARG(pred >= RTS_INTERSECTS && pred <= RTS_WITHIN, "rts_index_query_plan: unknown predicate");
+  RTS_CHECK_ARG(points >= RTS_QUERY_ENVELOPES && points <= RTS_QUERY_LINES,
+                "rts_index_query_plan: unknown query kind");
   std::lock_guard<std::mutex> lk(ix->mu);
-  const Plan p = plan_query(ix, pred, points != 0, n);
+  const Plan p = points == RTS_QUERY_LINES ? plan_query(ix, kPredLine, false, n)
+                                           : plan_query(ix, pred, points == RTS_QUERY_POINTS, n);
   out->traversal = (uint32_t)p.traversal;
   out->sort_queries = p.sort ? 1u : 2u;
   out->leaf_size = p.leaf_size;
@@ -862,6 +904,8 @@ int rts_index_order_queries(rts_index* ix, const void* queries, uint64_t n, int 
                             uint32_t* order, rts_stream_t stream) {
   RTS_CHECK_ARG(ix, "rts_index_order_queries: null index");
   RTS_CHECK_ARG(n < (1ull << 32), "rts_index_order_queries: n must be below 2^32");
+  RTS_CHECK_ARG(points >= RTS_QUERY_ENVELOPES && points <= RTS_QUERY_LINES,
+                "rts_index_order_queries: unknown query kind");
   if (n == 0) return RTS_OK;
   RTS_CHECK_ARG(queries && order, "rts_index_order_queries: null array");
   std::lock_guard<std::mutex> lk(ix->mu);
@@ -871,8 +915,10 @@ int rts_index_order_queries(rts_index* ix, const void* queries, uint64_t n, int 
   RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&block), keys_bytes + sort_scratch_bytes(n),
                          stream));
   const uint32_t* perm = nullptr;
-  int rc = order_queries(ix, queries, n, points != 0, ix->f64(), block, block + keys_bytes, stream,
-                         &perm);
+  // Envelopes and segments share a layout; a segment is ordered by the
+  // centre of its bounding box.
+  int rc = order_queries(ix, queries, n, points == RTS_QUERY_POINTS, ix->f64(), block,
+                         block + keys_bytes, stream, &perm);
   if (!rc) RTS_HIP(hipMemcpyAsync(order, perm, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
   RTS_HIP(hipFreeAsync(block, stream));
   return rc;

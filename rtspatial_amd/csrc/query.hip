@@ -1,8 +1,10 @@
-// LBVH traversal kernels: point and envelope queries with exact FP32 (or,
-// on a double index, exact FP64) predicates, replacing the reference's OptiX raygen/intersection programs
+// LBVH traversal kernels: point, envelope and line-segment queries with exact
+// FP32 (or, on a double index, exact FP64) predicates, replacing the
+// reference's OptiX raygen/intersection programs
 // (src/shaders/shaders_contains_point_query_2d.cu:14-49,
 //  shaders_contains_envelope_query_2d.cu:14-51,
-//  shaders_intersects_envelope_query_2d.cu:14-128).
+//  shaders_intersects_envelope_query_2d.cu:14-128,
+//  shaders_intersects_line_query_2d.cu:14-70).
 //
 // Node tests are necessary conditions of the leaf predicate on exact FP32
 // bounds, so pruning never drops a true pair; every reported pair passed the
@@ -65,7 +67,7 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
   const uint32_t qi = working ? (q.order ? q.order[i] : i) : 0;
   const uint32_t qid = qi + q.qid_base;
 
-  typename QueryOf<F64>::type qb{};
+  query_t<PRED, F64> qb{};
   if (working) load_q(q.queries, POINTS, qi, qb);
 
   uint32_t stack[kStackDepth];
@@ -187,24 +189,68 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
 // query's result count to counts[query]; kWaveCountSink adds it to the
 // counting handler's counter; kWaveWrite writes the pairs at queue slot
 // base + offsets[query] (offsets from the scan of the counts), so the queue
-// holds each query's pairs contiguously, in query order.
-enum { kWaveCount = 0, kWaveCountSink = 1, kWaveWrite = 2 };
+// holds each query's pairs contiguously, in query order. kWaveStaged
+// collects in one pass, for walks whose count pass would cost as much as
+// the write (segments: no subtree lies inside a segment): each wavefront
+// stages its pairs in LDS and flushes them with one device atomic per
+// kStageWave pairs; a query's pairs may then be split over flushes.
+enum { kWaveCount = 0, kWaveCountSink = 1, kWaveWrite = 2, kWaveStaged = 3 };
+constexpr uint32_t kStageWave = kScanStep;  // one scan step's results always fit
+
+struct StagedEmit {
+  rts_pair* stage;  // LDS, kStageWave pairs
+  unsigned long long* counter;
+  rts_pair* out;
+  uint64_t cap;
+  uint32_t fill = 0;  // wave-uniform: pairs staged
+  int64_t off = 0;    // result k of the current step goes to stage[off + k]
+  __device__ void flush() {
+    if (fill) {
+      lds_fence();
+      flush_stage(stage, fill, counter, out, cap);
+      fill = 0;
+    }
+  }
+  __device__ void begin_step(uint64_t first_k, uint32_t n) {
+    if (fill + n > kStageWave) flush();
+    off = (int64_t)fill - (int64_t)first_k;
+    fill += n;
+  }
+  __device__ void operator()(uint64_t k, uint32_t geom_id, uint32_t query_id) const {
+    rts_pair pr;
+    pr.geom_id = geom_id;
+    pr.query_id = query_id;
+    stage[(uint32_t)(off + (int64_t)k)] = pr;
+  }
+};
 
 // One wavefront per workgroup, so that a wave's LDS is released by itself.
 template <int PRED, bool POINTS, int MODE, bool F64>
 __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const WaveArgs a) {
   __shared__ WaveLds s_lds;
+  __shared__ rts_pair s_stage[MODE == kWaveStaged ? kStageWave : 1];
   const TreeView t = q.tree;
   if (t.n == 0) return;
+  StagedEmit se{s_stage, nullptr, nullptr, 0};
+  if (MODE == kWaveStaged) {
+    const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
+    se.counter = dq->counter;
+    se.out = dq->data;
+    se.cap = dq->capacity;
+  }
   uint32_t slice = blockIdx.x % kXcds;
   // Every iteration takes a query or finds none left: q.n + 1 iterations
   // bound the loop.
   for (uint32_t it = 0; it <= q.n; it++) {
     const uint32_t lin = grab_query(a.tickets, q.n, slice);
-    if (lin >= q.n) return;
+    if (lin >= q.n) break;
     const uint32_t qi = q.order ? q.order[lin] : lin;
-    typename QueryOf<F64>::type qb;
+    query_t<PRED, F64> qb;
     load_q(q.queries, POINTS, qi, qb);
+    if (MODE == kWaveStaged) {
+      wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base, se, s_lds);
+      continue;
+    }
     if (MODE == kWaveWrite) {
       const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
       wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base,
@@ -222,6 +268,7 @@ __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const
       }
     }
   }
+  if (MODE == kWaveStaged) se.flush();
 }
 
 // Persistent grid: 32 single-wave workgroups per CU, the most waves a CU
@@ -256,6 +303,8 @@ hipError_t launch_wave_f(const QueryLaunch& q, const WaveArgs& a, int phase, hip
       return launch_wave_mode<PRED, POINTS, kWaveCount, F64>(q, a, stream);
     case kWaveCountSink:
       return launch_wave_mode<PRED, POINTS, kWaveCountSink, F64>(q, a, stream);
+    case kWaveStaged:
+      return launch_wave_mode<PRED, POINTS, kWaveStaged, F64>(q, a, stream);
     default:
       return launch_wave_mode<PRED, POINTS, kWaveWrite, F64>(q, a, stream);
   }
@@ -292,6 +341,7 @@ hipError_t launch_query(const QueryLaunch& q, hipStream_t stream) {
     case RTS_INTERSECTS: return launch_pred<RTS_INTERSECTS, false>(q, stream);
     case RTS_CONTAINS: return launch_pred<RTS_CONTAINS, false>(q, stream);
     case RTS_WITHIN: return launch_pred<RTS_WITHIN, false>(q, stream);
+    case kPredLine: return launch_pred<kPredLine, false>(q, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -304,6 +354,7 @@ hipError_t launch_query_wave(const QueryLaunch& q, const WaveArgs& a, int phase,
     case RTS_INTERSECTS: return launch_wave<RTS_INTERSECTS, false>(q, a, phase, stream);
     case RTS_CONTAINS: return launch_wave<RTS_CONTAINS, false>(q, a, phase, stream);
     case RTS_WITHIN: return launch_wave<RTS_WITHIN, false>(q, a, phase, stream);
+    case kPredLine: return launch_wave<kPredLine, false>(q, a, phase, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -112,11 +112,11 @@ hipError_t launch_morton64(const void* queries, bool points, uint32_t n, const f
 // query.hip
 struct QueryLaunch {
   TreeView tree;
-  const void* queries;   // float2 points or float4 envelopes
+  const void* queries;   // float2 points, float4 envelopes or segments (double when f64)
   uint32_t n;
   uint32_t qid_base;     // added to the query index when reporting
   const uint32_t* order; // optional permutation: lane i works on queries[order[i]]
-  int pred;              // rts_predicate
+  int pred;              // rts_predicate, or kPredLine for segment queries
   int point_queries;     // 1 = points
   int f64;               // 1 = double queries (double2 / Box64), exact double predicates
   SinkArgs sink;
@@ -125,7 +125,8 @@ struct QueryLaunch {
 hipError_t launch_query(const QueryLaunch& q, hipStream_t stream);
 // Wave-per-query traversal. phase 0: per-query counts -> a.counts;
 // 1: counts added to the counting handler's counter; 2: pairs written at
-// queue slot a.base[0] + a.offsets[query].
+// queue slot a.base[0] + a.offsets[query]; 3: pairs collected in one pass
+// (staged per wavefront, reserved in the queue by atomics).
 struct WaveArgs {
   uint32_t* counts;
   const uint64_t* offsets;
@@ -133,7 +134,7 @@ struct WaveArgs {
   uint32_t* tickets;  // kWaveTicketWords words, cleared by the launch
 };
 constexpr uint32_t kWaveTicketWords = 8 * 16;
-enum { kPhaseCount = 0, kPhaseCountSink = 1, kPhaseWrite = 2 };
+enum { kPhaseCount = 0, kPhaseCountSink = 1, kPhaseWrite = 2, kPhaseWriteStaged = 3 };
 hipError_t launch_query_wave(const QueryLaunch& q, const WaveArgs& a, int phase,
                              hipStream_t stream);
 

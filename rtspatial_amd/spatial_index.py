@@ -82,6 +82,14 @@ def _stream_handle(stream):
     return ctypes.c_void_p(int(stream))
 
 
+def _handler_arg(arg, what):
+    if isinstance(arg, SharedValue):
+        return N.RTS_HANDLER_COUNT, arg.data()
+    if isinstance(arg, Queue):
+        return N.RTS_HANDLER_COLLECT, arg.DeviceObject()
+    raise TypeError(f"{what}: arg must be a SharedValue or a Queue")
+
+
 def _check_array(t, cols, dtype, what):
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{what}: expected a torch tensor")
@@ -259,12 +267,7 @@ class SpatialIndex:
         _check_array(queries, None, self.coord_t, "Query")
         if queries.dim() != 2 or queries.shape[1] not in (2, 4):
             raise ValueError("Query: queries must be [n, 2] points or [n, 4] envelopes")
-        if isinstance(arg, SharedValue):
-            handler, ptr = N.RTS_HANDLER_COUNT, arg.data()
-        elif isinstance(arg, Queue):
-            handler, ptr = N.RTS_HANDLER_COLLECT, arg.DeviceObject()
-        else:
-            raise TypeError("Query: arg must be a SharedValue or a Queue")
+        handler, ptr = _handler_arg(arg, "Query")
         L = N.lib()
         if self._d:
             fn = L.rts_index_query_points_d if queries.shape[1] == 2 else L.rts_index_query_envelopes_d
@@ -272,6 +275,18 @@ class SpatialIndex:
             fn = L.rts_index_query_points if queries.shape[1] == 2 else L.rts_index_query_envelopes
         N.check(fn(self._handle(), int(predicate), queries.data_ptr(), queries.shape[0], handler,
                    ptr, _stream_handle(stream)), "Query")
+
+    def QueryLines(self, predicate, segments, arg, stream=None):
+        """spatial_index.cuh:517-547: Query(p, ArrayView<line_t>, arg) for
+        segments [n, 4] = (x1, y1, x2, y2). INTERSECTS reports every data box
+        the closed segment meets (exact); CONTAINS reports nothing, as the
+        reference's; WITHIN is an error."""
+        _check_array(segments, 4, self.coord_t, "QueryLines")
+        handler, ptr = _handler_arg(arg, "QueryLines")
+        L = N.lib()
+        fn = L.rts_index_query_lines_d if self._d else L.rts_index_query_lines
+        N.check(fn(self._handle(), int(predicate), segments.data_ptr(), segments.shape[0], handler,
+                   ptr, _stream_handle(stream)), "QueryLines")
 
     # -- extras ----------------------------------------------------------------
     def size(self):

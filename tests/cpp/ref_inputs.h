@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "rtspatial/geom/envelope.h"
+#include "rtspatial/geom/line.h"
 #include "rtspatial/geom/point.h"
 
 namespace ref_inputs {
@@ -35,6 +36,15 @@ inline std::vector<point_t> uniform_points(size_t n) {
     float x = gx(mt), y = gy(mt);
     out[i] = point_t(x, y);
   }
+  return out;
+}
+
+// GenerateUniformLines (test/test_commons.h:69-97): the same stream as
+// uniform_boxes, each segment the diagonal from (x, y) to the clipped corner.
+inline std::vector<rtspatial::Line<point_t>> uniform_lines(size_t n, float max_w, float max_h) {
+  std::vector<rtspatial::Line<point_t>> out;
+  for (const envelope_t& e : uniform_boxes(n, max_w, max_h))
+    out.push_back(rtspatial::Line<point_t>(e.get_min(), e.get_max()));
   return out;
 }
 

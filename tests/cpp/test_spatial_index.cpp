@@ -288,6 +288,55 @@ TEST(PointQueries, bad_predicate_throws) {
   ASSERT_TRUE(threw);
 }
 
+// test/test_line_queries.h:9-29 (100K boxes, 1000 lines; the reference's
+// call omits the line generator's sizes, taken here as its envelope tests'
+// 0.1 x 0.1). The reference's line program reports nothing; 6370306 is the
+// exact count of box-segment contacts, from the oracle's exact join.
+TEST(LineQueries, fp32_intersects) {
+  using line_t = index_t::line_t;
+  device_vector<envelope_t> boxes(uniform_boxes(100000, 0.5, 0.5));
+  device_vector<line_t> lines(ref_inputs::uniform_lines(1000, 0.1, 0.1));
+  Stream s;
+  index_t index;
+  index.Init(Config());
+  index.Insert(ArrayView<envelope_t>(boxes), s.cuda_stream());
+  ASSERT_EQ(count(index, Predicate::kIntersects, lines, s), 6370306ull);
+  ASSERT_EQ(count(index, Predicate::kContains, lines, s), 0ull);  // spatial_index.cuh:524-526
+  bool threw = false;
+  try {
+    count(index, Predicate::kWithin, lines, s);
+  } catch (const Exception&) {
+    threw = true;
+  }
+  ASSERT_TRUE(threw);
+}
+
+// Closed-form cases on a double index: the unit box against segments that
+// cross it, touch a corner, run along an edge, or pass beside it.
+TEST(LineQueries, fp64_closed_form) {
+  using dindex_t = SpatialIndex<double, 2>;
+  using dpoint_t = dindex_t::point_t;
+  using denv_t = dindex_t::envelope_t;
+  using dline_t = dindex_t::line_t;
+  pinned_vector<denv_t> boxes;
+  boxes.push_back(denv_t(dpoint_t(0, 0), dpoint_t(1, 1)));
+  pinned_vector<dline_t> lines;
+  lines.push_back(dline_t(dpoint_t(-1, -1), dpoint_t(2, 2)));        // crosses
+  lines.push_back(dline_t(dpoint_t(0, 2), dpoint_t(2, 0)));          // touches (1, 1)
+  lines.push_back(dline_t(dpoint_t(1, -3), dpoint_t(1, 3)));         // along the right edge
+  lines.push_back(dline_t(dpoint_t(1, 2), dpoint_t(2, 1)));          // beside the corner
+  lines.push_back(dline_t(dpoint_t(0, 2.0000000001), dpoint_t(2.0000000001, 0)));  // 1e-10 off
+  lines.push_back(dline_t(dpoint_t(0.5, 0.5), dpoint_t(0.5, 0.5)));  // a point inside
+  Stream s;
+  dindex_t index;
+  index.Init(Config());
+  index.Insert(ArrayView<denv_t>(boxes), s.cuda_stream());
+  counter().set(s.cuda_stream(), 0);
+  index.Query(Predicate::kIntersects, ArrayView<dline_t>(lines), counter().data(),
+              s.cuda_stream());
+  ASSERT_EQ(counter().get(s.cuda_stream()), 4ull);
+}
+
 int main(int argc, char** argv) {
   int rc;
   {

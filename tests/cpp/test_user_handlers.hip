@@ -148,6 +148,22 @@ TEST(UserHandlers, reference_functions_fp64_known_answer) {
   ASSERT_EQ(ref_count(index, Predicate::kContains, ArrayView<dpoint_t>(points), s), 5ull);
 }
 
+// Line segments (test/test_line_queries.h:9-29 sizes): the reference's
+// handler function and handler functors against the built-in counting
+// handler; kContains has no results but done() still reports each query.
+TEST(UserHandlers, lines_known_answer) {
+  using line_t = index_t::line_t;
+  device_vector<envelope_t> boxes(uniform_boxes(100000, 0.5, 0.5));
+  device_vector<line_t> lines(ref_inputs::uniform_lines(1000, 0.1, 0.1));
+  Stream s;
+  index_t index;
+  index.Init(Config());
+  index.Insert(ArrayView<envelope_t>(boxes), s.cuda_stream());
+  ASSERT_EQ(ref_count(index, Predicate::kIntersects, ArrayView<line_t>(lines), s), 6370306ull);
+  check_functors(index, Predicate::kIntersects, ArrayView<line_t>(lines), 6370306ull, s);
+  check_functors(index, Predicate::kContains, ArrayView<line_t>(lines), 0ull, s);
+}
+
 // An empty index: done() still reports every query, with 0 results.
 TEST(UserHandlers, empty_index_reports_zero_per_query) {
   device_vector<point_t> queries(uniform_points(100));

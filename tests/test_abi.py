@@ -66,7 +66,8 @@ int main(void) {
   S(rts_update) S(rts_tune_result) S(rts_serve_config) S(rts_request) S(rts_request_result)
   S(rts_index_stats) F(rts_index_stats, delta_boxes) F(rts_config, insert_delta_pct)
   S(rts_tree_view) F(rts_tree_view, n) F(rts_tree_view, coord) S(rts_query_plan)
-  S(rts_memory_usage)
+  S(rts_memory_usage) S(rts_line) S(rts_line_d) F(rts_line, p2) F(rts_line_d, p2)
+  printf("RTS_QUERY_LINES %d\n", (int)RTS_QUERY_LINES);
   return 0;
 }
 """
@@ -95,6 +96,9 @@ def test_ctypes_structs_match_the_header(tmp_path):
     assert int(got["rts_index_stats.delta_boxes"]) == N.rts_index_stats.delta_boxes.offset == 52
     assert int(got["rts_config.insert_delta_pct"]) == N.rts_config.insert_delta_pct.offset == 80
     assert int(got["rts_tree_view.coord"]) == N.rts_tree_view.coord.offset
+    assert int(got["rts_line"]) == 16 and int(got["rts_line.p2"]) == 8
+    assert int(got["rts_line_d"]) == 32 and int(got["rts_line_d.p2"]) == 16
+    assert int(got["RTS_QUERY_LINES"]) == N.RTS_QUERY_LINES == 2
     for name in ("rts_tune_result", "rts_serve_config", "rts_request", "rts_request_result",
                  "rts_tree_view", "rts_query_plan", "rts_memory_usage"):
         assert int(got[name]) == ctypes.sizeof(getattr(N, name)), name

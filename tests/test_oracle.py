@@ -304,3 +304,150 @@ def test_f64_matches_numpy_restatement(oracle):
     within = (G[..., 0] >= Q[..., 0]) & (G[..., 2] <= Q[..., 2]) & (G[..., 1] >= Q[..., 1]) & (G[..., 3] <= Q[..., 3])
     for pred, m in ((oracle.INTERSECTS, inter), (oracle.CONTAINS, cont), (oracle.WITHIN, within)):
         assert np.array_equal(oracle.bf_pairs64(pred, g, q), np.argwhere(m).astype(np.uint32))
+
+
+# ------------------------------------------------------------ line segments --
+def _liang_barsky(seg, box):
+    # An independent statement: clip the segment's parameter range [0, 1] by
+    # the four slabs in exact rationals; it meets the box iff a range remains.
+    from fractions import Fraction as F
+    x1, y1, x2, y2 = (F(float(v)) for v in seg)
+    bx0, by0, bx1, by1 = (F(float(v)) for v in box)
+    t0, t1 = F(0), F(1)
+    for p, q in ((-(x2 - x1), x1 - bx0), (x2 - x1, bx1 - x1), (-(y2 - y1), y1 - by0),
+                 (y2 - y1, by1 - y1)):
+        if p == 0:
+            if q < 0:
+                return False
+        else:
+            r = q / p
+            if p < 0:
+                t0 = max(t0, r)
+            else:
+                t1 = min(t1, r)
+            if t0 > t1:
+                return False
+    return True
+
+
+@pytest.mark.parametrize("seg,box,want", [
+    ([0, 0, 1, 1], [0.4, 0.4, 0.6, 0.6], True),     # through the middle
+    ([0, 0, 1, 1], [0.6, 0.0, 0.8, 0.3], False),    # beside the diagonal
+    ([0, 0, 1, 1], [1, 1, 2, 2], True),              # touches a corner
+    ([0, 0, 1, 0], [0.5, 0, 0.6, 1], True),          # along an edge
+    ([0, 0, 1, 0], [0.5, 1e-30, 0.6, 1], False),     # just above it
+    ([0.5, 0.5, 0.5, 0.5], [0, 0, 1, 1], True),      # a point inside
+    ([2, 2, 2, 2], [0, 0, 1, 1], False),             # a point outside
+    ([0, 1, 1, 0], [0.5, 0.5, 0.7, 0.7], True),      # anti-diagonal through the corner
+    ([0, 1, 1, 0], [np.nextafter(0.5, 1), np.nextafter(0.5, 1), 0.7, 0.7], False),
+    ([0, 0, 1, 1], [np.nan] * 4, False),
+])
+def test_segment_closed_form(oracle, seg, box, want):
+    assert oracle.segment_meets_box(seg, box) == want
+    if not np.isnan(box[0]):
+        assert _liang_barsky(seg, box) == want
+
+
+def test_segment_predicate_matches_liang_barsky(oracle):
+    # Random segments and boxes on a coarse grid (many touching and collinear
+    # cases) and near-touching float cases: two exact methods agree.
+    rng = np.random.default_rng(17)
+    for _ in range(3000):
+        s = rng.integers(0, 9, 4) / 8
+        b = np.sort(rng.integers(0, 9, (2, 2)), axis=0).T.reshape(-1)[[0, 2, 1, 3]] / 8
+        assert oracle.segment_meets_box(s, b) == _liang_barsky(s, b), (s, b)
+    for _ in range(2000):
+        s = rng.random(4).astype(np.float32)
+        lo = rng.random(2).astype(np.float32)
+        b = np.concatenate([lo, lo + rng.random(2).astype(np.float32) * np.float32(0.05)])
+        if rng.random() < 0.5:  # put a corner on the segment's line, nudged by an ulp
+            t = np.float32(rng.random())
+            b[0] = s[0] + t * (s[2] - s[0])
+            b[1] = np.nextafter(s[1] + t * (s[3] - s[1]), np.float32(rng.choice([-1, 1])))
+            b[2], b[3] = max(b[0], b[2]), max(b[1], b[3])
+        assert oracle.segment_meets_box(s, b) == _liang_barsky(s, b), (s, b)
+
+
+def test_segment_pairs_prefilter(oracle):
+    rng = np.random.default_rng(18)
+    b = oracle.ref_uniform_boxes(2000, 0.05, 0.05)
+    segs = rng.random((60, 4)).astype(np.float32)
+    segs[:, 2:] = segs[:, :2] + (segs[:, 2:] - 0.5) * 0.2
+    got = oracle.segment_pairs(b, segs)
+    want = [(q, g) for q in range(segs.shape[0]) for g in range(b.shape[0])
+            if _liang_barsky(segs[q], b[g])]
+    assert got.tolist() == [list(p) for p in want]
+
+
+def _adversarial_segments(rng, n, dtype):
+    """Segments and boxes where corners sit on, or an ulp off, the segment's
+    line, across wide exponent ranges."""
+    out = []
+    for _ in range(n):
+        kind = rng.integers(0, 4)
+        s = rng.random(4).astype(dtype)
+        if kind == 1:  # tiny coordinates
+            s = (s * dtype(10.0) ** -rng.integers(5, 30)).astype(dtype)
+        t = dtype(rng.random())
+        px = s[0] + t * (s[2] - s[0])
+        py = s[1] + t * (s[3] - s[1])
+        if kind == 2:  # a corner one ulp beside the line
+            py = np.nextafter(py, dtype(rng.choice([-1.0, 1.0])))
+        w = dtype(rng.random() * 0.1) * (dtype(1e-20) if kind == 1 else dtype(1))
+        corner = rng.integers(0, 4)
+        x0 = px - (w if corner in (1, 2) else 0)
+        y0 = py - (w if corner in (2, 3) else 0)
+        b = np.array([x0, y0, x0 + w, y0 + w], dtype)
+        if kind == 3:  # degenerate segment / degenerate box
+            s[2:] = s[:2] if rng.random() < 0.5 else s[2:]
+            b[2:] = b[:2] if rng.random() < 0.5 else b[2:]
+        out.append((s, b))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_c_segment_predicate_matches_rational(oracle, dtype):
+    # The C oracle's line predicate (double filter + integer arithmetic)
+    # against the exact rational one.
+    rng = np.random.default_rng(19 if dtype == np.float32 else 20)
+    ev = oracle.evaluate if dtype == np.float32 else oracle.evaluate64
+    cases = _adversarial_segments(rng, 3000, dtype)
+    hits = 0
+    for s, b in cases:
+        want = oracle.segment_meets_box(s, b)
+        assert ev(oracle.INTERSECTS, b, s, lines=True) == want, (s, b)
+        assert not ev(oracle.CONTAINS, b, s, lines=True)
+        hits += want
+    assert 0.2 * len(cases) < hits < 0.95 * len(cases)  # both answers well represented
+
+
+def test_c_segment_join_matches_rational(oracle):
+    rng = np.random.default_rng(21)
+    b = oracle.ref_uniform_boxes(3000, 0.05, 0.05)
+    segs = rng.random((80, 4)).astype(np.float32)
+    segs[:, 2:] = segs[:, :2] + (segs[:, 2:] - 0.5) * 0.3
+    segs[0] = [0.5, 0.5, 0.5, 0.5]
+    segs[1] = [np.nan, 0.1, 0.2, 0.3]
+    segs[2] = b[7, [0, 1, 2, 3]]  # a box's diagonal
+    segs[3] = [b[9, 0], 0.0, b[9, 0], 1.0]  # along a box's left edge
+    alive = np.ones(3000, np.uint8)
+    alive[::7] = 0
+    got = oracle.bf_pairs(oracle.INTERSECTS, b, segs, alive=alive, lines=True)
+    assert got.tolist() == oracle.segment_pairs(b, segs, alive).tolist()
+    assert oracle.bf_count(oracle.CONTAINS, b, segs, lines=True) == 0
+    got64 = oracle.bf_pairs64(oracle.INTERSECTS, b.astype(np.float64), segs.astype(np.float64),
+                              alive=alive, lines=True)
+    assert got64.tolist() == got.tolist()  # float-exact inputs: same answer in double
+
+
+def test_c_segment_predicate_grid(oracle):
+    # Coarse-grid segments and boxes: many corners exactly on the line
+    # (orientation exactly 0) and segments along box edges.
+    rng = np.random.default_rng(22)
+    for _ in range(3000):
+        s = (rng.integers(0, 9, 4) / 8).astype(np.float32)
+        b = (np.sort(rng.integers(0, 9, (2, 2)), axis=0).T.reshape(-1)[[0, 2, 1, 3]] / 8)
+        b = b.astype(np.float32)
+        want = _liang_barsky(s, b)
+        assert oracle.evaluate(oracle.INTERSECTS, b, s, lines=True) == want, (s, b)
+        assert oracle.evaluate64(oracle.INTERSECTS, b, s, lines=True) == want, (s, b)
