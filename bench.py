@@ -553,7 +553,8 @@ def run_lines(args):
     value_plan = None
     for trav in (0, R._native.RTS_TRAVERSAL_THREAD, R._native.RTS_TRAVERSAL_WAVE):
         ix = R.SpatialIndex()
-        ix.Init(R.Config(max_geometries=args.boxes, traversal=trav))
+        kw = {"leaf_size": args.leaf_size} if args.leaf_size else {}
+        ix.Init(R.Config(max_geometries=args.boxes, traversal=trav, **kw))
         build = build_rate(R, ix, boxes, torch)
         cnt = R.SharedValue()
         cnt.set(0)
@@ -998,6 +999,8 @@ def main():
     ap.add_argument("--verify", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default plans)")
+    ap.add_argument("--leaf-size", type=int, default=0,
+                    help="lines: leaf-range size of the index (0: the library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU; run with --gpus 1",

@@ -46,7 +46,10 @@ __host__ __device__ __forceinline__ void two_prod(double a, double b, double& x,
 }
 
 // Sign (-1, 0, 1) of the exact orientation of c against the line p1 -> p2.
-__host__ __device__ inline int orient_exact(double p1x, double p1y, double p2x, double p2y,
+// Out of line: it runs only when the double evaluation is within its error
+// bound, and inlined it would raise the register count of every walk that
+// tests segments.
+__host__ __device__ __noinline__ inline int orient_exact(double p1x, double p1y, double p2x, double p2y,
                                             double cx, double cy) {
 #pragma clang fp contract(off)
   const double fa[6] = {p1x, -p1y, p2x, -p2y, cx, -cy};
@@ -77,33 +80,49 @@ __host__ __device__ inline int orient_exact(double p1x, double p1y, double p2x, 
   return sign;
 }
 
+// A segment as the tests take it: its ends s = (x1, y1, x2, y2) and the
+// rounded differences dx = x2 - x1, dy = y2 - y1 (rounding keeps their signs
+// exact: a difference of doubles rounds to 0 only when it is 0).
+struct Segment {
+  double s[4];
+  double dx, dy;
+};
+__host__ __device__ __forceinline__ Segment make_segment(double x1, double y1, double x2,
+                                                         double y2) {
+  Segment g;
+  g.s[0] = x1;
+  g.s[1] = y1;
+  g.s[2] = x2;
+  g.s[3] = y2;
+  g.dx = x2 - x1;
+  g.dy = y2 - y1;
+  return g;
+}
+
 // The corners of a box at which o(c) = orient(p1, p2, c) takes its largest
 // and smallest value: o(c) = dx (cy - y1) - dy (cx - x1) grows with cy when
-// dx > 0 and falls with cx when dy > 0. (The rounded dx, dy have the signs of
-// the exact differences, so the choice is exact.) A closed segment and a
-// closed box whose bounding boxes overlap meet iff o(max corner) >= 0 and
-// o(min corner) <= 0.
+// dx > 0 and falls with cx when dy > 0 (the choice is exact, as the signs of
+// dx and dy are). A closed segment and a closed box whose bounding boxes
+// overlap meet iff o(max corner) >= 0 and o(min corner) <= 0.
 struct SegmentCorners {
   double max_x, max_y, min_x, min_y;
 };
-__host__ __device__ __forceinline__ SegmentCorners segment_corners(const double s[4], double bx0,
+__host__ __device__ __forceinline__ SegmentCorners segment_corners(const Segment& g, double bx0,
                                                                    double by0, double bx1,
                                                                    double by1) {
-  const double dx = s[2] - s[0], dy = s[3] - s[1];
   SegmentCorners c;
-  c.max_y = dx >= 0.0 ? by1 : by0;
-  c.min_y = dx >= 0.0 ? by0 : by1;
-  c.max_x = dy >= 0.0 ? bx0 : bx1;
-  c.min_x = dy >= 0.0 ? bx1 : bx0;
+  c.max_y = g.dx >= 0.0 ? by1 : by0;
+  c.min_y = g.dx >= 0.0 ? by0 : by1;
+  c.max_x = g.dy >= 0.0 ? bx0 : bx1;
+  c.min_x = g.dy >= 0.0 ? bx1 : bx0;
   return c;
 }
 
 // o(c) in double with its rounding-error bound: 1 or -1 when the sign is
 // certain, 0 when it is not.
-__host__ __device__ __forceinline__ int orient_filtered(const double s[4], double cx, double cy) {
+__host__ __device__ __forceinline__ int orient_filtered(const Segment& g, double cx, double cy) {
 #pragma clang fp contract(off)
-  const double dx = s[2] - s[0], dy = s[3] - s[1];
-  const double a = dx * (cy - s[1]), b = dy * (cx - s[0]);
+  const double a = g.dx * (cy - g.s[1]), b = g.dy * (cx - g.s[0]);
   const double o = a - b;
   // Two differences, a product and the final difference round once each per
   // term: |error| <= (4 u + O(u^2)) (|a| + |b|), u = 2^-53; the bound keeps a
@@ -115,34 +134,42 @@ __host__ __device__ __forceinline__ int orient_filtered(const double s[4], doubl
   return 0;
 }
 
-// Bounding boxes of the segment (s = x1, y1, x2, y2) and the box (x0, y0, x1,
-// y1) overlap (closed; false for a NaN box).
-__host__ __device__ __forceinline__ bool segment_bbox_overlap(const double s[4], double bx0,
+// Bounding boxes of the segment and the box (x0, y0, x1, y1) overlap
+// (closed; false for a NaN box).
+__host__ __device__ __forceinline__ bool segment_bbox_overlap(const Segment& g, double bx0,
                                                               double by0, double bx1, double by1) {
-  const double sx0 = fmin(s[0], s[2]), sx1 = fmax(s[0], s[2]);
-  const double sy0 = fmin(s[1], s[3]), sy1 = fmax(s[1], s[3]);
+  const double sx0 = fmin(g.s[0], g.s[2]), sx1 = fmax(g.s[0], g.s[2]);
+  const double sy0 = fmin(g.s[1], g.s[3]), sy1 = fmax(g.s[1], g.s[3]);
   return sx0 <= bx1 && sx1 >= bx0 && sy0 <= by1 && sy1 >= by0;
 }
 
-// Exact: the closed segment meets the closed box.
-__host__ __device__ inline bool segment_meets_box(const double s[4], double bx0, double by0,
-                                                  double bx1, double by1) {
-  if (!segment_bbox_overlap(s, bx0, by0, bx1, by1)) return false;
-  const SegmentCorners c = segment_corners(s, bx0, by0, bx1, by1);
-  int hi = orient_filtered(s, c.max_x, c.max_y);
-  if (hi == 0) hi = orient_exact(s[0], s[1], s[2], s[3], c.max_x, c.max_y);
+// Exact, for a box whose bounding-box overlap with the segment is already
+// known: the box's corners are not all strictly on one side of the line.
+__host__ __device__ inline bool segment_line_crosses(const Segment& g, double bx0, double by0,
+                                                     double bx1, double by1) {
+  const SegmentCorners c = segment_corners(g, bx0, by0, bx1, by1);
+  int hi = orient_filtered(g, c.max_x, c.max_y);
+  if (hi == 0) hi = orient_exact(g.s[0], g.s[1], g.s[2], g.s[3], c.max_x, c.max_y);
   if (hi < 0) return false;
-  int lo = orient_filtered(s, c.min_x, c.min_y);
-  if (lo == 0) lo = orient_exact(s[0], s[1], s[2], s[3], c.min_x, c.min_y);
+  int lo = orient_filtered(g, c.min_x, c.min_y);
+  if (lo == 0) lo = orient_exact(g.s[0], g.s[1], g.s[2], g.s[3], c.min_x, c.min_y);
   return lo <= 0;
 }
 
-// Necessary condition for the segment to meet the box or anything inside it.
-__host__ __device__ __forceinline__ bool segment_may_meet_box(const double s[4], double bx0,
-                                                              double by0, double bx1, double by1) {
-  if (!segment_bbox_overlap(s, bx0, by0, bx1, by1)) return false;
-  const SegmentCorners c = segment_corners(s, bx0, by0, bx1, by1);
-  return orient_filtered(s, c.max_x, c.max_y) >= 0 && orient_filtered(s, c.min_x, c.min_y) <= 0;
+// Necessary condition for the line through the segment to meet the box or
+// anything inside it.
+__host__ __device__ __forceinline__ bool segment_line_may_cross(const Segment& g, double bx0,
+                                                                double by0, double bx1,
+                                                                double by1) {
+  const SegmentCorners c = segment_corners(g, bx0, by0, bx1, by1);
+  return orient_filtered(g, c.max_x, c.max_y) >= 0 && orient_filtered(g, c.min_x, c.min_y) <= 0;
+}
+
+// Exact: the closed segment meets the closed box.
+__host__ __device__ inline bool segment_meets_box(const Segment& g, double bx0, double by0,
+                                                  double bx1, double by1) {
+  return segment_bbox_overlap(g, bx0, by0, bx1, by1) &&
+         segment_line_crosses(g, bx0, by0, bx1, by1);
 }
 
 }  // namespace rts

@@ -54,7 +54,7 @@ struct QueryF64 {
 // outward on a double index, so the test stays a necessary condition).
 template <bool F64>
 struct QueryLine {
-  double s[4];
+  Segment g;
   float4 bb;
 };
 
@@ -176,10 +176,12 @@ __device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, cons
   if (P::suff(b, q)) return true;
   return P::exact(t.boxes64[t.leaf_ids[leaf]], q.q);
 }
-// Segments: no subtree lies "inside" a segment.
+// Segments: no subtree lies "inside" a segment. The float bounding-box test
+// comes first; on a float index it is the exact bounding-box test (bb is the
+// segment's exact bounding box), on a double index a necessary one.
 template <int PRED, bool POINTS, bool F64>
 __device__ __forceinline__ bool node_test(const float4 n, const QueryLine<F64>& q) {
-  return intersects(n, q.bb) && segment_may_meet_box(q.s, n.x, n.y, n.z, n.w);
+  return intersects(n, q.bb) && segment_line_may_cross(q.g, n.x, n.y, n.z, n.w);
 }
 template <int PRED, bool POINTS, bool F64>
 __device__ __forceinline__ bool inside_test(const float4, const QueryLine<F64>&) {
@@ -189,10 +191,10 @@ template <int PRED, bool POINTS, bool F64>
 __device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, const float4 b,
                                           const QueryLine<F64>& q) {
   if (!intersects(b, q.bb)) return false;
-  if (!F64) return segment_meets_box(q.s, b.x, b.y, b.z, b.w);
-  if (!segment_may_meet_box(q.s, b.x, b.y, b.z, b.w)) return false;
+  if (!F64) return segment_line_crosses(q.g, b.x, b.y, b.z, b.w);
+  if (!segment_line_may_cross(q.g, b.x, b.y, b.z, b.w)) return false;
   const Box64 d = t.boxes64[t.leaf_ids[leaf]];
-  return segment_meets_box(q.s, d.x0, d.y0, d.x1, d.y1);
+  return segment_meets_box(q.g, d.x0, d.y0, d.x1, d.y1);
 }
 
 // Emitters. The result queue: the query's k-th pair goes to slot base + k,
@@ -259,20 +261,18 @@ __device__ __forceinline__ void load_q(const void* queries, bool points, uint32_
 // with a NaN coordinate gets a NaN bounding box and so meets nothing.
 __device__ __forceinline__ void load_q(const void* queries, bool, uint32_t i, QueryLine<false>& q) {
   const float4 f = static_cast<const float4*>(queries)[i];
-  q.s[0] = f.x;
-  q.s[1] = f.y;
-  q.s[2] = f.z;
-  q.s[3] = f.w;
+  q.g = make_segment(f.x, f.y, f.z, f.w);
   q.bb = make_float4(fminf(f.x, f.z), fminf(f.y, f.w), fmaxf(f.x, f.z), fmaxf(f.y, f.w));
   if (!(f.x == f.x && f.y == f.y && f.z == f.z && f.w == f.w))
     q.bb = make_float4(__builtin_nanf(""), 0.0f, 0.0f, 0.0f);
 }
 __device__ __forceinline__ void load_q(const void* queries, bool, uint32_t i, QueryLine<true>& q) {
   const double* p = static_cast<const double*>(queries) + 4ull * i;
-  for (int k = 0; k < 4; k++) q.s[k] = p[k];
-  q.bb = make_float4(f32_rd(fmin(q.s[0], q.s[2])), f32_rd(fmin(q.s[1], q.s[3])),
-                     f32_ru(fmax(q.s[0], q.s[2])), f32_ru(fmax(q.s[1], q.s[3])));
-  if (!(q.s[0] == q.s[0] && q.s[1] == q.s[1] && q.s[2] == q.s[2] && q.s[3] == q.s[3]))
+  q.g = make_segment(p[0], p[1], p[2], p[3]);
+  const double* s = q.g.s;
+  q.bb = make_float4(f32_rd(fmin(s[0], s[2])), f32_rd(fmin(s[1], s[3])), f32_ru(fmax(s[0], s[2])),
+                     f32_ru(fmax(s[1], s[3])));
+  if (!(s[0] == s[0] && s[1] == s[1] && s[2] == s[2] && s[3] == s[3]))
     q.bb = make_float4(__builtin_nanf(""), 0.0f, 0.0f, 0.0f);
 }
 
