@@ -172,4 +172,61 @@ __host__ __device__ inline bool segment_meets_box(const Segment& g, double bx0, 
          segment_line_crosses(g, bx0, by0, bx1, by1);
 }
 
+// ---- float segments against float boxes (a float index) ----------------
+// The same two corners evaluated in float first: o = dx (cy - y1) - dy (cx -
+// x1) with each operation rounded once, |error| <= (4 u + O(u^2)) (|a| + |b|)
+// for u = 2^-24; the bound keeps a margin over that and an absolute floor
+// covers underflow. Only a corner within the bound goes on to the double and
+// exact evaluation.
+struct SegmentF {
+  float s[4];
+  float dx, dy;
+};
+__host__ __device__ __forceinline__ SegmentF make_segment_f(float x1, float y1, float x2,
+                                                            float y2) {
+  SegmentF g;
+  g.s[0] = x1;
+  g.s[1] = y1;
+  g.s[2] = x2;
+  g.s[3] = y2;
+  g.dx = x2 - x1;
+  g.dy = y2 - y1;
+  return g;
+}
+__host__ __device__ __forceinline__ int orient_filtered_f(const SegmentF& g, float cx, float cy) {
+#pragma clang fp contract(off)
+  const float a = g.dx * (cy - g.s[1]), b = g.dy * (cx - g.s[0]);
+  const float o = a - b;
+  const float bound = 1.0e-6f * (fabsf(a) + fabsf(b)) + 1.0e-37f;
+  if (o > bound) return 1;
+  if (o < -bound) return -1;
+  return 0;
+}
+// The double and exact evaluation of a float segment, out of line.
+__host__ __device__ __noinline__ inline bool segment_line_crosses_slow(float x1, float y1,
+                                                                       float x2, float y2,
+                                                                       float bx0, float by0,
+                                                                       float bx1, float by1) {
+  return segment_line_crosses(make_segment(x1, y1, x2, y2), bx0, by0, bx1, by1);
+}
+// segment_line_crosses for a float segment and a float box.
+__host__ __device__ __forceinline__ bool segment_line_crosses_f(const SegmentF& g, float bx0,
+                                                                float by0, float bx1, float by1) {
+  const float max_y = g.dx >= 0.0f ? by1 : by0, min_y = g.dx >= 0.0f ? by0 : by1;
+  const float max_x = g.dy >= 0.0f ? bx0 : bx1, min_x = g.dy >= 0.0f ? bx1 : bx0;
+  const int hi = orient_filtered_f(g, max_x, max_y);
+  const int lo = orient_filtered_f(g, min_x, min_y);
+  if (hi < 0 || lo > 0) return false;
+  if (hi > 0 && lo < 0) return true;
+  return segment_line_crosses_slow(g.s[0], g.s[1], g.s[2], g.s[3], bx0, by0, bx1, by1);
+}
+// segment_line_may_cross for a float segment and a float box.
+__host__ __device__ __forceinline__ bool segment_line_may_cross_f(const SegmentF& g, float bx0,
+                                                                  float by0, float bx1,
+                                                                  float by1) {
+  const float max_y = g.dx >= 0.0f ? by1 : by0, min_y = g.dx >= 0.0f ? by0 : by1;
+  const float max_x = g.dy >= 0.0f ? bx0 : bx1, min_x = g.dy >= 0.0f ? bx1 : bx0;
+  return orient_filtered_f(g, max_x, max_y) >= 0 && orient_filtered_f(g, min_x, min_y) <= 0;
+}
+
 }  // namespace rts

@@ -53,7 +53,14 @@ struct QueryF64 {
 // tests and its bounding box in floats for a first test of a node (rounded
 // outward on a double index, so the test stays a necessary condition).
 template <bool F64>
-struct QueryLine {
+struct QueryLine;
+template <>
+struct QueryLine<false> {
+  SegmentF g;
+  float4 bb;
+};
+template <>
+struct QueryLine<true> {
   Segment g;
   float4 bb;
 };
@@ -179,19 +186,27 @@ __device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, cons
 // Segments: no subtree lies "inside" a segment. The float bounding-box test
 // comes first; on a float index it is the exact bounding-box test (bb is the
 // segment's exact bounding box), on a double index a necessary one.
-template <int PRED, bool POINTS, bool F64>
-__device__ __forceinline__ bool node_test(const float4 n, const QueryLine<F64>& q) {
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool node_test(const float4 n, const QueryLine<false>& q) {
+  return intersects(n, q.bb) && segment_line_may_cross_f(q.g, n.x, n.y, n.z, n.w);
+}
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool node_test(const float4 n, const QueryLine<true>& q) {
   return intersects(n, q.bb) && segment_line_may_cross(q.g, n.x, n.y, n.z, n.w);
 }
 template <int PRED, bool POINTS, bool F64>
 __device__ __forceinline__ bool inside_test(const float4, const QueryLine<F64>&) {
   return false;
 }
-template <int PRED, bool POINTS, bool F64>
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool leaf_test(const TreeView&, uint32_t, const float4 b,
+                                          const QueryLine<false>& q) {
+  return intersects(b, q.bb) && segment_line_crosses_f(q.g, b.x, b.y, b.z, b.w);
+}
+template <int PRED, bool POINTS>
 __device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, const float4 b,
-                                          const QueryLine<F64>& q) {
+                                          const QueryLine<true>& q) {
   if (!intersects(b, q.bb)) return false;
-  if (!F64) return segment_line_crosses(q.g, b.x, b.y, b.z, b.w);
   if (!segment_line_may_cross(q.g, b.x, b.y, b.z, b.w)) return false;
   const Box64 d = t.boxes64[t.leaf_ids[leaf]];
   return segment_meets_box(q.g, d.x0, d.y0, d.x1, d.y1);
@@ -261,7 +276,7 @@ __device__ __forceinline__ void load_q(const void* queries, bool points, uint32_
 // with a NaN coordinate gets a NaN bounding box and so meets nothing.
 __device__ __forceinline__ void load_q(const void* queries, bool, uint32_t i, QueryLine<false>& q) {
   const float4 f = static_cast<const float4*>(queries)[i];
-  q.g = make_segment(f.x, f.y, f.z, f.w);
+  q.g = make_segment_f(f.x, f.y, f.z, f.w);
   q.bb = make_float4(fminf(f.x, f.z), fminf(f.y, f.w), fmaxf(f.x, f.z), fmaxf(f.y, f.w));
   if (!(f.x == f.x && f.y == f.y && f.z == f.z && f.w == f.w))
     q.bb = make_float4(__builtin_nanf(""), 0.0f, 0.0f, 0.0f);
