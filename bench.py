@@ -534,8 +534,8 @@ def run_lines(args):
     spatial_index.cuh:517-547): the segments are the diagonals of the skewed
     query envelopes of --workload c3 (min corner to max corner, as the
     reference's GenerateUniformLines draws them, test_commons.h:69-97), over
-    the same 10M boxes; all pairs collected on the device. Both traversals are
-    timed; `value` is the library's default plan."""
+    the same 10M boxes; all pairs collected on the device. `value` is the
+    library's default plan; --sweep also times each traversal mode."""
     import torch
 
     import rtspatial_amd as R
@@ -551,7 +551,8 @@ def run_lines(args):
     nq = segs.shape[0]
     results = {}
     value_plan = None
-    for trav in (0, R._native.RTS_TRAVERSAL_THREAD, R._native.RTS_TRAVERSAL_WAVE):
+    travs = (0, R._native.RTS_TRAVERSAL_THREAD, R._native.RTS_TRAVERSAL_WAVE) if args.sweep else (0,)
+    for trav in travs:
         ix = R.SpatialIndex()
         kw = {"leaf_size": args.leaf_size} if args.leaf_size else {}
         ix.Init(R.Config(max_geometries=args.boxes, traversal=trav, **kw))
@@ -999,6 +1000,7 @@ def main():
     ap.add_argument("--verify", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default plans)")
+    ap.add_argument("--sweep", action="store_true", help="lines: also time each traversal mode")
     ap.add_argument("--leaf-size", type=int, default=0,
                     help="lines: leaf-range size of the index (0: the library default)")
     args = ap.parse_args()
