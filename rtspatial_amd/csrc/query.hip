@@ -37,7 +37,10 @@ __device__ __forceinline__ void flush_stage(const rts_pair* stage, uint32_t fill
   lds_fence();
   for (uint32_t j = lane; j < fill; j += kWave) {
     const uint64_t slot = base + j;
-    if (slot < cap) out[slot] = stage[j];
+    if (slot < cap) {
+      const rts_pair pr = stage[j];
+      store_pair(out + slot, pr.geom_id, pr.query_id);
+    }
   }
   lds_fence();
 }
