@@ -292,11 +292,21 @@ def run_served(args):
     op_plain, _, _ = one("plain", lam0)
     target = op_plain["p99_ms"]
     sweep, best = [], None
-    for m in (1.0, 1.5, 2.0, 2.3, 2.6, 2.9):
+    # Fixed load points, then further points 0.3 x apart while the target
+    # holds and the offered rate stays within the served path's saturated rate.
+    served_rps = n_req / sat_served["wall_s"]
+    mults = [1.0, 1.5, 2.0, 2.3, 2.6, 2.9]
+    i = 0
+    while i < len(mults):
+        m = mults[i]
         s, _, _ = one("batched", lam0 * m)
         sweep.append(s)
-        if s["p99_ms"] <= target and (best is None or s["mq_per_s"] > best["mq_per_s"]):
+        ok = s["p99_ms"] <= target
+        if ok and (best is None or s["mq_per_s"] > best["mq_per_s"]):
             best = s
+        if i == len(mults) - 1 and ok and lam0 * (m + 0.3) <= served_rps:
+            mults.append(round(m + 0.3, 2))
+        i += 1
     met = best is not None
     if best is None:
         best = sweep[0]
