@@ -292,7 +292,7 @@ def run_served(args):
     op_plain, _, _ = one("plain", lam0)
     target = op_plain["p99_ms"]
     sweep, best = [], None
-    # Fixed load points, then further points 0.3 x apart while the target
+    # Fixed load points, then further points 0.15 x apart while the target
     # holds and the offered rate stays within the served path's saturated rate.
     served_rps = n_req / sat_served["wall_s"]
     mults = [1.0, 1.5, 2.0, 2.3, 2.6, 2.9]
@@ -304,8 +304,8 @@ def run_served(args):
         ok = s["p99_ms"] <= target
         if ok and (best is None or s["mq_per_s"] > best["mq_per_s"]):
             best = s
-        if i == len(mults) - 1 and ok and lam0 * (m + 0.3) <= served_rps:
-            mults.append(round(m + 0.3, 2))
+        if i == len(mults) - 1 and ok and lam0 * (m + 0.15) <= served_rps:
+            mults.append(round(m + 0.15, 2))
         i += 1
     met = best is not None
     if best is None:

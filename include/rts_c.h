@@ -129,7 +129,8 @@ typedef enum rts_handler {
 
 /* Rebuild policy for mutations. */
 typedef enum rts_rebuild_policy {
-  RTS_REBUILD_SAH = 0,     /* refit; rebuild when SAH cost > sah_rebuild_ratio x cost at build */
+  RTS_REBUILD_SAH = 0,     /* refit; rebuild when SAH cost > sah_rebuild_ratio x cost at build,
+                              or when updates since the last build cover >= 1/200 of the boxes */
   RTS_REBUILD_NEVER = 1,   /* refit only (the reference's behaviour, spatial_index.cuh:208-336) */
   RTS_REBUILD_ALWAYS = 2   /* rebuild after every mutation */
 } rts_rebuild_policy;

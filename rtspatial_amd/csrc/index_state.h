@@ -70,6 +70,7 @@ struct rts_index {
 
   uint64_t builds = 0;
   uint64_t refits = 0;
+  uint64_t updated_since_build = 0;  // boxes given new envelopes since the last full build
   uint64_t delta_builds = 0;
 
   // Plans stored by rts_index_autotune, per [predicate][points]; kept until

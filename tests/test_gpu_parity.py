@@ -386,9 +386,34 @@ def test_mutable_rounds_match_oracle_every_round(oracle, policy):
             assert after["refits"] == before["refits"]
         elif policy == "refit":
             assert after["refits"] == before["refits"] + 1 and after["builds"] == before["builds"]
-        elif policy == "rebuild":
+        else:
+            # rebuild; and sah, as 100k updates are over 1/200 of the boxes
             assert after["builds"] == before["builds"] + 1
+            assert after["refits"] == before["refits"]
     ix.Sync()
+
+
+def test_sah_policy_refits_small_updates_and_rebuilds_large_ones(oracle):
+    # Under RTS_REBUILD_SAH, updates covering less than 1/200 of the boxes
+    # since the last build are refitted; once they reach it, the tree is rebuilt.
+    rng = np.random.default_rng(31)
+    boxes = oracle.ref_uniform_boxes(200_000, 0.01, 0.01)
+    ix = make_index(boxes)
+    ix.Build()
+    pts = rng.random((20_000, 2)).astype(np.float32)
+    cur = boxes.copy()
+    for n_upd, rebuilt in ((100, False), (400, False), (600, True), (10, False)):
+        log(f"sah policy: {n_upd} updates")
+        ids = rng.choice(200_000, n_upd, replace=False)
+        nb = cur[ids] + np.float32(0.002)
+        cur[ids] = nb
+        before = ix.stats()
+        ix.Update((dev(ids.astype(np.int64)), dev(nb)))
+        got, _ = collect(ix, P.kContains, pts)
+        assert np.array_equal(got, oracle.RTree(cur).pairs(oracle.CONTAINS, pts))
+        after = ix.stats()
+        assert after["builds"] == before["builds"] + (1 if rebuilt else 0), n_upd
+        assert after["refits"] == before["refits"] + (0 if rebuilt else 1), n_upd
 
 
 @pytest.mark.parametrize("kind", ["points", "intersects", "within"])

@@ -18,7 +18,7 @@ if not torch.cuda.is_available():
 import rtspatial_amd as R  # noqa: E402
 from rtspatial_amd import serve as S  # noqa: E402
 from rtspatial_amd import workloads as W  # noqa: E402
-from gpu_util import canon, dev, log, make_index  # noqa: E402
+from gpu_util import canon, collect, dev, log, make_index  # noqa: E402
 
 P = R.Predicate
 
@@ -154,6 +154,10 @@ def test_updates_interleaved_each_request_sees_its_epoch(oracle, scene, mode):
         epochs.add(e)
         assert np.array_equal(canon(r["pairs"]), trees[e].pairs(int(pred), q)), (i, e)
     assert len(epochs) >= 2
+    # The caller's index ends with every update applied.
+    q, pred = host[0]
+    got, _ = collect(ix, pred, dev(q))
+    assert np.array_equal(got, trees[-1].pairs(int(pred), q))
 
 
 def test_empty_and_single_query_requests(oracle, scene):
