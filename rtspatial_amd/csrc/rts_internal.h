@@ -76,9 +76,10 @@ struct RefitArgs {
   uint32_t* boundary;  // refit_boundary_words(n): [0] count, then (node << 1 | side) codes
 };
 // Words of the boundary list of a refit over n leaves: a node spanning
-// 2048-leaf chunks contains a chunk boundary, and a boundary lies in at most
-// 64 nodes (tree depth), so the spanning nodes -- and the list's entries,
-// one more than their number -- are bounded by 64 per boundary.
+// refit chunks (at least 2048 leaves each) contains a chunk boundary, and a
+// boundary lies in at most 64 nodes (tree depth), so the spanning nodes --
+// and the list's entries, one more than their number -- are bounded by 64
+// per boundary.
 __host__ __device__ inline uint64_t refit_boundary_words(uint64_t n) {
   return 64 * (n / 2048 + 2) + 2;
 }
