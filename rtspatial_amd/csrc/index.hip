@@ -273,7 +273,7 @@ bool delta_ok(const rts_index* ix) {
 
 // Under RTS_REBUILD_SAH, updates that give new envelopes to at least 1/200
 // of the tree's boxes since its last build mark it stale: a full build costs
-// ~1.45x a refit here (10M boxes: 3.6 vs 2.5 ms), while a refit over boxes
+// ~1.5x a refit here (10M boxes: 2.8 vs 1.7 ms), while a refit over boxes
 // moved beyond their old extents leaves the tree slower to query. On the
 // served stream's 100k-box update batches (1% of 10M boxes, each moved by up
 // to 10x its size) the tree cost grew ~2% per batch and never reached the
