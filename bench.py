@@ -633,7 +633,7 @@ def run_lines(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (exact predicate, f64 filter)",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {
             "workload": (f"line-segment intersects: {args.boxes} Gaussian-mixture boxes x {nq} "
