@@ -306,7 +306,11 @@ __device__ __forceinline__ void load_q(const void* queries, bool, uint32_t i, Qu
 // slot `obase`.
 constexpr int kWaveStack = 256;     // LDS stack entries per wave
 constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
-constexpr int kFlushLeaves = 512;   // scan once this many leaves are queued
+// Scan once this many leaves are queued. Deferring the scan past one step's
+// 512 leaves leaves fewer part-filled final steps: on the 10M x 1M
+// intersects join 44.5 -> 42.4 ms per step (512 -> 2048; 4096 and 8192 no
+// better), segment join 12.81 -> 12.63 ms.
+constexpr int kFlushLeaves = 2048;
 constexpr int kPerLane = 8;         // leaves per lane per scan step
 constexpr uint32_t kScanStep = kPerLane * kWave;
 
