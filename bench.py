@@ -293,7 +293,8 @@ def run_served(args):
     target = op_plain["p99_ms"]
     sweep, best = [], None
     # Fixed load points, then further points 0.15 x apart while the target
-    # holds and the offered rate stays within the served path's saturated rate.
+    # holds and the offered rate stays within the served path's saturated rate,
+    # and last that rate itself.
     served_rps = n_req / sat_served["wall_s"]
     mults = [1.0, 1.5, 2.0, 2.3, 2.6, 2.9]
     i = 0
@@ -304,8 +305,11 @@ def run_served(args):
         ok = s["p99_ms"] <= target
         if ok and (best is None or s["mq_per_s"] > best["mq_per_s"]):
             best = s
-        if i == len(mults) - 1 and ok and lam0 * (m + 0.15) <= served_rps:
-            mults.append(round(m + 0.15, 2))
+        if i == len(mults) - 1 and ok:
+            if lam0 * (m + 0.15) <= served_rps:
+                mults.append(round(m + 0.15, 2))
+            elif lam0 * m < served_rps:
+                mults.append(served_rps / lam0)  # last point: the saturated rate itself
         i += 1
     met = best is not None
     if best is None:
