@@ -59,3 +59,18 @@ def test_device_segment_predicates_match_oracle(oracle, seg_host, dtype):
     assert np.all(got[want == 1, 1] == 1)                # node filters are necessary
     assert np.all(got[want == 1, 3] == 1)
     assert np.mean(got[want == 0, 1] == 0) > 0.5         # and prune most misses
+
+
+@pytest.mark.parametrize("scale", [2.0 ** -400, 2.0 ** 400])
+def test_device_segment_predicate_extreme_doubles(oracle, seg_host, scale):
+    # The exact predicate's documented range for doubles (magnitudes 0 and
+    # 2^-450 .. 2^500): adversarial cases scaled by a power of two keep their
+    # answers, and the host build of the device predicate must still give
+    # the exact rational one.
+    segs, boxes = cases(np.float64)
+    segs = segs * scale
+    boxes = boxes * scale
+    got = run(seg_host, segs, boxes, np.float64)
+    want = np.array([oracle.segment_meets_box(s, b) for s, b in zip(segs, boxes)])
+    np.testing.assert_array_equal(got[:, 0], want)
+    assert np.all(got[want, 1] == 1)  # the node filter stays a necessary condition
