@@ -297,6 +297,7 @@ def run_served(args):
     # and last that rate itself.
     served_rps = n_req / sat_served["wall_s"]
     mults = [1.0, 1.5, 2.0, 2.3, 2.6, 2.9]
+    at_saturation = False
     i = 0
     while i < len(mults):
         m = mults[i]
@@ -305,11 +306,12 @@ def run_served(args):
         ok = s["p99_ms"] <= target
         if ok and (best is None or s["mq_per_s"] > best["mq_per_s"]):
             best = s
-        if i == len(mults) - 1 and ok:
+        if i == len(mults) - 1 and ok and not at_saturation:
             if lam0 * (m + 0.15) <= served_rps:
                 mults.append(round(m + 0.15, 2))
             elif lam0 * m < served_rps:
                 mults.append(served_rps / lam0)  # last point: the saturated rate itself
+                at_saturation = True
         i += 1
     met = best is not None
     if best is None:
