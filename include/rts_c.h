@@ -354,9 +354,11 @@ int rts_index_memory(rts_index* index, rts_memory_usage* out);
 /* Autotuner. No reference counterpart: it stands where the reference's
  * CalculateBestParallelism sampling (spatial_index.cuh:41-57,690-760) picks a
  * ray parallelism. It times the query over an evenly strided sample of
- * `queries` under every candidate plan -- leaf size, one query per lane or
- * per wavefront, and, when the sample is the whole set, Morton-ordered or
- * input-ordered queries -- with the given handler. With `apply` set, it stores the fastest plan in the index for this
+ * `queries` (shrunk until its results are at most 2^27 pairs) under every
+ * candidate plan -- leaf size, one query per lane or per wavefront (per lane
+ * only when the sample averages at most 256 results per query), and, when the
+ * sample is the whole set, Morton-ordered or input-ordered queries -- with
+ * the given handler. With `apply` set, it stores the fastest plan in the index for this
  * (predicate, query kind); later queries of that kind, and the batches of
  * rts_serve_run, use it for every setting their config leaves on auto. The
  * plan survives rts_index_clear. Results are identical under every plan;
@@ -366,7 +368,7 @@ int rts_index_memory(rts_index* index, rts_memory_usage* out);
 typedef struct rts_tune_config {
   uint32_t sample;      /* queries probed (32768; all of them if fewer) */
   uint32_t leaf_sizes;  /* bit k set: try leaf size 2^k (bits 2..5: 4, 8, 16, 32) */
-  uint32_t reps;        /* timed runs per plan; the fastest counts (3) */
+  uint32_t reps;        /* timed runs per plan; the fastest counts (2) */
   int32_t apply;        /* store the winner in the index (1) */
 } rts_tune_config;
 
@@ -393,6 +395,18 @@ int rts_index_autotune(rts_index* index, rts_predicate pred, const void* queries
 int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                    uint32_t* vals_out, uint64_t n, int key_bits, rts_stream_t stream);
 
+/* Checksum of a result-pair array, for checking result sets too large to
+ * compare pair by pair (no reference counterpart). For each query id q <
+ * n_queries: counts[q] = pairs with that query id, gsums[q] = sum over them
+ * of ghash(geom_id) mod 2^64, where ghash(b) = splitmix64(b ^
+ * 0xD1B54A32D192ED03) (splitmix64: Steele, Lea and Flood, OOPSLA 2014).
+ * Pairs whose query id is >= n_queries are skipped. counts and gsums are
+ * device arrays of n_queries words, overwritten. A request's checksum is the
+ * sum over its queries of qmult(q) * gsums[q], qmult(q) = splitmix64(q) | 1
+ * (rts_request_result.hash). Queued on `stream`. */
+int rts_pairs_checksum(const rts_pair* pairs, uint64_t n, uint64_t n_queries, uint64_t* counts,
+                       uint64_t* gsums, rts_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Served / batched execution (no reference counterpart: the reference answers
  * one Query call at a time, spatial_index.cuh:352-515).
@@ -407,8 +421,7 @@ int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* k
  *     on separate streams; whenever a slot is free, every arrived request
  *     (split into chunks when large) is admitted while its predicted result
  *     bytes fit the slot's arena, and the batch runs as one persistent
- *     work-stealing launch sequence (count, scan, write); with use_graphs the
- *     sequence is replayed from a hipGraph per slot. Chunks whose results
+ *     work-stealing launch sequence (count, scan, write). Chunks whose results
  *     overflow the arena are re-launched.
  *   RTS_SERVE_PLAIN: the per-request path: each request (split the same way)
  *     is one rts_index_query_* call on one stream, in arrival order.
@@ -443,7 +456,9 @@ typedef struct rts_request_result {
   double admit_s;        /* first launch of the request */
   double done_s;         /* all of its results complete */
   uint64_t count;        /* results */
-  uint64_t hash;         /* with hash_results: sum of splitmix64((q << 32) | geom_id) */
+  uint64_t hash;         /* with hash_results: checksum of the request's pairs, the sum
+                            over them of qmult(q) * ghash(geom_id) mod 2^64, q = query
+                            index within the request (see rts_pairs_checksum) */
   uint32_t epoch;        /* update batches applied before the request ran */
   uint32_t relaunches;   /* chunk re-launches after an arena overflow */
   int32_t truncated;     /* `out` was too small for the results */
@@ -454,9 +469,9 @@ typedef struct rts_serve_config {
   uint32_t mode;               /* rts_serve_mode */
   uint32_t slots;              /* batches in flight (batched mode) (3) */
   uint32_t max_batch_requests; /* request chunks per batch (4096) */
-  uint32_t use_graphs;         /* replay each slot's launches as a hipGraph (0: off by
-                                  default, because a replay run did not complete on the
-                                  MI355X and the cause is not established) */
+  uint32_t use_graphs;         /* must be 0: hipGraph replay of a slot's launches is refused
+                                  (RTS_ERR_INVALID), because a replay run did not complete
+                                  on the MI355X and the cause is not established */
   uint64_t max_batch_queries;  /* queries per batch (4M) */
   uint64_t chunk_queries;      /* largest chunk of one request (256K) */
   uint64_t arena_pairs;        /* result arena per slot (or of the plain path), pairs */

@@ -648,11 +648,11 @@ __device__ uint64_t lane_query(const TreeView& t, const Q& qb, const uint32_t qi
       if (!WRITE && inside) {
         found += live[side];
       } else if (ref[side] & kLeafFlag) {
-        if (sp < kStackDepth) stack[sp++] = ref[side];
+        lane_push(stack, sp, ref[side], t.error);
       } else if (cnt[side] <= t.leaf_size || inside) {
-        if (sp < kStackDepth) stack[sp++] = ref[side] | kRangeFlag;
+        lane_push(stack, sp, ref[side] | kRangeFlag, t.error);
       } else {
-        if (sp < kStackDepth) stack[sp++] = ref[side];
+        lane_push(stack, sp, ref[side], t.error);
       }
     }
   }

@@ -62,6 +62,19 @@ constexpr uint32_t kErrSortWait = 1u;
 constexpr uint32_t kErrRefitWalk = 2u;
 constexpr uint32_t kErrTraversal = 4u;
 
+// Pushes v onto a lane's private traversal stack. kStackDepth exceeds the
+// deepest tree the builder emits (ties in the Morton keys are split by leaf
+// index, so depth <= 64); should a push still find the stack full, the walk
+// sets kErrTraversal in the index's error word (the call then fails) rather
+// than dropping the subtree silently.
+__device__ __forceinline__ void lane_push(uint32_t* stack, int& sp, uint32_t v, uint32_t* error) {
+  if (sp < kStackDepth) {
+    stack[sp++] = v;
+  } else {
+    atomicOr(error, kErrTraversal);
+  }
+}
+
 // ----------------------------------------------------------- predicates ----
 // Exact closed-interval tests in FP32 (reference geom/envelope.cuh:29-69).
 // Boxes are float4 (x0, y0, x1, y1). Every comparison with a NaN is false, so

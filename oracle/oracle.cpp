@@ -311,18 +311,38 @@ void parallel_dynamic(uint64_t n, int nt, uint64_t grain,
 // ---------------------------------------------------------------- R-tree ----
 // Sort-Tile-Recursive bulk-loaded R-tree with exact FP32 node bounds. Node
 // tests are necessary conditions for the leaf predicate, so pruning never
-// drops a true pair.
+// drops a true pair. Every node also holds the number of its matchable
+// entries (boxes without a NaN coordinate) and the sum of their id hashes
+// ghash(id) (below), so a node whose bounds lie inside an envelope query
+// answers intersects and within for its whole subtree at once. Entries can
+// be given new boxes in place (rtree_update), after which every node's
+// bounds and sums are recomputed (a refit: same tree, exact bounds).
 struct RTree {
   int fanout = 16;
   std::vector<Box> nodes;              // bounds of every node (leaves first)
   std::vector<uint32_t> child_begin;   // per node: first child index (or first entry)
   std::vector<uint32_t> child_count;
   std::vector<uint8_t> is_leaf;
+  std::vector<uint64_t> node_cnt;      // matchable entries below the node
+  std::vector<uint64_t> node_gsum;     // sum of ghash(id) over them
   std::vector<uint32_t> entries;       // leaf entry -> box id
   std::vector<Box> entry_boxes;        // leaf entry -> box (copied for locality)
+  std::vector<uint32_t> pos_of_id;     // box id -> entry (UINT32_MAX: not in the tree)
   uint32_t root = 0;
   bool empty = true;
+  int threads = 0;
 };
+
+// Factored pair checksum. A pair (query q, box b) hashes to
+// qmult(q) * ghash(b) mod 2^64, with qmult odd; the checksum of a query's
+// result set is the sum of ghash(b) over it, and of a request the sum over
+// its queries of qmult(q) times that. Because the hash factors, a subtree
+// that matches whole contributes its stored sum. A missing, extra or
+// substituted pair changes a query's sum by ghash differences (splitmix64 is
+// a bijection, so distinct ids hash apart), which qmult (odd, so invertible
+// mod 2^64) carries into the request's sum.
+inline uint64_t ghash(uint64_t b) { return splitmix64(b ^ 0xD1B54A32D192ED03ull); }
+inline uint64_t qmult(uint64_t q) { return splitmix64(q) | 1ull; }
 
 Box unite(const Box& a, const Box& b) {
   return Box{std::min(a.x0, b.x0), std::min(a.y0, b.y0), std::max(a.x1, b.x1), std::max(a.y1, b.y1)};
@@ -330,64 +350,152 @@ Box unite(const Box& a, const Box& b) {
 
 const Box kEmptyBox{INFINITY, INFINITY, -INFINITY, -INFINITY};
 
-// STR packing of `items` (indices into `cb`, the item bounds) into groups of
-// `fanout`; returns the groups as ranges over the reordered `items`.
-void str_order(std::vector<uint32_t>& items, const std::vector<Box>& cb, int fanout) {
-  size_t n = items.size();
-  if (n <= (size_t)fanout) return;
-  size_t leaves = (n + fanout - 1) / fanout;
-  size_t slabs = (size_t)std::ceil(std::sqrt((double)leaves));
-  size_t per_slab = slabs ? ((leaves + slabs - 1) / slabs) * fanout : n;
-  // NaN centres (never-matching boxes) sort last so the order stays strict-weak.
-  auto cx = [&](uint32_t i) {
-    float v = 0.5f * (cb[i].x0 + cb[i].x1);
-    return std::isnan(v) ? INFINITY : v;
-  };
-  auto cy = [&](uint32_t i) {
-    float v = 0.5f * (cb[i].y0 + cb[i].y1);
-    return std::isnan(v) ? INFINITY : v;
-  };
-  std::sort(items.begin(), items.end(), [&](uint32_t a, uint32_t b) {
-    float xa = cx(a), xb = cx(b);
-    return xa < xb || (xa == xb && a < b);
+inline bool box_has_nan(const Box& x) {
+  return std::isnan(x.x0) || std::isnan(x.y0) || std::isnan(x.x1) || std::isnan(x.y1);
+}
+
+// The extent a box covers, corners in either order.
+inline Box normalized(const Box& x) {
+  return Box{std::min(x.x0, x.x1), std::min(x.y0, x.y1), std::max(x.x0, x.x1), std::max(x.y0, x.y1)};
+}
+
+// Ascending-order integer image of a float; NaN sorts last.
+inline uint32_t float_key(float v) {
+  if (std::isnan(v)) return 0xFFFFFFFFu;
+  uint32_t u;
+  std::memcpy(&u, &v, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Sorts `v` ascending: chunks sorted in parallel, then merged pairwise.
+void parallel_sort_u64(std::vector<uint64_t>& v, int nt) {
+  const size_t n = v.size();
+  int parts = 1;
+  while (parts * 2 <= nt && n / (size_t)(parts * 2) >= (1u << 15)) parts *= 2;
+  if (parts == 1) {
+    std::sort(v.begin(), v.end());
+    return;
+  }
+  std::vector<size_t> cut(parts + 1);
+  for (int i = 0; i <= parts; i++) cut[i] = n * (size_t)i / (size_t)parts;
+  parallel_ranges((uint64_t)parts, parts, [&](uint64_t b, uint64_t e, int) {
+    for (uint64_t i = b; i < e; i++) std::sort(v.begin() + cut[i], v.begin() + cut[i + 1]);
   });
-  for (size_t s = 0; s < n; s += per_slab) {
-    size_t e = std::min(n, s + per_slab);
-    std::sort(items.begin() + s, items.begin() + e, [&](uint32_t a, uint32_t b) {
-      float ya = cy(a), yb = cy(b);
-      return ya < yb || (ya == yb && a < b);
+  std::vector<uint64_t> tmp(n);
+  for (int w = 1; w < parts; w *= 2) {
+    const int pairs = parts / (2 * w);
+    parallel_ranges((uint64_t)pairs, pairs, [&](uint64_t b, uint64_t e, int) {
+      for (uint64_t k = b; k < e; k++) {
+        const size_t lo = cut[2 * w * k], mid = cut[2 * w * k + w], hi = cut[2 * w * (k + 1)];
+        std::merge(v.begin() + lo, v.begin() + mid, v.begin() + mid, v.begin() + hi,
+                   tmp.begin() + lo);
+      }
     });
+    v.swap(tmp);
   }
 }
 
-RTree* rtree_build(const Box* boxes, uint64_t n, const uint8_t* alive, int fanout) {
+// STR packing of `items` (indices into `cb`, the item bounds) into groups of
+// `fanout`: items sorted by centre x (ties by index), then each slab of
+// ~sqrt(groups) groups sorted by centre y. NaN centres sort last.
+void str_order(std::vector<uint32_t>& items, const std::vector<Box>& cb, int fanout, int nt) {
+  const size_t n = items.size();
+  if (n <= (size_t)fanout) return;
+  const size_t leaves = (n + fanout - 1) / fanout;
+  const size_t slabs = (size_t)std::ceil(std::sqrt((double)leaves));
+  const size_t per_slab = slabs ? ((leaves + slabs - 1) / slabs) * fanout : n;
+  std::vector<uint64_t> key(n);
+  parallel_ranges(n, nt, [&](uint64_t b, uint64_t e, int) {
+    for (uint64_t i = b; i < e; i++) {
+      const Box& x = cb[items[i]];
+      key[i] = ((uint64_t)float_key(0.5f * (x.x0 + x.x1)) << 32) | items[i];
+    }
+  });
+  parallel_sort_u64(key, nt);
+  const uint64_t n_slabs = (n + per_slab - 1) / per_slab;
+  parallel_dynamic(n_slabs, nt, 1, [&](uint64_t sb, uint64_t se, int) {
+    for (uint64_t sl = sb; sl < se; sl++) {
+      const size_t s = sl * per_slab, e = std::min(n, s + per_slab);
+      for (size_t i = s; i < e; i++) {
+        const uint32_t it = (uint32_t)key[i];
+        const Box& x = cb[it];
+        key[i] = ((uint64_t)float_key(0.5f * (x.y0 + x.y1)) << 32) | it;
+      }
+      std::sort(key.begin() + s, key.begin() + e);
+    }
+  });
+  for (size_t i = 0; i < n; i++) items[i] = (uint32_t)key[i];
+}
+
+// Bounds and sums of every node from the entries: leaf nodes in parallel,
+// then inner nodes in index order (children always precede their parent).
+void rtree_refit(RTree* t) {
+  const size_t m = t->nodes.size();
+  parallel_ranges(m, resolve_threads(t->threads), [&](uint64_t b, uint64_t e, int) {
+    for (uint64_t v = b; v < e; v++) {
+      if (!t->is_leaf[v]) continue;
+      Box bx = kEmptyBox;
+      uint64_t c = 0, g = 0;
+      const uint32_t cb = t->child_begin[v], cc = t->child_count[v];
+      for (uint32_t i = cb; i < cb + cc; i++) {
+        const Box& x = t->entry_boxes[i];
+        if (box_has_nan(x)) continue;
+        bx = unite(bx, normalized(x));
+        c++;
+        g += ghash(t->entries[i]);
+      }
+      t->nodes[v] = bx;
+      t->node_cnt[v] = c;
+      t->node_gsum[v] = g;
+    }
+  });
+  for (size_t v = 0; v < m; v++) {
+    if (t->is_leaf[v]) continue;
+    Box bx = kEmptyBox;
+    uint64_t c = 0, g = 0;
+    const uint32_t cb = t->child_begin[v], cc = t->child_count[v];
+    for (uint32_t i = cb; i < cb + cc; i++) {
+      bx = unite(bx, t->nodes[i]);
+      c += t->node_cnt[i];
+      g += t->node_gsum[i];
+    }
+    t->nodes[v] = bx;
+    t->node_cnt[v] = c;
+    t->node_gsum[v] = g;
+  }
+}
+
+RTree* rtree_build(const Box* boxes, uint64_t n, const uint8_t* alive, int fanout, int threads) {
   auto* t = new RTree();
   t->fanout = fanout < 2 ? 16 : fanout;
+  t->threads = threads;
+  const int nt = resolve_threads(threads);
+  t->pos_of_id.assign(n, UINT32_MAX);
   std::vector<uint32_t> ids;
   ids.reserve(n);
   for (uint64_t i = 0; i < n; i++)
     if (!alive || alive[i]) ids.push_back((uint32_t)i);
   if (ids.empty()) return t;
   t->empty = false;
-  std::vector<Box> cb(boxes, boxes + n);
-  // Dead boxes stay out of the tree; NaN centres sort arbitrarily but are kept
-  // (their predicates are false anyway).
-  str_order(ids, cb, t->fanout);
+  {
+    std::vector<Box> cb(boxes, boxes + n);
+    // Dead boxes stay out of the tree; NaN centres sort last but are kept
+    // (their predicates are false anyway).
+    str_order(ids, cb, t->fanout, nt);
+  }
   t->entries = ids;
   t->entry_boxes.resize(ids.size());
-  for (size_t i = 0; i < ids.size(); i++) t->entry_boxes[i] = boxes[ids[i]];
-  // Level 0: leaves over entries.
+  for (size_t i = 0; i < ids.size(); i++) {
+    t->entry_boxes[i] = boxes[ids[i]];
+    t->pos_of_id[ids[i]] = (uint32_t)i;
+  }
+  // Level 0: leaves over entries (bounds and sums come from rtree_refit).
   std::vector<uint32_t> level;
   for (size_t s = 0; s < ids.size(); s += t->fanout) {
-    size_t e = std::min(ids.size(), s + (size_t)t->fanout);
+    const size_t e = std::min(ids.size(), s + (size_t)t->fanout);
     Box b = kEmptyBox;
-    for (size_t i = s; i < e; i++) {
-      const Box& x = t->entry_boxes[i];
-      if (std::isnan(x.x0) || std::isnan(x.y0) || std::isnan(x.x1) || std::isnan(x.y1)) continue;
-      // Cover the box's extent even when its corners are swapped.
-      Box nx{std::min(x.x0, x.x1), std::min(x.y0, x.y1), std::max(x.x0, x.x1), std::max(x.y0, x.y1)};
-      b = unite(b, nx);
-    }
+    for (size_t i = s; i < e; i++)
+      if (!box_has_nan(t->entry_boxes[i])) b = unite(b, normalized(t->entry_boxes[i]));
     level.push_back((uint32_t)t->nodes.size());
     t->nodes.push_back(b);
     t->child_begin.push_back((uint32_t)s);
@@ -395,21 +503,19 @@ RTree* rtree_build(const Box* boxes, uint64_t n, const uint8_t* alive, int fanou
     t->is_leaf.push_back(1);
   }
   while (level.size() > 1) {
-    std::vector<Box> lb(t->nodes.size());
-    for (size_t i = 0; i < t->nodes.size(); i++) lb[i] = t->nodes[i];
     std::vector<uint32_t> order = level;
-    str_order(order, lb, t->fanout);
+    str_order(order, t->nodes, t->fanout, nt);
     // Children of one parent must be contiguous in `nodes`: re-emit them.
     std::vector<uint32_t> next;
     std::vector<uint32_t> copies;
     for (size_t s = 0; s < order.size(); s += t->fanout) {
-      size_t e = std::min(order.size(), s + (size_t)t->fanout);
-      uint32_t first = (uint32_t)t->nodes.size();
+      const size_t e = std::min(order.size(), s + (size_t)t->fanout);
+      const uint32_t first = (uint32_t)t->nodes.size();
       for (size_t i = s; i < e; i++) {
-        uint32_t c = order[i];
-        Box cbox = t->nodes[c];
-        uint32_t cbeg = t->child_begin[c], ccnt = t->child_count[c];
-        uint8_t cleaf = t->is_leaf[c];
+        const uint32_t c = order[i];
+        const Box cbox = t->nodes[c];
+        const uint32_t cbeg = t->child_begin[c], ccnt = t->child_count[c];
+        const uint8_t cleaf = t->is_leaf[c];
         t->nodes.push_back(cbox);
         t->child_begin.push_back(cbeg);
         t->child_count.push_back(ccnt);
@@ -419,7 +525,7 @@ RTree* rtree_build(const Box* boxes, uint64_t n, const uint8_t* alive, int fanou
       copies.push_back((uint32_t)(e - s));
     }
     for (size_t k = 0; k < copies.size(); k += 2) {
-      uint32_t first = copies[k], cnt = copies[k + 1];
+      const uint32_t first = copies[k], cnt = copies[k + 1];
       Box b = kEmptyBox;
       for (uint32_t i = 0; i < cnt; i++) b = unite(b, t->nodes[first + i]);
       next.push_back((uint32_t)t->nodes.size());
@@ -431,7 +537,16 @@ RTree* rtree_build(const Box* boxes, uint64_t n, const uint8_t* alive, int fanou
     level.swap(next);
   }
   t->root = level[0];
+  t->node_cnt.assign(t->nodes.size(), 0);
+  t->node_gsum.assign(t->nodes.size(), 0);
+  rtree_refit(t);
   return t;
+}
+
+// Node bounds inside the query envelope (closed): every matchable box below
+// then intersects the query and lies within it.
+inline bool node_inside(const Box& n, const float* q) {
+  return q[0] <= n.x0 && q[1] <= n.y0 && n.x1 <= q[2] && n.y1 <= q[3];
 }
 
 // Necessary condition on a node's bounds for any descendant box to satisfy
@@ -466,6 +581,67 @@ void rtree_query_one(const RTree* t, int pred, int kind, const float* q, Emit&& 
       }
     }
   }
+}
+
+// Count and ghash sum of the boxes that satisfy pred against q. Envelope
+// intersects and within take a node whose bounds lie inside q whole, from its
+// stored count and sum; everything else is tested box by box.
+template <int PRED, int KIND>
+void rtree_checksum_t(const RTree* t, const float* qp, uint64_t* cnt, uint64_t* gsum,
+                      std::vector<uint32_t>& stack) {
+  uint64_t c = 0, g = 0;
+  constexpr bool kWhole = KIND == Q_ENVELOPE && (PRED == P_INTERSECTS || PRED == P_WITHIN);
+  const Box* nodes = t->nodes.data();
+  const uint32_t* cbeg = t->child_begin.data();
+  const uint32_t* ccnt = t->child_count.data();
+  const uint8_t* leaf = t->is_leaf.data();
+  const uint64_t* ncnt = t->node_cnt.data();
+  const uint64_t* ngs = t->node_gsum.data();
+  const Box* eb = t->entry_boxes.data();
+  const uint32_t* ent = t->entries.data();
+  const float q[4] = {qp[0], qp[1], KIND == Q_POINT ? qp[0] : qp[2], KIND == Q_POINT ? qp[1] : qp[3]};
+  stack.resize(64 * (size_t)t->fanout + 16);
+  uint32_t* st = stack.data();
+  size_t sp = 0;
+  if (t->empty || !node_may_match(PRED, KIND, nodes[t->root], q)) {
+    *cnt = 0;
+    *gsum = 0;
+    return;
+  }
+  st[sp++] = t->root;
+  while (sp) {
+    const uint32_t v = st[--sp];
+    if (kWhole && node_inside(nodes[v], q)) {
+      c += ncnt[v];
+      g += ngs[v];
+      continue;
+    }
+    const uint32_t b = cbeg[v], n = ccnt[v];
+    if (leaf[v]) {
+      for (uint32_t i = b; i < b + n; i++) {
+        if (eval(PRED, KIND, eb[i], q)) {
+          c++;
+          g += ghash(ent[i]);
+        }
+      }
+    } else {
+      for (uint32_t i = b; i < b + n; i++)
+        if (node_may_match(PRED, KIND, nodes[i], q)) st[sp++] = i;
+    }
+  }
+  *cnt = c;
+  *gsum = g;
+}
+
+void rtree_checksum_one(const RTree* t, int pred, int kind, const float* q, uint64_t* cnt,
+                        uint64_t* gsum, std::vector<uint32_t>& stack) {
+  if (kind == Q_POINT) return rtree_checksum_t<P_CONTAINS, Q_POINT>(t, q, cnt, gsum, stack);
+  if (kind == Q_ENVELOPE) {
+    if (pred == P_INTERSECTS) return rtree_checksum_t<P_INTERSECTS, Q_ENVELOPE>(t, q, cnt, gsum, stack);
+    if (pred == P_WITHIN) return rtree_checksum_t<P_WITHIN, Q_ENVELOPE>(t, q, cnt, gsum, stack);
+    return rtree_checksum_t<P_CONTAINS, Q_ENVELOPE>(t, q, cnt, gsum, stack);
+  }
+  return rtree_checksum_t<P_INTERSECTS, Q_LINE>(t, q, cnt, gsum, stack);
 }
 
 // ------------------------------------------- reference ray formulation ----
@@ -678,9 +854,80 @@ int orc_eval_f64(int pred, int kind, const double* box, const double* query) {
 }
 
 // -------------------------------------------------------------- R-tree ----
-void* orc_rtree_build(const float* boxes, uint64_t nb, const uint8_t* alive, int fanout) {
-  return rtree_build(reinterpret_cast<const Box*>(boxes), nb, alive, fanout);
+void* orc_rtree_build(const float* boxes, uint64_t nb, const uint8_t* alive, int fanout,
+                      int threads) {
+  return rtree_build(reinterpret_cast<const Box*>(boxes), nb, alive, fanout, threads);
 }
+
+// Gives the boxes `ids` new envelopes (a NaN box matches nothing: a delete)
+// and refits the tree. Ids that are not in the tree (dead at build time, or
+// beyond it) are ignored. Returns the number applied.
+uint64_t orc_rtree_update(void* tree, const uint64_t* ids, const float* boxes, uint64_t n) {
+  RTree* t = static_cast<RTree*>(tree);
+  uint64_t applied = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    if (ids[i] >= t->pos_of_id.size()) continue;
+    const uint32_t pos = t->pos_of_id[ids[i]];
+    if (pos == UINT32_MAX) continue;
+    t->entry_boxes[pos] = Box{boxes[4 * i], boxes[4 * i + 1], boxes[4 * i + 2], boxes[4 * i + 3]};
+    applied++;
+  }
+  if (!t->empty) rtree_refit(t);
+  return applied;
+}
+
+// Per query: count and ghash sum of its results (factored checksum); returns
+// the total count. qmult / ghash are exported for the checkers.
+uint64_t orc_rtree_checksums(void* tree, int pred, int kind, const float* queries, uint64_t nq,
+                             int threads, uint64_t* per_query, uint64_t* gsum) {
+  const RTree* t = static_cast<const RTree*>(tree);
+  const int qs = kind == Q_POINT ? 2 : 4;
+  const int nt = resolve_threads(threads);
+  // Queries in Z-order of their centres (over the tree's root bounds): a
+  // thread's consecutive queries then walk nearby nodes, which stay cached.
+  std::vector<uint64_t> order(nq);
+  {
+    const Box r = t->empty ? Box{0, 0, 1, 1} : t->nodes[t->root];
+    const float sx = r.x1 > r.x0 ? 65535.0f / (r.x1 - r.x0) : 0.0f;
+    const float sy = r.y1 > r.y0 ? 65535.0f / (r.y1 - r.y0) : 0.0f;
+    parallel_ranges(nq, nt, [&](uint64_t b, uint64_t e, int) {
+      for (uint64_t i = b; i < e; i++) {
+        const float* q = queries + i * qs;
+        const float cx = kind == Q_POINT ? q[0] : 0.5f * (q[0] + q[2]);
+        const float cy = kind == Q_POINT ? q[1] : 0.5f * (q[1] + q[3]);
+        const float fx = std::min(65535.0f, std::max(0.0f, (cx - r.x0) * sx));
+        const float fy = std::min(65535.0f, std::max(0.0f, (cy - r.y0) * sy));
+        // NaN centres land at 0 (the comparisons above fail): any order is fine.
+        const uint32_t ix = std::isnan(fx) ? 0u : (uint32_t)fx, iy = std::isnan(fy) ? 0u : (uint32_t)fy;
+        uint64_t z = 0;
+        for (int k = 0; k < 16; k++)
+          z |= (uint64_t)((ix >> k) & 1u) << (2 * k) | (uint64_t)((iy >> k) & 1u) << (2 * k + 1);
+        order[i] = (z << 32) | i;
+      }
+    });
+    parallel_sort_u64(order, nt);
+  }
+  std::vector<uint64_t> part(nt, 0);
+  parallel_dynamic(nq, nt, 64, [&](uint64_t qb, uint64_t qe, int th) {
+    std::vector<uint32_t> stack;
+    uint64_t tot = 0;
+    for (uint64_t k = qb; k < qe; k++) {
+      const uint64_t q = (uint32_t)order[k];
+      uint64_t c = 0, g = 0;
+      rtree_checksum_one(t, pred, kind, queries + q * qs, &c, &g, stack);
+      if (per_query) per_query[q] = c;
+      if (gsum) gsum[q] = g;
+      tot += c;
+    }
+    part[th] += tot;
+  });
+  uint64_t total = 0;
+  for (auto v : part) total += v;
+  return total;
+}
+
+uint64_t orc_ghash(uint64_t b) { return ghash(b); }
+uint64_t orc_qmult(uint64_t q) { return qmult(q); }
 
 void orc_rtree_free(void* t) { delete static_cast<RTree*>(t); }
 

@@ -56,8 +56,16 @@ def lib():
         L.orc_bf_pairs_f64.restype = u64
         L.orc_eval_f64.argtypes = [i32, i32, vp, vp]
         L.orc_eval_f64.restype = i32
-        L.orc_rtree_build.argtypes = [vp, u64, vp, i32]
+        L.orc_rtree_build.argtypes = [vp, u64, vp, i32, i32]
         L.orc_rtree_build.restype = vp
+        L.orc_rtree_update.argtypes = [vp, vp, vp, u64]
+        L.orc_rtree_update.restype = u64
+        L.orc_rtree_checksums.argtypes = [vp, i32, i32, vp, u64, i32, vp, vp]
+        L.orc_rtree_checksums.restype = u64
+        L.orc_ghash.argtypes = [u64]
+        L.orc_ghash.restype = u64
+        L.orc_qmult.argtypes = [u64]
+        L.orc_qmult.restype = u64
         L.orc_rtree_free.argtypes = [vp]
         L.orc_rtree_summary.argtypes = [vp, i32, i32, vp, u64, i32, vp, vp, vp]
         L.orc_rtree_summary.restype = u64
@@ -253,12 +261,14 @@ def segment_pairs(boxes, segs, alive=None):
 
 # ----------------------------------------------------------------- R-tree --
 class RTree:
-    """STR bulk-loaded R-tree (fan-out 16 by default) with exact FP32 bounds."""
+    """STR bulk-loaded R-tree (fan-out 16 by default) with exact FP32 bounds.
+    `threads` (0: every hardware thread) is used by the build and refits."""
 
-    def __init__(self, boxes, alive=None, fanout=16):
+    def __init__(self, boxes, alive=None, fanout=16, threads=0):
         self._boxes = _f32(boxes, 4)
         al = _alive(alive, self._boxes.shape[0])
-        self._h = lib().orc_rtree_build(_p(self._boxes), self._boxes.shape[0], _p(al), fanout)
+        self._h = lib().orc_rtree_build(_p(self._boxes), self._boxes.shape[0], _p(al), fanout,
+                                        threads)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -279,6 +289,30 @@ class RTree:
         queries = np.ascontiguousarray(queries, dtype=np.float32)
         return int(lib().orc_rtree_summary(self._h, pred, _kind(queries), _p(queries),
                                            queries.shape[0], threads, None, None, None))
+
+    def checksums(self, pred, queries, threads=0):
+        """(total, per-query counts, per-query ghash sums): the factored
+        checksum of each query's result set (see pair_checksums); envelope
+        intersects / within count subtrees inside the query whole."""
+        queries = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = queries.shape[0]
+        cnt = np.zeros(nq, np.uint64)
+        gs = np.zeros(nq, np.uint64)
+        total = lib().orc_rtree_checksums(self._h, pred, _kind(queries), _p(queries), nq, threads,
+                                          _p(cnt), _p(gs))
+        return int(total), cnt, gs
+
+    def update(self, ids, boxes):
+        """Boxes `ids` take the new envelopes (NaN: deleted), then the tree is
+        refitted. Ids not in the tree are ignored; returns how many applied."""
+        ids = np.ascontiguousarray(ids, dtype=np.uint64)
+        b = _f32(boxes, 4)
+        assert b.shape[0] == ids.shape[0]
+        return int(lib().orc_rtree_update(self._h, _p(ids), _p(b), ids.shape[0]))
+
+    def delete(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.uint64)
+        return self.update(ids, np.full((ids.shape[0], 4), np.nan, np.float32))
 
     def pairs(self, pred, queries, threads=0):
         queries = np.ascontiguousarray(queries, dtype=np.float32)
@@ -328,3 +362,37 @@ def _splitmix_np(x):
         x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         return x ^ (x >> np.uint64(31))
+
+
+# ------------------------------------------------ factored pair checksum --
+# A pair (query q, box b) hashes to qmult(q) * ghash(b) mod 2^64 with
+# ghash(b) = splitmix64(b ^ 0xD1B54A32D192ED03) and qmult(q) = splitmix64(q) | 1.
+# A query's result set is summarised by its count and gsum = sum of ghash(b);
+# a request's checksum is sum over its queries of qmult(q) * gsum(q).
+_GSALT = np.uint64(0xD1B54A32D192ED03)
+
+
+def ghash_np(b):
+    return _splitmix_np(np.asarray(b, dtype=np.uint64) ^ _GSALT)
+
+
+def qmult_np(q):
+    return _splitmix_np(np.asarray(q, dtype=np.uint64)) | np.uint64(1)
+
+
+def pair_checksums(pairs, nq):
+    """Per-query counts and ghash sums of a (query_id, box_id) pair array."""
+    pairs = np.asarray(pairs, dtype=np.uint64).reshape(-1, 2)
+    q = pairs[:, 0].astype(np.int64)
+    cnt = np.bincount(q, minlength=nq).astype(np.uint64)
+    gs = np.zeros(nq, np.uint64)
+    np.add.at(gs, q, ghash_np(pairs[:, 1]))
+    return cnt, gs
+
+
+def request_checksum(gsums, qids=None):
+    """sum over queries of qmult(q) * gsum(q), q = 0..n-1 or `qids`."""
+    gsums = np.asarray(gsums, dtype=np.uint64)
+    q = np.arange(gsums.shape[0], dtype=np.uint64) if qids is None else np.asarray(qids, np.uint64)
+    with np.errstate(over="ignore"):
+        return int(np.sum(qmult_np(q) * gsums, dtype=np.uint64))

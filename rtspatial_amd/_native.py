@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("RTS_LIBRARY") or os.path.join(_HERE, "librtspatial_amd.so")
 
 RTS_OK = 0
+RTS_ERR_INVALID, RTS_ERR_HIP, RTS_ERR_DEVICE, RTS_ERR_MISSING = 1, 2, 3, 4
 RTS_INTERSECTS, RTS_CONTAINS, RTS_WITHIN = 0, 1, 2
 RTS_HANDLER_COUNT, RTS_HANDLER_COLLECT = 0, 1
 RTS_REBUILD_SAH, RTS_REBUILD_NEVER, RTS_REBUILD_ALWAYS = 0, 1, 2
@@ -199,6 +200,7 @@ SIGNATURES = [
     ("rts_index_update", _I32, [_VP, _VP, _U64, _VP]),
     ("rts_index_query_points", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_query_envelopes", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_pairs_checksum", _I32, [_VP, _U64, _U64, _VP, _VP, _VP]),
     ("rts_index_insert_d", _I32, [_VP, _VP, _U64, _VP]),
     ("rts_index_update_d", _I32, [_VP, _VP, _U64, _VP]),
     ("rts_index_query_points_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),

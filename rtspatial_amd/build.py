@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librtspatial_amd.so")
 SOURCES = ["sort.hip", "build.hip", "query.hip", "scan.hip", "index.hip", "profile.hip",
-           "batch.hip", "server.hip", "coord64.hip"]
+           "batch.hip", "server.hip", "coord64.hip", "checksum.hip"]
 ARCH = os.environ.get("RTS_OFFLOAD_ARCH", "gfx950")
 
 

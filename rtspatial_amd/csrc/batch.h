@@ -84,7 +84,7 @@ struct BatchSort {
 // view.
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a, const BatchSort& sort,
                         hipStream_t stream);
-// hashes[c] = sum of splitmix64((query_id << 32) | geom_id) over range c of
+// hashes[c] = sum of pair_hash(query_id, geom_id) (checksum.h) over range c of
 // the arena (ranges as written by the batch: {first slot, count}, in order).
 hipError_t launch_hash_ranges(const rts_pair* arena, uint64_t cap, const uint64_t* ranges,
                               const uint32_t* n_ranges, const uint64_t* total,

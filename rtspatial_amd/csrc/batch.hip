@@ -17,6 +17,7 @@
 // and reads the batch's sizes from device memory, so one captured hipGraph
 // replays any batch.
 #include "batch.h"
+#include "checksum.h"
 #include "morton.h"
 #include "rtspatial/device/traverse.h"
 #include "rtspatial/device/wave.h"
@@ -102,7 +103,11 @@ __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const Batch
   const BatchHeader h = *a.header;
   const uint32_t lo = CLS == 0 ? 0u : h.class_end[CLS > 0 ? CLS - 1 : 0];
   const uint32_t hi = h.class_end[CLS];
-  if (lo >= hi || h.n_requests == 0 || t.n == 0) return;
+  if (lo >= hi || h.n_requests == 0) return;
+  // An empty index (no boxes inserted, or cleared) has no tree: the count
+  // pass still writes every query's count (zero), the write pass has nothing
+  // to write.
+  if (WRITE && t.n == 0) return;
   uint32_t* counters = a.tile_counters + ((WRITE ? kClasses : 0) + CLS) * kXcds;
   const uint32_t xcd = blockIdx.x % kXcds;
   // Every iteration consumes a tile (or finds none left), so hi - lo + 1
@@ -114,6 +119,15 @@ __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const Batch
     const uint32_t ri = find_request(a.requests, h.n_requests, tile);
     BatchRequest r = a.requests[ri];
     const uint32_t local = tile - r.tbase;
+    if (t.n == 0) {
+      if (kWaveCls) {
+        if (local < r.n && lane_id() == 0) a.counts[r.qbase + local] = 0;
+      } else {
+        const uint32_t q = local * kWave + lane_id();
+        if (q < r.n) a.counts[r.qbase + q] = 0;
+      }
+      continue;
+    }
     const uint32_t lb = (r.flags >> kReqLeafShift) & 7u;
     t.leaf_size = lb ? 1u << (lb - 1) : t0.leaf_size;
     if (CLS == kClassWaveIntersects) {
@@ -310,16 +324,9 @@ size_t batch_seg_sums_words() { return kScanBlocks; }
 
 namespace {
 
-__device__ __forceinline__ uint64_t splitmix(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-
 // hashes[c] = sum over the pairs of range c (ranges[2c] = first slot,
 // ranges[2c+1] = count; ranges tile [0, *total) in order) of
-// splitmix64((query_id << 32) | geom_id). hashes must be zero.
+// pair_hash(query_id, geom_id) (checksum.h). hashes must be zero.
 // A wavefront takes blocks of kHashBlock consecutive slots (lanes interleaved,
 // so loads coalesce). Each lane keeps a running sum for its current range and
 // adds it to hashes[] only when its range changes, so a request's pairs cost
@@ -362,7 +369,7 @@ __global__ void __launch_bounds__(kThreads) k_hash_ranges(const rts_pair* __rest
       }
       if (i >= end) continue;  // a slot in no range (ranges tile [0, total): not expected)
       const rts_pair p = arena[i];
-      h += splitmix(((uint64_t)p.query_id << 32) | p.geom_id);
+      h += pair_hash(p.query_id, p.geom_id);
     }
     const uint32_t a0 = readlane_u32(a, 0);
     if (__all(a == a0)) {
@@ -387,7 +394,7 @@ __global__ void __launch_bounds__(kThreads) k_rebase_hash_span(rts_pair* __restr
     rts_pair p = arena[i];
     p.query_id += add;
     if (add) arena[i] = p;
-    if (do_hash) h += splitmix(((uint64_t)p.query_id << 32) | p.geom_id);
+    if (do_hash) h += pair_hash(p.query_id, p.geom_id);
   }
   if (do_hash) {
     h = wave_sum_u64(h);

@@ -590,8 +590,13 @@ struct QueryScratch {
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Carves the scratch of a query call over nq queries. With `ext` (the
+// allocation-free entry point) the block is ext's first ext_bytes bytes, and
+// a call needing more is an error; otherwise it is allocated on `stream`
+// (and qs->block is freed by the caller).
 int alloc_scratch(uint64_t nq, bool sort, bool wave, bool two_pass, hipStream_t stream,
-                  QueryScratch* qs, size_t* order_off, size_t* sort_off) {
+                  QueryScratch* qs, size_t* order_off, size_t* sort_off, char* ext = nullptr,
+                  size_t ext_bytes = 0, char** base_out = nullptr) {
   size_t off = 0;
   size_t o_keys = 0, o_sort = 0, o_counts = 0, o_offsets = 0, o_total = 0, o_scan = 0, o_tk = 0;
   if (wave) {
@@ -614,9 +619,16 @@ int alloc_scratch(uint64_t nq, bool sort, bool wave, bool two_pass, hipStream_t 
     o_scan = off;
     off = align_up(off + scan_scratch_bytes(nq));
   }
-  if (off == 0) return RTS_OK;
-  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&qs->block), off, stream));
-  char* b = qs->block;
+  char* b = nullptr;
+  if (ext) {
+    RTS_CHECK_ARG(off <= ext_bytes, "query: the call's scratch exceeds the static scratch array");
+    b = ext;
+  } else {
+    if (off == 0) return RTS_OK;
+    RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&qs->block), off, stream));
+    b = qs->block;
+  }
+  if (base_out) *base_out = b;
   if (wave) qs->tickets = reinterpret_cast<uint32_t*>(b + o_tk);
   if (two_pass) {
     qs->counts = reinterpret_cast<uint32_t*>(b + o_counts);
@@ -699,8 +711,11 @@ size_t query_bytes(bool points, bool f64) { return (points ? 8 : 16) * (f64 ? 2 
 
 // Runs one query call under `plan`; the caller holds the index lock and has
 // brought the tree up to date. f64: double queries on an exact double index.
+// ext / ext_bytes: device scratch supplied by the caller (nothing is then
+// allocated; the call must fit it), else scratch is allocated per call.
 int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uint64_t nq,
-             bool points, bool f64, rts_handler handler, void* arg, hipStream_t stream) {
+             bool points, bool f64, rts_handler handler, void* arg, hipStream_t stream,
+             char* ext = nullptr, size_t ext_bytes = 0) {
   const bool sort = plan.sort;
   const int trav = plan.traversal;
   // Collecting wave walks count, scan and then write each query's pairs
@@ -717,12 +732,13 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
     const void* qp = static_cast<const char*>(queries) + off * query_bytes(points, f64);
     QueryScratch qs;
     size_t o_keys = 0, o_sort = 0;
+    char* base = nullptr;
     int rc = alloc_scratch(m, sort, trav == RTS_TRAVERSAL_WAVE, two_pass, stream, &qs, &o_keys,
-                           &o_sort);
+                           &o_sort, ext, ext_bytes, &base);
     if (rc) return rc;
     if (sort) {
       PhaseToken ph = phase_begin(RTS_PHASE_QUERY_ORDER, stream);
-      rc = order_queries(ix, qp, m, points, f64, qs.block + o_keys, qs.block + o_sort, stream,
+      rc = order_queries(ix, qp, m, points, f64, base + o_keys, base + o_sort, stream,
                          &qs.order);
       phase_end(ph, stream);
       if (rc) return rc;
@@ -1063,7 +1079,7 @@ void rts_tune_config_default(rts_tune_config* c) {
   if (!c) return;
   c->sample = 32768;
   c->leaf_sizes = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5);
-  c->reps = 3;
+  c->reps = 2;
   c->apply = 1;
 }
 
@@ -1089,11 +1105,13 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   rc = ensure_built(ix, stream);
   if (rc) return rc;
 
-  // An evenly strided sample, shrunk until its results fit a 2^30-pair queue.
-  // A double index's sample is taken in doubles; one that rounds doubles to
-  // floats times the rounded sample.
+  // An evenly strided sample, shrunk until its results fit kMaxPairs (about
+  // 1 GiB of pairs: one timed run then writes at most that much, which keeps
+  // a run near a millisecond or two on the 10M-box index). A double index's
+  // sample is taken in doubles; one that rounds doubles to floats times the
+  // rounded sample.
   const size_t qbytes = query_bytes(pts, ix->f64());
-  const uint64_t kMaxPairs = 1ull << 30;
+  const uint64_t kMaxPairs = 1ull << 27;
   TuneBuffers tb;
   RTS_HIP(hipMalloc(&tb.small, 64));
   RTS_HIP(hipEventCreate(&tb.ev[0]));
@@ -1121,8 +1139,8 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     if (rc) return rc;
     RTS_HIP(hipMemcpyAsync(&total, counter, sizeof(total), hipMemcpyDeviceToHost, stream));
     RTS_HIP(hipStreamSynchronize(stream));
-    if (handler == RTS_HANDLER_COUNT || total <= kMaxPairs || m <= 64 || attempt == 3) break;
-    m = std::max<uint64_t>(64, (uint64_t)((double)m * (double)kMaxPairs / (double)total * 0.9));
+    if (handler == RTS_HANDLER_COUNT || total <= kMaxPairs || m <= 4096 || attempt == 3) break;
+    m = std::max<uint64_t>(4096, (uint64_t)((double)m * (double)kMaxPairs / (double)total * 0.9));
   }
   void* arg = counter;
   if (handler == RTS_HANDLER_COLLECT) {
@@ -1133,14 +1151,18 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     arg = tb.small;
   }
 
-  // Time one plan: one untimed run, then the fastest of `reps` runs. A plan
-  // whose untimed run is already 3x slower than the best so far is not
-  // repeated.
+  // Time one plan: the fastest of `reps` runs, after an untimed one when
+  // the plan's traversal mode has not run yet (its kernels then load and
+  // warm up). A plan whose first run is already 3x slower than the best so
+  // far is not repeated.
   double best = 1e300;
+  bool warm[3] = {false, false, false};
   auto time_plan = [&](const Plan& p, double* ms_out) -> int {
     float ms = 0.0f;
     double fastest = 1e300;
-    for (uint32_t r = 0; r <= cfg.reps; ++r) {
+    const uint32_t r0 = warm[p.traversal] ? 1u : 0u;
+    warm[p.traversal] = true;
+    for (uint32_t r = r0; r <= cfg.reps; ++r) {
       RTS_HIP(hipMemsetAsync(counter, 0, sizeof(*counter), stream));
       RTS_HIP(hipEventRecord(tb.ev[0], stream));
       int e = run_plan(ix, p, pred, tb.sample, m, pts, f64, handler, arg, stream);
@@ -1148,7 +1170,7 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
       RTS_HIP(hipEventRecord(tb.ev[1], stream));
       RTS_HIP(hipEventSynchronize(tb.ev[1]));
       RTS_HIP(hipEventElapsedTime(&ms, tb.ev[0], tb.ev[1]));
-      if (r == 0 && ms > 3.0 * best) {
+      if (r == r0 && ms > 3.0 * best) {
         fastest = ms;
         break;
       }
@@ -1173,7 +1195,14 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   const bool whole = m == n;
   const int other =
       base.traversal == RTS_TRAVERSAL_WAVE ? (int)RTS_TRAVERSAL_THREAD : (int)RTS_TRAVERSAL_WAVE;
+  // A lane emits its query's results one per step, while a wavefront scans
+  // 512 leaves per step: queries averaging more than kLaneMaxResults results
+  // are not timed one per lane (on the 10M-box index the per-lane probe of
+  // intersects queries with ~1e4 results each took 446 ms against 6 ms).
+  constexpr double kLaneMaxResults = 256.0;
+  const bool lane_hopeless = (double)total > kLaneMaxResults * (double)m;
   for (int trav : {base.traversal, other}) {
+    if (trav == RTS_TRAVERSAL_THREAD && lane_hopeless) continue;
     bool first = true;
     for (uint32_t k = 1; k <= 6; ++k) {
       if (!(cfg.leaf_sizes & (1u << k))) continue;

@@ -130,22 +130,24 @@ __global__ void __launch_bounds__(kThreads) k_query_thread(const QueryLaunch q) 
           // right child first so that the left one is expanded next
           if (node_test<PRED, POINTS>(rbx, qb)) {
             if (m.y & kLeafFlag) {
-              if (sp < kStackDepth) stack[sp++] = m.y;
+              lane_push(stack, sp, m.y, t.error);
             } else {
               const uint32_t size = m.w - gamma;
-              if (sp < kStackDepth)
-                stack[sp++] = (size <= t.leaf_size || inside_test<PRED, POINTS>(rbx, qb))
-                                  ? (m.y | kRangeFlag) : m.y;
+              lane_push(stack, sp,
+                        (size <= t.leaf_size || inside_test<PRED, POINTS>(rbx, qb))
+                            ? (m.y | kRangeFlag) : m.y,
+                        t.error);
             }
           }
           if (node_test<PRED, POINTS>(lb, qb)) {
             if (m.x & kLeafFlag) {
-              if (sp < kStackDepth) stack[sp++] = m.x;
+              lane_push(stack, sp, m.x, t.error);
             } else {
               const uint32_t size = gamma - m.z + 1;
-              if (sp < kStackDepth)
-                stack[sp++] = (size <= t.leaf_size || inside_test<PRED, POINTS>(lb, qb))
-                                  ? (m.x | kRangeFlag) : m.x;
+              lane_push(stack, sp,
+                        (size <= t.leaf_size || inside_test<PRED, POINTS>(lb, qb))
+                            ? (m.x | kRangeFlag) : m.x,
+                        t.error);
             }
           }
         }

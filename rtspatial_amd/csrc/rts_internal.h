@@ -158,5 +158,13 @@ struct PhaseToken {
 };
 PhaseToken phase_begin(int phase, hipStream_t stream);
 void phase_end(const PhaseToken& t, hipStream_t stream);
+// While one is alive, phase_begin on this thread records nothing (a call
+// that must not create events, e.g. one that allocates nothing).
+struct PhasesOff {
+  PhasesOff();
+  ~PhasesOff();
+  PhasesOff(const PhasesOff&) = delete;
+  PhasesOff& operator=(const PhasesOff&) = delete;
+};
 
 }  // namespace rts

@@ -5,11 +5,10 @@
 // Batched mode is continuous batching over `slots` streams. Each slot owns
 // device buffers for one batch (descriptors, per-query counts and offsets,
 // result arena) and a pinned staging area; its launch sequence (descriptor
-// upload, count pass, scan, request ranges, write pass, range download) has
-// fixed grids and reads the batch's sizes from device memory, so it can be
-// captured once as a hipGraph and replayed for every batch (use_graphs); by
-// default it is launched directly, because replay did not complete on the
-// MI355X and the cause is not established.
+// upload, count pass, scan, request ranges, write pass, range download) is
+// launched directly for every batch. (Replaying that sequence from a hipGraph
+// did not complete on the MI355X, with the cause not established, so
+// rts_serve_run refuses use_graphs.)
 //
 // Admission: arrived requests are taken in arrival order and split into
 // chunks of at most chunk_queries queries; chunks join the batch while their
@@ -106,7 +105,6 @@ int kind_of(const rts_request& r) { return r.points ? 3 : (r.predicate & 3); }
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  hipGraphExec_t exec = nullptr;
   char* h_desc = nullptr;                // pinned: [TreeView | BatchHeader | BatchRequest x max]
   uint64_t* h_ranges = nullptr;          // pinned: 2 x max
   unsigned long long* h_hash = nullptr;  // pinned: max
@@ -354,8 +352,7 @@ BatchArgs slot_args(const Engine& E, const Slot& s) {
 int record_slot_work(Engine& E, Slot& s, size_t up_bytes, size_t n_req) {
   SRV_HIP(hipMemcpyAsync(s.d_desc, s.h_desc, up_bytes, hipMemcpyHostToDevice, s.stream));
   const BatchArgs a = slot_args(E, s);
-  // Morton-order the wave tiles of large batches (not under graph capture,
-  // whose grids must not depend on the batch).
+  // Morton-order the wave tiles of large batches.
   const BatchHeader* h = reinterpret_cast<const BatchHeader*>(s.h_desc + kHeaderOff);
   const uint32_t wave_tiles = h->class_end[kClassWavePoints];
   BatchSort sort{};
@@ -367,7 +364,7 @@ int record_slot_work(Engine& E, Slot& s, size_t up_bytes, size_t n_req) {
   sort_scratch_carve(s.d_sort + 4 * maxq, maxq, &sort.ss);
   sort.ss.error = E.ix->error;  // a look-back that gives up is reported after the run
   sort.scene = E.ix->scene;
-  sort.n = (!E.cfg.use_graphs && wave_tiles >= 65536) ? wave_tiles : 0;
+  sort.n = wave_tiles >= 65536 ? wave_tiles : 0;
   SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, sort, s.stream));
   SRV_HIP(hipMemcpyAsync(s.h_ranges, s.d_ranges, 2 * n_req * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, s.stream));
@@ -402,29 +399,11 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipMalloc(&s.d_sort, 4 * maxq * sizeof(uint32_t) + sort_scratch_bytes(maxq)));
   std::memset(s.h_desc, 0, E.desc_bytes);
   SRV_TRACE("slot buffers ready (arena %llu pairs)", (unsigned long long)E.cfg.arena_pairs);
-  if (E.cfg.use_graphs) {
-    // Run the sequence once outside capture (an empty batch) so that every
-    // kernel is loaded before the stream is captured.
-    const int wrc = record_slot_work(E, s, kReqsOff, 1);
-    if (wrc) return wrc;
-    SRV_HIP(hipStreamSynchronize(s.stream));
-    SRV_TRACE("warm-up done; capturing");
-    SRV_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-    const int rc = record_slot_work(E, s, E.desc_bytes, maxr);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(s.stream, &g);
-    if (rc) return rc;
-    SRV_HIP(ec);
-    SRV_HIP(hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0));
-    SRV_HIP(hipGraphDestroy(g));
-    SRV_TRACE("graph instantiated");
-  }
   return RTS_OK;
 }
 
 void slot_free(Slot& s) {
   if (s.stream) (void)hipStreamSynchronize(s.stream);
-  if (s.exec) (void)hipGraphExecDestroy(s.exec);
   if (s.h_desc) (void)hipHostFree(s.h_desc);
   if (s.h_ranges) (void)hipHostFree(s.h_ranges);
   if (s.h_hash) (void)hipHostFree(s.h_hash);
@@ -576,9 +555,7 @@ int launch_slot(Engine& E, Slot& s, std::vector<Chunk> chunks) {
   h->pad = 0;
   h->n_queries = qb;
   if (E.have_upd_event) SRV_HIP(hipStreamWaitEvent(s.stream, E.upd_event, 0));
-  if (E.cfg.use_graphs) {
-    SRV_HIP(hipGraphLaunch(s.exec, s.stream));
-  } else {
+  {
     const int rc = record_slot_work(E, s, kReqsOff + chunks.size() * sizeof(BatchRequest),
                                     chunks.size());
     if (rc) return rc;
@@ -766,8 +743,7 @@ void rts_serve_config_default(rts_serve_config* c) {
   c->mode = RTS_SERVE_BATCHED;
   c->slots = 3;
   c->max_batch_requests = 4096;
-  // hipGraph replay of the batch sequence is off by default: a run with it on
-  // did not complete on the MI355X and the cause has not been established.
+  // hipGraph replay of the batch sequence is refused (see rts_serve_run).
   c->use_graphs = 0;
   c->max_batch_queries = 4ull << 20;
   c->chunk_queries = 256ull << 10;
@@ -796,6 +772,14 @@ int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request*
       E.cfg.chunk_queries == 0 || E.cfg.arena_pairs == 0 || E.cfg.head_share_pct == 0 ||
       E.cfg.head_share_pct > 100) {
     set_error("rts_serve_run: invalid configuration");
+    return RTS_ERR_INVALID;
+  }
+  // hipGraph replay of a slot's launch sequence is refused: a run with it did
+  // not complete on the MI355X (the host stayed blocked inside a HIP call)
+  // and the cause has not been established.
+  if (E.cfg.use_graphs != 0) {
+    set_error("rts_serve_run: use_graphs must be 0 (hipGraph replay of the batch sequence "
+              "did not complete on the MI355X; the cause is not established)");
     return RTS_ERR_INVALID;
   }
   for (uint64_t i = 0; i < n_reqs; i++) {

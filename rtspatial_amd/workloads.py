@@ -134,3 +134,63 @@ def request_trace(n_requests, rate_hz, seed_cfg=5, max_batch=1_000_000,
     size = np.rint(np.exp(rng.uniform(0.0, math.log(max_batch), size=n_requests))).astype(np.int64)
     size = np.maximum(size, 1)
     return t, kind, size
+
+
+class ServedStream:
+    """c5: the served mixed stream over the 10M-box Gaussian-mixture index.
+
+    Boxes: n_boxes mixture boxes (seed 3001, sides LN(ln 1e-4, 1)). Query
+    pools: pool_n envelopes and pool_n points from the same mixture (seed
+    5002, sides LN(ln 5e-3, 1)). Requests (seed 5004): kind 0 = point
+    (contains), 1 = within, 2 = intersects with probabilities `mix`, size
+    round(exp(U[0, ln max_request])), each a contiguous slice of its pool at a
+    random offset, arrival gaps exponential with mean 1 (divided by the
+    offered rate). Update batches (seed 5003): batch b gives the next
+    `update_size` ids of one random permutation of the boxes their original
+    box shifted by U[-1e-3, 1e-3] per axis; they arrive at 10 Hz."""
+
+    def __init__(self, n_boxes=10_000_000, n_requests=600, max_request=1_000_000,
+                 pool_n=1 << 20, mix=(0.4, 0.3, 0.3), n_update_batches=100,
+                 update_size=100_000):
+        rb = np.random.default_rng(3001)
+        self.means = mixture_means(rb)
+        self.boxes = mixture_boxes(rb, self.means, n_boxes, math.log(1e-4))
+        rq = np.random.default_rng(5002)
+        self.pool_n = pool_n
+        self.env_pool = mixture_boxes(rq, self.means, pool_n, math.log(5e-3))
+        self.pt_pool = mixture_boxes(rq, self.means, pool_n, math.log(5e-3))[:, :2].copy()
+        rt = np.random.default_rng(5004)
+        self.kinds = rt.choice(3, size=n_requests, p=list(mix))
+        self.sizes = np.maximum(
+            1, np.rint(np.exp(rt.uniform(0, math.log(max_request), n_requests)))).astype(np.int64)
+        self.sizes = np.minimum(self.sizes, pool_n)
+        self.offsets = rt.integers(0, pool_n - self.sizes + 1)
+        self.gaps = rt.exponential(1.0, size=n_requests)
+        ru = np.random.default_rng(5003)
+        perm = ru.permutation(n_boxes)
+        self.updates = []
+        for b in range(n_update_batches):
+            s = (b * update_size) % n_boxes
+            ids = np.sort(perm[s:s + update_size]).astype(np.int64)
+            sh = ru.uniform(-1e-3, 1e-3, size=(ids.shape[0], 2)).astype(np.float32)
+            nb = self.boxes[ids].copy()
+            nb[:, [0, 2]] += sh[:, :1]
+            nb[:, [1, 3]] += sh[:, 1:]
+            self.updates.append((ids, nb))
+
+    def predicate(self, i):
+        """Oracle predicate code of request i: contains (points), within, intersects."""
+        return (1, 2, 0)[int(self.kinds[i])]
+
+    def queries(self, i):
+        o, n = int(self.offsets[i]), int(self.sizes[i])
+        return self.pt_pool[o:o + n] if self.kinds[i] == 0 else self.env_pool[o:o + n]
+
+    def arrivals(self, rate):
+        """Arrival times at `rate` requests per second (0: all at t = 0)."""
+        return np.cumsum(self.gaps) / rate if rate else np.zeros(self.gaps.shape[0])
+
+    def update_times(self, horizon_s):
+        """Arrival times of the update batches that arrive within horizon_s."""
+        n = min(len(self.updates), int(horizon_s * 10) + 1)
+        return [0.1 * i for i in range(n)]

@@ -115,7 +115,7 @@ def test_tune_config_defaults_and_null_args():
     from rtspatial_amd import _native as N
     c = N.rts_tune_config()
     N.lib().rts_tune_config_default(ctypes.byref(c))
-    assert c.sample == 32768 and c.reps == 3 and c.apply == 1
+    assert c.sample == 32768 and c.reps == 2 and c.apply == 1
     assert c.leaf_sizes == (1 << 2) | (1 << 3) | (1 << 4) | (1 << 5)
     r = N.rts_tune_result()
     assert N.lib().rts_index_autotune(None, 0, None, 1, 0, 1, None, ctypes.byref(r), None) == 1

@@ -217,6 +217,98 @@ def test_summaries_agree(oracle):
     assert t2 == total and np.array_equal(cnt3, cnt) and np.array_equal(hs3, hs) and np.array_equal(hx3, hx)
 
 
+def test_factored_checksum_functions(oracle):
+    # ghash(b) = splitmix64(b ^ salt): at b = salt it is splitmix64(0), the
+    # published first SplitMix64 output; the C and numpy forms agree, and
+    # qmult is odd.
+    salt = 0xD1B54A32D192ED03
+    assert oracle.lib().orc_ghash(salt) == 0xE220A8397B1DCDAF
+    vals = np.array([0, 1, 2, 3, 1 << 31, (1 << 32) - 1, 12345678, salt & 0xFFFFFFFF], np.uint64)
+    for v in vals:
+        assert oracle.lib().orc_ghash(int(v)) == int(oracle.ghash_np(v))
+        assert oracle.lib().orc_qmult(int(v)) == int(oracle.qmult_np(v))
+        assert int(oracle.qmult_np(v)) & 1
+    # A request's checksum is the per-pair sum of qmult(q) * ghash(b).
+    rng = np.random.default_rng(3)
+    pairs = np.stack([rng.integers(0, 50, 4000), rng.integers(0, 1 << 31, 4000)], 1)
+    cnt, gs = oracle.pair_checksums(pairs, 50)
+    with np.errstate(over="ignore"):
+        direct = int(np.sum(oracle.qmult_np(pairs[:, 0]) * oracle.ghash_np(pairs[:, 1]),
+                            dtype=np.uint64))
+    assert oracle.request_checksum(gs) == direct
+    assert int(cnt.sum()) == 4000
+    # One substituted box id changes the checksum.
+    p2 = pairs.copy()
+    p2[17, 1] += 1
+    assert oracle.request_checksum(oracle.pair_checksums(p2, 50)[1]) != direct
+
+
+def _adversarial_boxes(rng, n):
+    """Boxes on a 1/64 grid (edges that exactly touch grid-aligned queries),
+    some zero-width, some with swapped corners, some with a NaN coordinate."""
+    g = rng.integers(0, 64, size=(n, 2)).astype(np.float32) / np.float32(64)
+    w = rng.integers(0, 4, size=(n, 2)).astype(np.float32) / np.float32(64)
+    b = np.concatenate([g, g + w], 1).astype(np.float32)
+    sw = rng.random(n) < 0.05
+    b[sw] = b[sw][:, [2, 3, 0, 1]]
+    nan = rng.random(n) < 0.03
+    b[nan, rng.integers(0, 4, size=int(nan.sum()))] = np.nan
+    return b
+
+
+def test_rtree_checksums_equal_brute_force(oracle):
+    # Whole-subtree counting must agree with enumerating every pair: grid
+    # queries touch box edges exactly (closed intervals), and NaN, swapped,
+    # zero-width and dead boxes are all present; several fan-outs move the
+    # shortcut to different levels.
+    rng = np.random.default_rng(11)
+    b = _adversarial_boxes(rng, 6000)
+    alive = (rng.random(6000) > 0.1).astype(np.uint8)
+    qg = rng.integers(0, 64, size=(400, 2)).astype(np.float32) / np.float32(64)
+    qw = rng.integers(0, 24, size=(400, 2)).astype(np.float32) / np.float32(64)
+    qe = np.concatenate([qg, qg + qw], 1).astype(np.float32)
+    qe[:10] = np.float32([0, 0, 1, 1])           # covers every box: the root is inside
+    qe[10:15] = qe[10:15][:, [2, 3, 0, 1]]       # swapped query corners
+    pts = rng.integers(0, 65, size=(300, 2)).astype(np.float32) / np.float32(64)
+    for fan in (2, 4, 16):
+        tree = oracle.RTree(b, alive=alive, fanout=fan)
+        for pred, q in ((oracle.INTERSECTS, qe), (oracle.WITHIN, qe), (oracle.CONTAINS, qe),
+                        (oracle.CONTAINS, pts)):
+            want = oracle.pair_checksums(oracle.bf_pairs(pred, b, q, alive=alive), q.shape[0])
+            total, cnt, gs = tree.checksums(pred, q)
+            assert np.array_equal(cnt, want[0]) and np.array_equal(gs, want[1]), (fan, pred)
+            assert total == int(want[0].sum())
+
+
+def test_rtree_update_equals_rebuild(oracle):
+    # In-place updates (moves, NaN deletes, revivals) plus a refit answer
+    # exactly as a tree built over the updated boxes.
+    rng = np.random.default_rng(12)
+    b = _adversarial_boxes(rng, 5000)
+    alive = (rng.random(5000) > 0.1).astype(np.uint8)
+    tree = oracle.RTree(b, alive=alive)
+    cur = b.copy()
+    for step in range(3):
+        ids = rng.choice(5000, 700, replace=False).astype(np.uint64)
+        nb = _adversarial_boxes(rng, 700)
+        nb[:50] = np.nan
+        applied = tree.update(ids, nb)
+        live = alive[ids.astype(np.int64)] == 1
+        assert applied == int(live.sum())
+        cur[ids[live].astype(np.int64)] = nb[live]
+        q = _adversarial_boxes(rng, 300)
+        q = np.nan_to_num(q, nan=0.5)
+        fresh = oracle.RTree(cur, alive=alive)
+        for pred in (oracle.INTERSECTS, oracle.WITHIN, oracle.CONTAINS):
+            got = tree.checksums(pred, q)
+            want = fresh.checksums(pred, q)
+            assert got[0] == want[0] and np.array_equal(got[1], want[1])
+            assert np.array_equal(got[2], want[2])
+            assert np.array_equal(tree.pairs(pred, q), oracle.bf_pairs(pred, cur, q, alive=alive))
+    assert tree.delete(np.arange(5000, dtype=np.uint64)) == int(alive.sum())
+    assert tree.count(oracle.INTERSECTS, np.float32([[0, 0, 1, 1]])) == 0
+
+
 # ----------------------------------------------------------------- fixtures --
 def test_wkt_fixtures_consistent(oracle):
     d = np.load(os.path.join(GOLDEN, "wkt_1000.npz"))
