@@ -480,7 +480,9 @@ typedef struct rts_serve_config {
   uint32_t head_share_pct;     /* while later requests wait, the largest share of a batch's
                                   remaining result budget one request may take, in percent
                                   (50); requests that do not fit are skipped for later ones */
-  uint32_t reserved;           /* 0 */
+  uint32_t sort_min_tiles;     /* batched: Morton-order the wave tiles of a batch that has at
+                                  least this many (4096; larger batches gain L2 locality from
+                                  the order, smaller ones would mostly pay the sort's launches) */
 } rts_serve_config;
 
 typedef struct rts_serve_stats {

@@ -176,6 +176,32 @@ __device__ __forceinline__ bool leaf_test(const TreeView&, uint32_t, const float
                                           const QueryF32& q) {
   return Pred<PRED, POINTS>::leaf(b, q.b);
 }
+// A float query whose predicate is chosen at run time: RTS_INTERSECTS,
+// RTS_WITHIN or RTS_CONTAINS (data box contains the query). A point is the
+// degenerate envelope (x, y, x, y) under RTS_CONTAINS: contains_box(b, (x, y,
+// x, y)) is contains_point(b, x, y) comparison for comparison, and no subtree
+// lies inside a point. The persistent batch kernel answers every request kind
+// with one instantiation of each walk this way; `pred` is the same for every
+// lane of a wave (a request's), so the tests branch on scalar registers.
+struct QueryRT {
+  float4 b;
+  int pred;
+};
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool node_test(const float4 n, const QueryRT& q) {
+  return q.pred == RTS_CONTAINS ? contains_box(n, q.b) : intersects(n, q.b);
+}
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool inside_test(const float4 n, const QueryRT& q) {
+  return q.pred != RTS_CONTAINS && contains_box(q.b, n);
+}
+template <int PRED, bool POINTS>
+__device__ __forceinline__ bool leaf_test(const TreeView&, uint32_t, const float4 b,
+                                          const QueryRT& q) {
+  if (q.pred == RTS_INTERSECTS) return intersects(b, q.b);
+  if (q.pred == RTS_WITHIN) return contains_box(q.b, b);
+  return contains_box(b, q.b);
+}
 template <int PRED, bool POINTS>
 __device__ __forceinline__ bool leaf_test(const TreeView& t, uint32_t leaf, const float4 b,
                                           const QueryF64& q) {

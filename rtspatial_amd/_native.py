@@ -162,7 +162,7 @@ class rts_serve_config(ctypes.Structure):
         ("hash_results", ctypes.c_uint32),
         ("traversal", ctypes.c_uint32),
         ("head_share_pct", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("sort_min_tiles", ctypes.c_uint32),
     ]
 
 

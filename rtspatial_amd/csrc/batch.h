@@ -22,9 +22,11 @@ struct BatchRequest {
 };
 static_assert(sizeof(BatchRequest) == 32, "32-byte request descriptor");
 
-// Tile classes: one kernel each, so that each holds only its traversal
-// (a kernel with every traversal inlined needs ~100 VGPRs and runs 4 waves
-// per SIMD). A batch lists its requests class by class.
+// Tile classes. A batch lists its requests class by class, so each class
+// owns a contiguous range of tiles: the heavy wave classes first, the light
+// ones (points, lane tiles) last, where they fill the tail of the pass. One
+// persistent kernel per pass takes tiles of every class (the predicate is a
+// run-time, wave-uniform value of the walk), from per-class per-XCD counters.
 constexpr int kClassWaveIntersects = 0;
 constexpr int kClassWaveWithin = 1;
 constexpr int kClassWaveContains = 2;
@@ -42,6 +44,11 @@ __host__ __device__ inline int tile_class(uint32_t flags) {
   }
 }
 
+// Tile counters of a batch: for each pass (count, write), class and XCD, one
+// counter on its own 64-byte line, zeroed by the batch's first launch.
+constexpr uint32_t kCounterStride = 16;
+constexpr uint32_t kTileCounterWords = 2 * kClasses * 8 * kCounterStride;
+
 struct BatchHeader {
   uint32_t n_requests;
   uint32_t n_tiles;
@@ -53,7 +60,7 @@ struct BatchHeader {
 struct BatchArgs {
   const BatchHeader* header;     // device
   const BatchRequest* requests;  // device, sorted by tbase
-  uint32_t* tile_counters;       // 16 words: per-XCD counters of the count and write passes
+  uint32_t* tile_counters;       // kTileCounterWords: per-pass, per-class, per-XCD counters
   uint32_t* counts;              // per global query
   uint64_t* offsets;             // per global query: exclusive scan of counts
   uint64_t* seg_sums;            // scan scratch (batch_seg_sums_words())

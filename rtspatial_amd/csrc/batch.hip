@@ -6,16 +6,16 @@
 // queries [qbase_r, qbase_r + n_r) and tiles [tbase_r, tbase_r + tiles_r).
 // A tile is either 64 queries of one request answered one per lane (lane
 // tiles: points and data-contains-query, few results each) or one query
-// answered by a whole wavefront (wave tiles: intersects / within). Waves take
-// tiles from per-XCD counters -- each XCD first drains its own contiguous
-// eighth of the tiles, which keeps neighbouring queries on one L2 -- and then
-// steal from the other XCDs.
+// answered by a whole wavefront (wave tiles: intersects / within). Requests
+// are listed class by class (batch.h), and one persistent kernel per pass
+// takes the tiles of every class: per class, each XCD first drains its own
+// contiguous eighth of the class's tiles, which keeps neighbouring queries on
+// one L2, and then steals from the other XCDs.
 //
 // Two passes: the count pass writes each query's result count, a scan turns
 // the counts into exclusive offsets into the slot's result arena, and the
-// write pass writes every pair at its offset. Every launch has a fixed grid
-// and reads the batch's sizes from device memory, so one captured hipGraph
-// replays any batch.
+// write pass writes every pair at its offset. Every launch reads the batch's
+// sizes from device memory.
 #include "batch.h"
 #include "checksum.h"
 #include "morton.h"
@@ -38,89 +38,80 @@ __device__ __forceinline__ uint32_t find_request(const BatchRequest* reqs, uint3
   return lo;
 }
 
-// Next tile of [lo, hi) for this wave, or >= hi when all are taken.
-__device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, uint32_t lo, uint32_t hi,
-                                              uint32_t xcd) {
-  const uint32_t n = hi - lo;
-  const uint32_t per = (n + kXcds - 1) / kXcds;
-  uint32_t got = 0xFFFFFFFFu;
+// Tile slices: class c's tiles [lo_c, hi_c) are cut into kXcds contiguous
+// slices, one counter each. Wave w on XCD x first drains its own XCD's slices
+// class by class (heaviest first: neighbouring Morton-ordered queries then
+// share that XCD's L2), then steals, heaviest class first, from the other
+// XCDs. Counters only grow, so a slice found drained stays drained: `pos`
+// (wave-uniform) walks the 5 x 8 slice positions of this wave's order once.
+constexpr uint32_t kSlicePositions = kClasses * kXcds;
+
+__device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, const uint32_t* class_end,
+                                              uint32_t xcd, uint32_t& pos) {
+  uint32_t got = 0xFFFFFFFFu, at = kSlicePositions;
   if (lane_id() == 0) {
-    for (uint32_t k = 0; k < kXcds; k++) {
-      const uint32_t x = (xcd + k) % kXcds;
+    for (uint32_t p = pos; p < kSlicePositions; p++) {
+      uint32_t c, x;
+      if (p < kClasses) {
+        c = p;
+        x = xcd;
+      } else {
+        c = (p - kClasses) / (kXcds - 1);
+        x = (xcd + 1 + (p - kClasses) % (kXcds - 1)) % kXcds;
+      }
+      const uint32_t lo = c == 0 ? 0u : class_end[c - 1];
+      const uint32_t n = class_end[c] - lo;
+      const uint32_t per = (n + kXcds - 1) / kXcds;
       const uint32_t begin = x * per;
       const uint32_t end = begin + per < n ? begin + per : n;
       if (begin >= end) continue;
-      const uint32_t t = atomicAdd(&counters[x], 1u);
-      if (t < end - begin) {
-        got = lo + begin + t;
+      const uint32_t v = atomicAdd(&counters[(c * kXcds + x) * kCounterStride], 1u);
+      if (v < end - begin) {
+        got = lo + begin + v;
+        at = p;
         break;
       }
     }
   }
+  pos = readlane_u32(at, 0);
   return readlane_u32(got, 0);
 }
 
-template <int PRED, bool POINTS, bool WRITE>
-__device__ void run_lane_tile(const TreeView& t, const BatchRequest& r, uint32_t q0,
-                              const BatchArgs& a) {
-  const uint32_t q = q0 + lane_id();
-  if (q >= r.n) return;
-  const QueryF32 qb{load_query(r.queries, POINTS, q)};
-  const uint64_t g = r.qbase + q;
-  if (WRITE) {
-    lane_query<PRED, POINTS, true>(t, qb, q + r.qid_base, QueueEmit{a.arena, a.arena_cap, a.offsets[g]});
-  } else {
-    a.counts[g] = (uint32_t)lane_query<PRED, POINTS, false>(t, qb, q, NoEmit{});
-  }
-}
-
-template <int PRED, bool POINTS, bool WRITE>
-__device__ void run_wave_tile(const TreeView& t, const BatchRequest& r, uint32_t q,
-                              const BatchArgs& a, WaveLds& L) {
-  const QueryF32 qb{load_query(r.queries, POINTS, q)};
-  const uint64_t g = r.qbase + q;
-  if (WRITE) {
-    wave_query<PRED, POINTS, true>(t, qb, q + r.qid_base, QueueEmit{a.arena, a.arena_cap, a.offsets[g]},
-                                   L);
-  } else {
-    const uint64_t c = wave_query<PRED, POINTS, false>(t, qb, q, NoEmit{}, L);
-    if (lane_id() == 0) a.counts[g] = (uint32_t)c;
-  }
-}
-
-// The tree view is read from device memory so that a captured graph stays
-// valid when the index is rebuilt between batches. The kernel of class CLS
-// takes that class's tiles (see tile_class). One wavefront per workgroup:
-// the waves take tiles independently, and a workgroup's slot is freed as
-// soon as its one wave runs out of tiles (four-wave workgroups held theirs
-// until the slowest wave ended: 27.6 instead of 28.1 Mq/s served capacity).
-template <bool WRITE, int CLS>
+// One kernel per pass answers every tile of the batch. A request's predicate
+// is a run-time value of the walks (QueryRT), so the kernel holds one wave
+// walk and one lane walk. The tree view is read from device memory so that
+// the launch does not depend on the index's state. One wavefront per
+// workgroup: a workgroup's slot is freed as soon as its one wave runs out of
+// tiles (four-wave workgroups held theirs until the slowest wave ended).
+template <bool WRITE>
 __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const BatchArgs a) {
-  constexpr bool kWaveCls = CLS != kClassLane;
-  __shared__ WaveLds s_lds;  // wave classes only
+  __shared__ WaveLds s_lds;
   const TreeView t0 = *tv;
   TreeView t = t0;  // leaf_size may differ per request
   const BatchHeader h = *a.header;
-  const uint32_t lo = CLS == 0 ? 0u : h.class_end[CLS > 0 ? CLS - 1 : 0];
-  const uint32_t hi = h.class_end[CLS];
-  if (lo >= hi || h.n_requests == 0) return;
+  if (h.n_tiles == 0 || h.n_requests == 0) return;
   // An empty index (no boxes inserted, or cleared) has no tree: the count
   // pass still writes every query's count (zero), the write pass has nothing
   // to write.
   if (WRITE && t.n == 0) return;
-  uint32_t* counters = a.tile_counters + ((WRITE ? kClasses : 0) + CLS) * kXcds;
+  uint32_t* counters = a.tile_counters + (WRITE ? kClasses * kXcds * kCounterStride : 0u);
   const uint32_t xcd = blockIdx.x % kXcds;
-  // Every iteration consumes a tile (or finds none left), so hi - lo + 1
+  const uint32_t wave_end = h.class_end[kClassWavePoints];
+  uint32_t pos = 0;
+  // Every iteration consumes a tile (or finds none left), so n_tiles + 1
   // iterations bound the loop.
-  for (uint32_t it = 0; it <= hi - lo; it++) {
-    uint32_t tile = grab_tile(counters, lo, hi, xcd);
-    if (tile >= hi) break;
-    if (kWaveCls && a.order) tile = a.order[tile];  // same class: the keys lead with it
+  for (uint32_t it = 0; it <= h.n_tiles; it++) {
+    uint32_t tile = grab_tile(counters, h.class_end, xcd, pos);
+    if (tile >= h.n_tiles) break;
+    // Wave tiles are Morton-ordered within their class (the sort keys lead
+    // with the class).
+    if (a.order && tile < wave_end) tile = a.order[tile];
     const uint32_t ri = find_request(a.requests, h.n_requests, tile);
-    BatchRequest r = a.requests[ri];
+    const BatchRequest r = a.requests[ri];
     const uint32_t local = tile - r.tbase;
+    const bool wave = r.flags & kReqWaveTiles;
     if (t.n == 0) {
-      if (kWaveCls) {
+      if (wave) {
         if (local < r.n && lane_id() == 0) a.counts[r.qbase + local] = 0;
       } else {
         const uint32_t q = local * kWave + lane_id();
@@ -130,25 +121,29 @@ __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const Batch
     }
     const uint32_t lb = (r.flags >> kReqLeafShift) & 7u;
     t.leaf_size = lb ? 1u << (lb - 1) : t0.leaf_size;
-    if (CLS == kClassWaveIntersects) {
-      if (local < r.n) run_wave_tile<RTS_INTERSECTS, false, WRITE>(t, r, local, a, s_lds);
-    } else if (CLS == kClassWaveWithin) {
-      if (local < r.n) run_wave_tile<RTS_WITHIN, false, WRITE>(t, r, local, a, s_lds);
-    } else if (CLS == kClassWaveContains) {
-      if (local < r.n) run_wave_tile<RTS_CONTAINS, false, WRITE>(t, r, local, a, s_lds);
-    } else if (CLS == kClassWavePoints) {
-      if (local < r.n) run_wave_tile<RTS_CONTAINS, true, WRITE>(t, r, local, a, s_lds);
-    } else {
-      const uint32_t q0 = local * kWave;
-      const int pred = (r.flags >> kReqPredShift) & 3;
-      if (r.flags & kReqPoints) {
-        run_lane_tile<RTS_CONTAINS, true, WRITE>(t, r, q0, a);
-      } else if (pred == RTS_INTERSECTS) {
-        run_lane_tile<RTS_INTERSECTS, false, WRITE>(t, r, q0, a);
-      } else if (pred == RTS_WITHIN) {
-        run_lane_tile<RTS_WITHIN, false, WRITE>(t, r, q0, a);
+    const bool points = r.flags & kReqPoints;
+    const int pred = points ? (int)RTS_CONTAINS : (int)((r.flags >> kReqPredShift) & 3u);
+    if (wave) {
+      if (local >= r.n) continue;
+      const QueryRT qb{load_query(r.queries, points, local), pred};
+      const uint64_t g = r.qbase + local;
+      if (WRITE) {
+        wave_query<0, false, true>(t, qb, local + r.qid_base,
+                                   QueueEmit{a.arena, a.arena_cap, a.offsets[g]}, s_lds);
       } else {
-        run_lane_tile<RTS_CONTAINS, false, WRITE>(t, r, q0, a);
+        const uint64_t c = wave_query<0, false, false>(t, qb, local, NoEmit{}, s_lds);
+        if (lane_id() == 0) a.counts[g] = (uint32_t)c;
+      }
+    } else {
+      const uint32_t q = local * kWave + lane_id();
+      if (q >= r.n) continue;
+      const QueryRT qb{load_query(r.queries, points, q), pred};
+      const uint64_t g = r.qbase + q;
+      if (WRITE) {
+        lane_query<0, false, true>(t, qb, q + r.qid_base,
+                                   QueueEmit{a.arena, a.arena_cap, a.offsets[g]});
+      } else {
+        a.counts[g] = (uint32_t)lane_query<0, false, false>(t, qb, q, NoEmit{});
       }
     }
   }
@@ -252,10 +247,10 @@ uint32_t persistent_blocks(K kernel, int threads) {
   return (blocks + kXcds - 1) / kXcds * kXcds;
 }
 
-template <bool WRITE, int CLS>
+template <bool WRITE>
 void launch_batch_kernel(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
-  static const uint32_t blocks = persistent_blocks(k_batch<WRITE, CLS>, kWave);
-  hipLaunchKernelGGL((k_batch<WRITE, CLS>), dim3(blocks), dim3(kWave), 0, stream, t, a);
+  static const uint32_t blocks = persistent_blocks(k_batch<WRITE>, kWave);
+  hipLaunchKernelGGL((k_batch<WRITE>), dim3(blocks), dim3(kWave), 0, stream, t, a);
 }
 
 // Sort keys of the wave tiles [0, n): class in the top 2 bits (the wave
@@ -273,16 +268,6 @@ __global__ void __launch_bounds__(kThreads) k_batch_keys(const BatchArgs a, cons
     const uint32_t cls = (uint32_t)tile_class(r.flags) & 3u;
     keys[t] = (cls << kMortonBits) | morton_of(0.5f * (q.x + q.z), 0.5f * (q.y + q.w), sc);
   }
-}
-
-// Every class kernel in turn; a class without tiles exits at once.
-template <bool WRITE>
-void launch_batch_pass(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
-  launch_batch_kernel<WRITE, kClassWaveIntersects>(t, a, stream);
-  launch_batch_kernel<WRITE, kClassWaveWithin>(t, a, stream);
-  launch_batch_kernel<WRITE, kClassWaveContains>(t, a, stream);
-  launch_batch_kernel<WRITE, kClassWavePoints>(t, a, stream);
-  launch_batch_kernel<WRITE, kClassLane>(t, a, stream);
 }
 
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSort& sort,
@@ -306,16 +291,16 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
   (void)hipStreamIsCapturing(stream, &cap);
   const bool timed = cap == hipStreamCaptureStatusNone;
   PhaseToken ph = timed ? phase_begin(RTS_PHASE_BATCH_COUNT, stream) : PhaseToken{-1, nullptr};
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, a.tile_counters,
-                     2 * kClasses * kXcds);
-  launch_batch_pass<false>(t, a, stream);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, a.tile_counters,
+                     kTileCounterWords);
+  launch_batch_kernel<false>(t, a, stream);
   hipLaunchKernelGGL(k_seg_sums, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_apply, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_request_ranges, dim3(16), dim3(kThreads), 0, stream, a);
   phase_end(ph, stream);
   ph = timed ? phase_begin(RTS_PHASE_BATCH_WRITE, stream) : PhaseToken{-1, nullptr};
-  launch_batch_pass<true>(t, a, stream);
+  launch_batch_kernel<true>(t, a, stream);
   phase_end(ph, stream);
   return hipGetLastError();
 }

@@ -364,7 +364,7 @@ int record_slot_work(Engine& E, Slot& s, size_t up_bytes, size_t n_req) {
   sort_scratch_carve(s.d_sort + 4 * maxq, maxq, &sort.ss);
   sort.ss.error = E.ix->error;  // a look-back that gives up is reported after the run
   sort.scene = E.ix->scene;
-  sort.n = wave_tiles >= 65536 ? wave_tiles : 0;
+  sort.n = wave_tiles >= E.cfg.sort_min_tiles ? wave_tiles : 0;
   SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, sort, s.stream));
   SRV_HIP(hipMemcpyAsync(s.h_ranges, s.d_ranges, 2 * n_req * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, s.stream));
@@ -388,7 +388,7 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_ranges), 2 * maxr * sizeof(uint64_t)));
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_hash), maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_desc, E.desc_bytes));
-  SRV_HIP(hipMalloc(&s.d_counters, 2 * kClasses * 8 * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_counters, kTileCounterWords * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_counts, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_offsets, maxq * sizeof(uint64_t)));
   SRV_HIP(hipMalloc(&s.d_seg, batch_seg_sums_words() * sizeof(uint64_t)));
@@ -751,6 +751,7 @@ void rts_serve_config_default(rts_serve_config* c) {
   c->hash_results = 0;
   c->traversal = RTS_TRAVERSAL_AUTO;
   c->head_share_pct = 50;
+  c->sort_min_tiles = 4096;
 }
 
 int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request* reqs,

@@ -436,17 +436,23 @@ def test_autotune_keeps_results_exact(oracle, kind):
     r = ix.Autotune(pred, dq, sample=4096, reps=2)
     # A strided sample: 8 plans (4 leaf sizes x 2 traversals, the query order
     # left to the call's size), 5 when a traversal mode is dropped after its
-    # first plan.
-    assert r["sample"] == 4096 and 5 <= r["plans"] <= 8 and r["sort_queries"] == 0
+    # first plan, 4 when the sample's queries average more than 256 results
+    # (one query per lane is then not timed at all).
+    lane_hopeless = r["sample_results"] > 256 * r["sample"]
+    lo_plans = 4 if lane_hopeless else 5
+    assert r["sample"] == 4096 and lo_plans <= r["plans"] <= 8 and r["sort_queries"] == 0
+    if lane_hopeless:
+        assert r["plans"] == 4 and r["traversal"] == WAVE
     assert r["leaf_size"] in (4, 8, 16, 32) and r["traversal"] in (THREAD, WAVE)
     assert 0 < r["best_ms"] <= r["default_ms"]
     sample_want = oracle.RTree(b).count(int(pred), q[:: q.shape[0] // 4096][:4096])
     assert r["sample_results"] == sample_want
     if kind == "intersects":
         # The whole set as the sample: the order is tuned too (16 plans, 9 or
-        # more), and the stored plan includes it.
+        # more; 8 without the per-lane plans), and the stored plan includes it.
         r2 = ix.Autotune(pred, dq, sample=q.shape[0], reps=1)
-        assert r2["sample"] == q.shape[0] and 9 <= r2["plans"] <= 16
+        hopeless2 = r2["sample_results"] > 256 * r2["sample"]
+        assert r2["sample"] == q.shape[0] and (8 if hopeless2 else 9) <= r2["plans"] <= 16
         assert r2["sort_queries"] in (1, 2) and r2["sample_results"] == want.shape[0]
     got, n = collect(ix, pred, dq)
     assert n == want.shape[0] and np.array_equal(got, want)
