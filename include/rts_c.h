@@ -311,6 +311,28 @@ int rts_index_query_plan(rts_index* index, rts_predicate pred, int points, uint6
 int rts_index_order_queries(rts_index* index, const void* queries, uint64_t n, int points,
                             uint32_t* order, rts_stream_t stream);
 
+/* Allocation-free query. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The same query as
+ * rts_index_query_points / _envelopes / _lines (`kind`: an rts_query_kind;
+ * queries are floats on a float index, doubles on an RTS_COORD_F64 index),
+ * with the same results, but its scratch (query order, per-query counts and
+ * offsets, scan and ticket words) lives in a fixed 32 MiB array that the
+ * library holds on each GPU it runs on until the process ends, and it calls
+ * no hipMalloc, hipFree or other allocating routine. Limits, each refused
+ * with RTS_ERR_INVALID before anything is queued:
+ *   - n <= 1048576 queries per call;
+ *   - the index is up to date (after rts_index_build, or a query, following
+ *     the last Insert / Delete / Update), since a build can allocate;
+ *   - not on an RTS_COORD_F64_AS_F32 index (its queries are rounded into an
+ *     allocated buffer).
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (on different streams, or from different threads with
+ * different indexes) race on it and must be serialised by the caller; calls
+ * queued on one stream are ordered and safe. */
+int rts_index_query_static(rts_index* index, rts_predicate pred, int kind, const void* queries,
+                           uint64_t n, rts_handler handler, void* arg, rts_stream_t stream);
+
 /* Number of ids handed out so far (live and deleted). */
 uint64_t rts_index_size(const rts_index* index);
 

@@ -207,6 +207,7 @@ SIGNATURES = [
     ("rts_index_query_envelopes_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_query_lines", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_query_lines_d", _I32, [_VP, _I32, _VP, _U64, _I32, _VP, _VP]),
+    ("rts_index_query_static", _I32, [_VP, _I32, _I32, _VP, _U64, _I32, _VP, _VP]),
     ("rts_index_size", _U64, [_VP]),
     ("rts_index_tree_view", _I32, [_VP, _VP, ctypes.POINTER(rts_tree_view)]),
     ("rts_index_query_plan", _I32, [_VP, _I32, _I32, _U64, ctypes.POINTER(rts_query_plan)]),

@@ -919,6 +919,81 @@ int rts_index_query_lines_d(rts_index* ix, rts_predicate pred, const rts_line_d*
   return run_lines(ix, pred, q, n, true, handler, arg, stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of rts_index_query_static: one array per GPU in the library's device
+// image, held until the process ends. kStaticMaxQueries queries under any plan
+// need at most ~29.2 MiB of it (Morton keys and values 16 MiB, radix-sort
+// status 1 MiB, counts 4 MiB, offsets 8 MiB, scan tiles and tickets); a call
+// that would need more is refused before anything is launched. In host code
+// `scratch` names a host-side stand-in, so kernels are only ever given the
+// device address hipGetSymbolAddress returns for the current device.
+constexpr uint64_t kStaticMaxQueries = 1ull << 20;
+constexpr size_t kStaticScratchBytes = 32ull << 20;
+__device__ __align__(256) char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up once.
+int static_scratch(int device, char** out) {
+  RTS_CHECK_ARG(device >= 0 && device < kMaxDevices, "rts_index_query_static: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    RTS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    RTS_CHECK_ARG(p, "rts_index_query_static: no device address for the scratch array");
+    g_scratch_ptr[device] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return RTS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rts_index_query_static(rts_index* ix, rts_predicate pred, int kind, const void* q, uint64_t n,
+                           rts_handler handler, void* arg, rts_stream_t stream) {
+  RTS_CHECK_ARG(ix, "rts_index_query_static: null index");
+  RTS_CHECK_ARG(kind >= RTS_QUERY_ENVELOPES && kind <= RTS_QUERY_LINES,
+                "rts_index_query_static: unknown query kind");
+  RTS_CHECK_ARG(n <= kStaticMaxQueries,
+                "rts_index_query_static: at most 1048576 queries per call (the static scratch)");
+  RTS_CHECK_ARG(ix->cfg.coord != RTS_COORD_F64_AS_F32,
+                "rts_index_query_static: an index with coord RTS_COORD_F64_AS_F32 rounds its "
+                "queries into an allocated buffer; use the allocating query calls");
+  const bool points = kind == RTS_QUERY_POINTS;
+  int pred_k = pred;
+  if (kind == RTS_QUERY_LINES) {
+    RTS_CHECK_ARG(pred == RTS_INTERSECTS || pred == RTS_CONTAINS,
+                  "rts_index_query_static: segment queries take RTS_INTERSECTS or RTS_CONTAINS");
+    int rc = check_query_args(RTS_CONTAINS, q, n, false, handler, arg);
+    if (rc) return rc;
+    if (pred == RTS_CONTAINS) return RTS_OK;  // no results, as rts_index_query_lines
+    pred_k = kPredLine;
+  }
+  int rc = check_query_args(pred_k, q, n, points, handler, arg);
+  if (rc) return rc;
+  if (n == 0) return RTS_OK;
+  std::lock_guard<std::mutex> lk(ix->mu);
+  RTS_HIP(hipSetDevice(ix->device));
+  if (ix->n == 0) return RTS_OK;
+  // Building or refitting can allocate (the tree grows with the boxes), so
+  // the index must already be up to date.
+  RTS_CHECK_ARG(!ix->stale && !ix->refit_needed && ix->n_tree == ix->n,
+                "rts_index_query_static: the index has mutations pending; call rts_index_build "
+                "first");
+  char* ext = nullptr;
+  rc = static_scratch(ix->device, &ext);
+  if (rc) return rc;
+  return run_plan(ix, plan_query(ix, pred_k, points, n), pred_k, q, n, points, ix->f64(), handler,
+                  arg, stream, ext, kStaticScratchBytes);
+}
+
 uint64_t rts_index_size(const rts_index* ix) { return ix ? ix->n : 0; }
 
 int rts_index_query_plan(rts_index* ix, rts_predicate pred, int points, uint64_t n,
