@@ -129,7 +129,8 @@ __device__ __forceinline__ int prefix_len(const uint32_t* __restrict__ keys, int
 __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict__ keys, uint32_t n,
                                                      NodeRec* __restrict__ nodes,
                                                      uint32_t* __restrict__ parent,
-                                                     uint32_t* __restrict__ flags) {
+                                                     uint32_t* __restrict__ flags,
+                                                     uint32_t chunk_log2) {
   const int64_t N = n;
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < N - 1;
        i += (int64_t)gridDim.x * kThreads) {
@@ -155,46 +156,78 @@ __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict_
     nodes[i].meta = make_uint4(left, right, (uint32_t)first, (uint32_t)last);
     const uint32_t lslot = (left & kLeafFlag) ? (uint32_t)(N - 1) + (left & ~kLeafFlag) : left;
     const uint32_t rslot = (right & kLeafFlag) ? (uint32_t)(N - 1) + (right & ~kLeafFlag) : right;
-    parent[lslot] = ((uint32_t)i << 1) | 0u;
-    parent[rslot] = ((uint32_t)i << 1) | 1u;
+    // kSpanBit: node i's leaf range crosses a refit chunk boundary.
+    const uint32_t span =
+        ((uint64_t)first >> chunk_log2) != ((uint64_t)last >> chunk_log2) ? kSpanBit : 0u;
+    parent[lslot] = ((uint32_t)i << 1) | 0u | span;
+    parent[rslot] = ((uint32_t)i << 1) | 1u | span;
     flags[i] = 0;
   }
 }
 
 // ------------------------------------------------------------- refit ----
+// Bottom-up refit in three levels, so that no level needs ordering across
+// workgroups (agent-scope acquire/release across the XCDs' non-coherent L2s
+// made a one-kernel refit 2.6x slower):
+//
+//   1. k_refit_chunk: workgroup c takes leaves [c*C, (c+1)*C) (C = refit
+//      chunk). Every node whose leaf range lies in that chunk has its index
+//      in the chunk too (a Karras node is one end of its range), so its
+//      arrival counter, live count and the union of its children's bounds
+//      live in LDS: each arrival folds its box in with LDS float min/max and
+//      then adds to the counter, and the second arrival reads the union back.
+//      A wave's LDS operations complete in order, so the union is whole once
+//      the counter shows both arrivals; no wave waits on another. Nodes that
+//      span chunks are marked in the parent codes by the hierarchy kernel
+//      (kSpanBit); their child slot is written and the code listed in the
+//      chunk's boundary slots.
+//   2. k_refit_span<false>: workgroup g takes the boundary codes of chunks
+//      [g*G, (g+1)*G) and finishes, with workgroup-scope atomics on global
+//      flags, the nodes whose range lies in those chunks; codes of nodes that
+//      reach further go to the top list.
+//   3. k_refit_span<true>: one workgroup finishes the top list.
+//
 // Every walk ends at the root within the tree's depth (at most 65 levels for
 // 30-bit keys with position tie-breaks); the level bound only guards against
 // a corrupt parent array, and reports it through `error`.
 constexpr int kMaxLevels = 128;
-// Leaves per workgroup (chunk). The single workgroup of k_refit_top walks
-// every node that spans chunks, so larger chunks shorten it, while each
-// thread's walks run one after another and fewer workgroups share the GPU.
-// Measured on 10M boxes (tree phase of a build, refit included): 2048 leaves
-// 3.20 ms, 4096 2.70, 8192 2.36, 12288 2.48, 16384 2.64, 32768 3.60; a tree
-// of 10k-100k boxes (a delta tree) wants 2048-leaf chunks (0.25 ms per
-// delta build instead of 0.55 ms with 8192).
-__host__ __device__ constexpr uint32_t refit_chunk(uint32_t n) {
-  return n >= (1u << 22) ? 8192u : 2048u;
+constexpr int kRefitThreads = 1024;
+constexpr uint32_t kRefitGroup = 16;  // chunks per level-2 workgroup
+constexpr int kRefitSpanThreads = 256;
+constexpr int kRefitTopThreads = 1024;
+
+__device__ __forceinline__ void put_child(NodeRec* np, uint32_t side, float4 b, uint32_t live) {
+  np->box[side] = b;
+  if (side) np->live.y = live; else np->live.x = live;
 }
 
-// Workgroup b walks up from leaves [b*C, (b+1)*C), C = refit_chunk(n). A node whose leaf
-// range lies inside that chunk has both children finished by this workgroup,
-// so its arrival counter only needs workgroup-scope ordering (no L2
-// write-back or invalidation); only nodes that straddle chunks go to
-// k_refit_top. Each child slot also receives the
-// number of live (not deleted) leaves below it.
-__global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ leaf_boxes, uint32_t n,
-                                                    const uint32_t* __restrict__ parent,
-                                                    NodeRec* nodes, float4* __restrict__ root_box,
-                                                    uint32_t* __restrict__ root_live,
-                                                    uint32_t* flags, uint32_t* error,
-                                                    uint32_t* boundary) {
-  const uint32_t chunk = refit_chunk(n);
-  const uint32_t c0 = blockIdx.x * chunk;
-  const uint32_t c1 = (c0 + chunk < n ? c0 + chunk : n) - 1;  // inclusive
-  for (uint32_t r = 0; r < chunk / kThreads; r++) {
-    const uint32_t k = c0 + r * kThreads + threadIdx.x;
-    if (k >= n) break;
+__device__ __forceinline__ void push_boundary(uint32_t* slots, uint32_t code, uint32_t* error) {
+  const uint32_t at = atomicAdd(&slots[0], 1u);
+  if (at < kRefitChunkSlots) slots[1 + at] = code;
+  else atomicOr(error, 2u);
+}
+
+// LDS of a level-1 workgroup, per node of the chunk (index - c0): the union
+// of the bounds of the children that arrived, (live leaves << 1) | arrivals,
+// and the node's own parent code.
+__global__ void __launch_bounds__(kRefitThreads) k_refit_chunk(
+    const float4* __restrict__ leaf_boxes, uint32_t n, uint32_t chunk_log2,
+    const uint32_t* __restrict__ parent, NodeRec* nodes, float4* __restrict__ root_box,
+    uint32_t* __restrict__ root_live, uint32_t* error, uint32_t* boundary) {
+  extern __shared__ float4 s_box[];
+  const uint32_t C = 1u << chunk_log2;
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_box + C);
+  uint32_t* s_par = s_cnt + C;
+  const uint32_t c0 = blockIdx.x << chunk_log2;
+  const uint32_t c1 = (c0 + C < n ? c0 + C : n) - 1;  // inclusive
+  uint32_t* slots = boundary + (uint64_t)blockIdx.x * kRefitChunkStride;
+  for (uint32_t j = threadIdx.x; j < C; j += kRefitThreads) {
+    s_box[j] = empty_box();
+    s_cnt[j] = 0;
+    s_par[j] = c0 + j + 1 < n ? parent[c0 + j] : 0u;  // internal nodes are 0 .. n - 2
+  }
+  __syncthreads();
+  for (uint32_t k = c0 + threadIdx.x; k <= c1; k += kRefitThreads) {
     float4 b = leaf_boxes[k];
     uint32_t live = 1;
     if (box_dead(b)) {
@@ -212,62 +245,95 @@ __global__ void __launch_bounds__(kThreads) k_refit(const float4* __restrict__ l
     }
     uint32_t code = parent[(n - 1) + k];
     for (int level = 0;; level++) {
-      const uint32_t p = code >> 1, side = code & 1u;
+      const uint32_t p = (code & ~kSpanBit) >> 1, side = code & 1u;
       if (level >= kMaxLevels || p >= n - 1) {
         atomicOr(error, 2u);
         break;
       }
-      NodeRec* np = nodes + p;
-      np->box[side] = b;
-      if (side) np->live.y = live; else np->live.x = live;
-      const uint4 mp = np->meta;
-      if (!(mp.z >= c0 && mp.w <= c1)) {
-        // The node spans chunks: k_refit_top finishes it (and everything
-        // above) after this kernel, so no agent-scope ordering is needed.
-        const uint32_t slot = atomicAdd(&boundary[0], 1u);
-        if (1 + (uint64_t)slot < refit_boundary_words(n)) boundary[1 + slot] = code;
-        else atomicOr(error, 2u);
+      if (code & kSpanBit) {
+        // The node spans chunks: level 2 or 3 finishes it.
+        put_child(nodes + p, side, b, live);
+        push_boundary(slots, code, error);
         break;
       }
-      // Release our child's bounds; acquire the sibling's if we arrive second.
-      const uint32_t old =
-          __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      const float4 o = np->box[side ^ 1u];
-      live += side ? np->live.x : np->live.y;
-      b = unite(b, o);
+      const uint32_t j = p - c0;
+      if (j >= C) {  // not in this chunk: a corrupt parent array
+        atomicOr(error, 2u);
+        break;
+      }
+      __hip_atomic_fetch_min(&s_box[j].x, b.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_min(&s_box[j].y, b.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_max(&s_box[j].z, b.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_max(&s_box[j].w, b.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the union before the count, in program order
+      const uint32_t old = __hip_atomic_fetch_add(&s_cnt[j], (live << 1) | 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      put_child(nodes + p, side, b, live);
+      if (!(old & 1u)) break;  // first arrival: the sibling carries on
+      live += old >> 1;
+      b = s_box[j];
       if (p == 0) {
         root_box[0] = b;
         root_live[0] = live;
         break;
       }
-      code = parent[p];
+      code = s_par[j];
     }
   }
 }
 
-// The nodes that span chunks, bottom-up, in one workgroup: each boundary
-// code names a child slot that k_refit filled; the second arrival at a node
-// unites both slots and fills the node's slot in its parent.
-constexpr int kRefitTopThreads = 1024;
-
-__global__ void __launch_bounds__(kRefitTopThreads) k_refit_top(const uint32_t* __restrict__ parent,
-                                                              NodeRec* nodes, uint32_t n,
-                                                              float4* __restrict__ root_box,
-                                                              uint32_t* __restrict__ root_live,
-                                                              uint32_t* flags, uint32_t* error,
-                                                              const uint32_t* boundary) {
-  const uint64_t cap = refit_boundary_words(n) - 1;
-  const uint32_t cnt = boundary[0] < cap ? boundary[0] : (uint32_t)cap;
-  for (uint32_t j = threadIdx.x; j < cnt; j += kRefitTopThreads) {
-    uint32_t code = boundary[1 + j];
+// Levels 2 and 3. Each code names a child slot already filled; the second
+// arrival at a node unites both slots and fills the node's slot in its
+// parent. TOP = false: workgroup g owns the nodes whose leaf range lies in
+// chunks [g*G, (g+1)*G) and lists codes of nodes reaching further in the top
+// list; TOP = true: one workgroup takes the top list to the root.
+template <bool TOP>
+__global__ void __launch_bounds__(TOP ? kRefitTopThreads : kRefitSpanThreads) k_refit_span(
+    const uint32_t* __restrict__ parent, NodeRec* nodes, uint32_t n, uint32_t chunk_log2,
+    uint32_t n_chunks, float4* __restrict__ root_box, uint32_t* __restrict__ root_live,
+    uint32_t* flags, uint32_t* error, uint32_t* boundary) {
+  constexpr uint32_t kThreadsHere = TOP ? kRefitTopThreads : kRefitSpanThreads;
+  uint32_t* top = boundary + (uint64_t)n_chunks * kRefitChunkStride;  // [0] count, then codes
+  const uint64_t top_cap = refit_top_words(n) - 1;
+  const uint32_t g0 = blockIdx.x * kRefitGroup;
+  const uint32_t lo = g0 << chunk_log2;
+  const uint64_t hi64 = ((uint64_t)(g0 + kRefitGroup) << chunk_log2) - 1;
+  const uint32_t hi = hi64 < n ? (uint32_t)hi64 : n - 1;
+  uint32_t total;
+  if (TOP) {
+    total = top[0] < top_cap ? top[0] : (uint32_t)top_cap;
+  } else {
+    const uint32_t groups = n_chunks - g0 < kRefitGroup ? n_chunks - g0 : kRefitGroup;
+    total = groups * kRefitChunkSlots;
+  }
+  for (uint32_t t = threadIdx.x; t < total; t += kThreadsHere) {
+    uint32_t code;
+    if (TOP) {
+      code = top[1 + t];
+    } else {
+      const uint32_t* slots = boundary + (uint64_t)(g0 + t / kRefitChunkSlots) * kRefitChunkStride;
+      const uint32_t i = t % kRefitChunkSlots;
+      const uint32_t cnt = slots[0] < kRefitChunkSlots ? slots[0] : kRefitChunkSlots;
+      if (i >= cnt) continue;
+      code = slots[1 + i];
+    }
     for (int level = 0;; level++) {
-      const uint32_t p = code >> 1;
+      const uint32_t p = (code & ~kSpanBit) >> 1;
       if (level >= kMaxLevels || p >= n - 1) {
         atomicOr(error, 2u);
         break;
       }
       NodeRec* np = nodes + p;
+      if (!TOP) {
+        const uint4 m = np->meta;
+        if (m.z < lo || m.w > hi) {  // reaches beyond this group's chunks
+          const uint32_t at = atomicAdd(&top[0], 1u);
+          if (at < top_cap) top[1 + at] = code;
+          else atomicOr(error, 2u);
+          break;
+        }
+      }
       const uint32_t old =
           __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old == 0) break;
@@ -279,9 +345,7 @@ __global__ void __launch_bounds__(kRefitTopThreads) k_refit_top(const uint32_t* 
         break;
       }
       code = parent[p];
-      NodeRec* pp = nodes + (code >> 1);
-      pp->box[code & 1u] = b;
-      if (code & 1u) pp->live.y = live; else pp->live.x = live;
+      put_child(nodes + ((code & ~kSpanBit) >> 1), code & 1u, b, live);
     }
   }
 }
@@ -428,24 +492,34 @@ hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes
                          uint32_t* flags, hipStream_t stream) {
   if (n < 2) return hipSuccess;
   hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, sorted_keys, n,
-                     nodes, parent, flags);
+                     nodes, parent, flags, refit_chunk_log2(n));
   return hipGetLastError();
 }
 
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream) {
   const uint32_t n = a.n;
   if (n == 0) return hipSuccess;
-  if (n > 1)
+  const uint32_t lg = refit_chunk_log2(n);
+  const uint32_t chunks = (uint32_t)(((uint64_t)n + (1u << lg) - 1) >> lg);
+  // Level-1 slots (count + codes per chunk) and the top list's count start at zero.
+  hipError_t e = launch_fill_u32(a.boundary, 0u, (uint64_t)chunks * kRefitChunkStride + 1, stream);
+  if (e != hipSuccess) return e;
+  if (chunks > 1) {
     hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, a.flags,
                        n - 1);
-  hipError_t e = launch_fill_u32(a.boundary, 0u, 1, stream);
-  if (e != hipSuccess) return e;
-  const uint32_t blocks = (n + refit_chunk(n) - 1) / refit_chunk(n);
-  hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(kThreads), 0, stream, a.leaf_boxes, n, a.parent,
-                     a.nodes, a.root_box, a.root_live, a.flags, a.error, a.boundary);
-  if (n > 1)
-    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(kRefitTopThreads), 0, stream, a.parent, a.nodes,
-                       n, a.root_box, a.root_live, a.flags, a.error, a.boundary);
+  }
+  const size_t lds = (size_t)(1u << lg) * (sizeof(float4) + 2 * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_refit_chunk, dim3(chunks), dim3(kRefitThreads), lds, stream, a.leaf_boxes,
+                     n, lg, a.parent, a.nodes, a.root_box, a.root_live, a.error, a.boundary);
+  if (chunks > 1) {
+    const uint32_t groups = (chunks + kRefitGroup - 1) / kRefitGroup;
+    hipLaunchKernelGGL(k_refit_span<false>, dim3(groups), dim3(kRefitSpanThreads), 0, stream,
+                       a.parent, a.nodes, n, lg, chunks, a.root_box, a.root_live, a.flags, a.error,
+                       a.boundary);
+    hipLaunchKernelGGL(k_refit_span<true>, dim3(1), dim3(kRefitTopThreads), 0, stream, a.parent,
+                       a.nodes, n, lg, chunks, a.root_box, a.root_live, a.flags, a.error,
+                       a.boundary);
+  }
   return hipGetLastError();
 }
 

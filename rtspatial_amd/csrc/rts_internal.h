@@ -73,15 +73,27 @@ struct RefitArgs {
   uint32_t* root_live;
   uint32_t* flags;
   uint32_t* error;
-  uint32_t* boundary;  // refit_boundary_words(n): [0] count, then (node << 1 | side) codes
+  uint32_t* boundary;  // refit_boundary_words(n): per-chunk boundary slots, then the top list
 };
-// Words of the boundary list of a refit over n leaves: a node spanning
-// refit chunks (at least 2048 leaves each) contains a chunk boundary, and a
-// boundary lies in at most 64 nodes (tree depth), so the spanning nodes --
-// and the list's entries, one more than their number -- are bounded by 64
-// per boundary.
-__host__ __device__ inline uint64_t refit_boundary_words(uint64_t n) {
+// Refit chunks: C = 2^refit_chunk_log2(n) leaves per level-1 workgroup, whose
+// LDS holds 24 bytes per node of the chunk (48 KB at 2048 leaves: three
+// workgroups per CU).
+__host__ __device__ constexpr uint32_t refit_chunk_log2(uint64_t) { return 11u; }
+// Parent codes (node << 1 | side) carry kSpanBit when the parent's leaf range
+// crosses a refit chunk boundary.
+constexpr uint32_t kSpanBit = 0x80000000u;
+// A chunk's boundary codes name the spanning nodes that receive a child from
+// the chunk; those nodes cross the chunk's first or last leaf boundary, so
+// they lie on two chains of ancestors of at most 64 nodes each (tree depth).
+constexpr uint32_t kRefitChunkSlots = 128;
+constexpr uint32_t kRefitChunkStride = kRefitChunkSlots + 1;  // [0] count, then codes
+// The top list: a node reaching beyond its level-2 group contains a group
+// boundary, and a boundary lies in at most 64 nodes.
+__host__ __device__ inline uint64_t refit_top_words(uint64_t n) {
   return 64 * (n / 2048 + 2) + 2;
+}
+__host__ __device__ inline uint64_t refit_boundary_words(uint64_t n) {
+  return (n / 2048 + 1) * kRefitChunkStride + refit_top_words(n);
 }
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
 hipError_t launch_tree_cost(const NodeRec* nodes, const float4* root_box, uint32_t n,
