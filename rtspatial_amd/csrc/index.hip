@@ -1180,13 +1180,13 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   rc = ensure_built(ix, stream);
   if (rc) return rc;
 
-  // An evenly strided sample, shrunk until its results fit kMaxPairs (about
-  // 1 GiB of pairs: one timed run then writes at most that much, which keeps
-  // a run near a millisecond or two on the 10M-box index). A double index's
+  // An evenly strided sample, shrunk until its results fit kMaxPairs (512 MiB
+  // of pairs: one timed run then writes at most that much, about 2 ms on the
+  // 10M-box index, which keeps a predicate's tuning under 50 ms). A double index's
   // sample is taken in doubles; one that rounds doubles to floats times the
   // rounded sample.
   const size_t qbytes = query_bytes(pts, ix->f64());
-  const uint64_t kMaxPairs = 1ull << 27;
+  const uint64_t kMaxPairs = 1ull << 26;
   TuneBuffers tb;
   RTS_HIP(hipMalloc(&tb.small, 64));
   RTS_HIP(hipEventCreate(&tb.ev[0]));
