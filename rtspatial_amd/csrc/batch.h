@@ -44,10 +44,21 @@ __host__ __device__ inline int tile_class(uint32_t flags) {
   }
 }
 
-// Tile counters of a batch: for each pass (count, write), class and XCD, one
-// counter on its own 64-byte line, zeroed by the batch's first launch.
+// Longest first: the count pass lists each wave tile whose query has at
+// least giant_min(b) results in bucket b (bucket 0 the largest), and the
+// write pass takes the listed tiles, bucket by bucket, before every other
+// tile, so that a query of millions of results does not start late and hold
+// one wave past the end of the pass (one wave writes a few hundred million
+// pairs per second).
+constexpr int kGiantBuckets = 3;
+__host__ __device__ constexpr uint32_t giant_min(int b) { return 1u << (20 - 2 * b); }
+
+// Tile counters of a batch, each on its own 64-byte line, zeroed by the
+// batch's first launch: for each pass (count, write), class and XCD one tile
+// counter, then per giant bucket an append and a take counter.
 constexpr uint32_t kCounterStride = 16;
-constexpr uint32_t kTileCounterWords = 2 * kClasses * 8 * kCounterStride;
+constexpr uint32_t kTileCounterWords = (2 * kClasses * 8 + 2 * kGiantBuckets) * kCounterStride;
+constexpr uint32_t kGiantCounterOff = 2 * kClasses * 8 * kCounterStride;
 
 struct BatchHeader {
   uint32_t n_requests;
@@ -69,6 +80,8 @@ struct BatchArgs {
   rts_pair* arena;               // results
   uint64_t arena_cap;            // pairs the arena holds
   const uint32_t* order;         // wave tiles in Morton order (tile -> tile), or null
+  uint32_t* giants;              // kGiantBuckets lists of giant_cap tiles each
+  uint32_t giant_cap;
 };
 
 // Morton ordering of a batch's wave tiles: keys (class << 30 | Morton code of

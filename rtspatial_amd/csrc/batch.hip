@@ -77,12 +77,75 @@ __device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, const uint32_t
   return readlane_u32(got, 0);
 }
 
+// One tile of the batch: a wave tile's one query (by the whole wave) or a
+// lane tile's 64 queries. With skip_giants a wave tile whose query is on a
+// giant list is left to that list.
+template <bool WRITE>
+__device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const BatchArgs& a,
+                                         const BatchHeader& h, uint32_t tile, bool skip_giants,
+                                         WaveLds& lds) {
+  const uint32_t ri = find_request(a.requests, h.n_requests, tile);
+  const BatchRequest r = a.requests[ri];
+  const uint32_t local = tile - r.tbase;
+  const bool wave = r.flags & kReqWaveTiles;
+  if (WRITE && skip_giants && wave && local < r.n &&
+      a.counts[r.qbase + local] >= giant_min(kGiantBuckets - 1))
+    return;
+  if (t.n == 0) {
+    if (wave) {
+      if (local < r.n && lane_id() == 0) a.counts[r.qbase + local] = 0;
+    } else {
+      const uint32_t q = local * kWave + lane_id();
+      if (q < r.n) a.counts[r.qbase + q] = 0;
+    }
+    return;
+  }
+  const uint32_t lb = (r.flags >> kReqLeafShift) & 7u;
+  t.leaf_size = lb ? 1u << (lb - 1) : t0.leaf_size;
+  const bool points = r.flags & kReqPoints;
+  const int pred = points ? (int)RTS_CONTAINS : (int)((r.flags >> kReqPredShift) & 3u);
+  if (wave) {
+    if (local >= r.n) return;
+    const QueryRT qb{load_query(r.queries, points, local), pred};
+    const uint64_t g = r.qbase + local;
+    if (WRITE) {
+      wave_query<0, false, true>(t, qb, local + r.qid_base,
+                                 QueueEmit{a.arena, a.arena_cap, a.offsets[g]}, lds);
+    } else {
+      const uint64_t c = wave_query<0, false, false>(t, qb, local, NoEmit{}, lds);
+      if (lane_id() == 0) {
+        a.counts[g] = (uint32_t)c;
+        // Giants are listed as they are counted (rare: one atomic each).
+        if (a.giants && c >= giant_min(kGiantBuckets - 1)) {
+          int b = kGiantBuckets - 1;
+          while (b > 0 && c >= giant_min(b - 1)) b--;
+          const uint32_t at =
+              atomicAdd(&a.tile_counters[kGiantCounterOff + 2 * b * kCounterStride], 1u);
+          if (at < a.giant_cap) a.giants[(uint64_t)b * a.giant_cap + at] = tile;
+        }
+      }
+    }
+  } else {
+    const uint32_t q = local * kWave + lane_id();
+    if (q >= r.n) return;
+    const QueryRT qb{load_query(r.queries, points, q), pred};
+    const uint64_t g = r.qbase + q;
+    if (WRITE) {
+      lane_query<0, false, true>(t, qb, q + r.qid_base,
+                                 QueueEmit{a.arena, a.arena_cap, a.offsets[g]});
+    } else {
+      a.counts[g] = (uint32_t)lane_query<0, false, false>(t, qb, q, NoEmit{});
+    }
+  }
+}
+
 // One kernel per pass answers every tile of the batch. A request's predicate
 // is a run-time value of the walks (QueryRT), so the kernel holds one wave
 // walk and one lane walk. The tree view is read from device memory so that
 // the launch does not depend on the index's state. One wavefront per
 // workgroup: a workgroup's slot is freed as soon as its one wave runs out of
-// tiles (four-wave workgroups held theirs until the slowest wave ended).
+// tiles (four-wave workgroups held theirs until the slowest wave ended). The
+// write pass first takes the giant tiles, largest bucket first.
 template <bool WRITE>
 __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const BatchArgs a) {
   __shared__ WaveLds s_lds;
@@ -97,55 +160,41 @@ __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const Batch
   uint32_t* counters = a.tile_counters + (WRITE ? kClasses * kXcds * kCounterStride : 0u);
   const uint32_t xcd = blockIdx.x % kXcds;
   const uint32_t wave_end = h.class_end[kClassWavePoints];
+  // The giant lists are used only when every one of them held its tiles
+  // (a batch never has more wave tiles than the lists' capacity).
+  uint32_t* gc = a.tile_counters + kGiantCounterOff;
+  bool giants = false;
+  if (WRITE && a.giants) {
+    giants = true;
+    for (int b = 0; b < kGiantBuckets; b++) giants &= gc[2 * b * kCounterStride] <= a.giant_cap;
+  }
+  int bucket = giants ? 0 : kGiantBuckets;  // wave-uniform: list being drained
   uint32_t pos = 0;
-  // Every iteration consumes a tile (or finds none left), so n_tiles + 1
-  // iterations bound the loop.
-  for (uint32_t it = 0; it <= h.n_tiles; it++) {
-    uint32_t tile = grab_tile(counters, h.class_end, xcd, pos);
-    if (tile >= h.n_tiles) break;
-    // Wave tiles are Morton-ordered within their class (the sort keys lead
-    // with the class).
-    if (a.order && tile < wave_end) tile = a.order[tile];
-    const uint32_t ri = find_request(a.requests, h.n_requests, tile);
-    const BatchRequest r = a.requests[ri];
-    const uint32_t local = tile - r.tbase;
-    const bool wave = r.flags & kReqWaveTiles;
-    if (t.n == 0) {
-      if (wave) {
-        if (local < r.n && lane_id() == 0) a.counts[r.qbase + local] = 0;
-      } else {
-        const uint32_t q = local * kWave + lane_id();
-        if (q < r.n) a.counts[r.qbase + q] = 0;
-      }
-      continue;
-    }
-    const uint32_t lb = (r.flags >> kReqLeafShift) & 7u;
-    t.leaf_size = lb ? 1u << (lb - 1) : t0.leaf_size;
-    const bool points = r.flags & kReqPoints;
-    const int pred = points ? (int)RTS_CONTAINS : (int)((r.flags >> kReqPredShift) & 3u);
-    if (wave) {
-      if (local >= r.n) continue;
-      const QueryRT qb{load_query(r.queries, points, local), pred};
-      const uint64_t g = r.qbase + local;
-      if (WRITE) {
-        wave_query<0, false, true>(t, qb, local + r.qid_base,
-                                   QueueEmit{a.arena, a.arena_cap, a.offsets[g]}, s_lds);
-      } else {
-        const uint64_t c = wave_query<0, false, false>(t, qb, local, NoEmit{}, s_lds);
-        if (lane_id() == 0) a.counts[g] = (uint32_t)c;
-      }
-    } else {
-      const uint32_t q = local * kWave + lane_id();
-      if (q >= r.n) continue;
-      const QueryRT qb{load_query(r.queries, points, q), pred};
-      const uint64_t g = r.qbase + q;
-      if (WRITE) {
-        lane_query<0, false, true>(t, qb, q + r.qid_base,
-                                   QueueEmit{a.arena, a.arena_cap, a.offsets[g]});
-      } else {
-        a.counts[g] = (uint32_t)lane_query<0, false, false>(t, qb, q, NoEmit{});
+  // Every iteration takes a tile, from a list or the queue, or finds none
+  // left; listed tiles are wave tiles, so 2 * n_tiles + 1 iterations bound
+  // the loop.
+  for (uint64_t it = 0; it <= 2ull * h.n_tiles; it++) {
+    uint32_t tile = 0xFFFFFFFFu;
+    for (; bucket < kGiantBuckets; bucket++) {
+      uint32_t k = 0;
+      if (lane_id() == 0) k = atomicAdd(&gc[(2 * bucket + 1) * kCounterStride], 1u);
+      k = readlane_u32(k, 0);
+      if (k < gc[2 * bucket * kCounterStride]) {
+        tile = a.giants[(uint64_t)bucket * a.giant_cap + k];
+        break;
       }
     }
+    const bool listed = tile != 0xFFFFFFFFu;
+    if (!listed) {
+      // class_end from device memory: indexing the local copy by a run-time
+      // class would put that copy in private memory.
+      tile = grab_tile(counters, a.header->class_end, xcd, pos);
+      if (tile >= h.n_tiles) break;
+      // Wave tiles are Morton-ordered within their class (the sort keys
+      // lead with the class).
+      if (a.order && tile < wave_end) tile = a.order[tile];
+    }
+    run_tile<WRITE>(t0, t, a, h, tile, giants && !listed, s_lds);
   }
 }
 

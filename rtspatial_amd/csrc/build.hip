@@ -2,13 +2,14 @@
 //
 //   scene bounds  -> 30-bit Morton codes of FP32 box centres (15 bits/axis)
 //   radix sort    -> (sort.hip)
-//   gather        -> boxes in Morton order + id -> sorted-position map
 //   Karras 2012   -> binary hierarchy over the sorted codes ("Maximizing
 //                    Parallelism in the Construction of BVHs, Octrees, and k-d
-//                    Trees"), ties broken by position
-//   refit         -> bottom-up union of child bounds; at every internal node the
-//                    second of its two children to arrive carries on upward
-//                    (atomic counter per node, device-scope acquire/release)
+//                    Trees"), ties broken by position; small nodes answered
+//                    from an LDS window of the keys
+//   refit         -> bottom-up union of child bounds in three levels (LDS
+//                    node state per chunk, then the nodes that span chunks);
+//                    a build's refit also gathers the boxes into Morton order
+//                    and the id -> sorted-position map
 //
 // Mutations (replacing the reference's OptiX refits, spatial_index.cuh:208-336)
 // write boxes by id and in sorted position, then refit the tree.
@@ -100,22 +101,16 @@ __global__ void __launch_bounds__(kThreads) k_morton_points(const float2* __rest
   }
 }
 
-// ------------------------------------------------------------ gather ----
-__global__ void __launch_bounds__(kThreads) k_gather(const float4* __restrict__ boxes,
-                                                     uint32_t* __restrict__ ids, uint32_t n,
-                                                     uint32_t off, float4* __restrict__ leaf_boxes,
-                                                     uint32_t* __restrict__ leaf_of_id) {
-  for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < n; k += gridDim.x * kThreads) {
-    const uint32_t local = ids[k];
-    if (local >= n) continue;  // unreachable for a correct sort; keeps reads in bounds
-    const uint32_t id = local + off;
-    if (off) ids[k] = id;
-    leaf_boxes[k] = boxes[id];
-    leaf_of_id[id] = off + k;
-  }
+// ------------------------------------------------------------ karras ----
+// Record meta of internal node i with far end j and split gamma: {left ref,
+// right ref, first leaf, last leaf}; a child is a leaf when its range is the
+// one position gamma (left) or gamma + 1 (right).
+__device__ __forceinline__ uint4 node_meta(uint32_t i, uint32_t j, uint32_t gamma) {
+  const uint32_t first = i < j ? i : j, last = i < j ? j : i;
+  return make_uint4(first == gamma ? (gamma | kLeafFlag) : gamma,
+                    last == gamma + 1 ? ((gamma + 1) | kLeafFlag) : gamma + 1, first, last);
 }
 
-// ------------------------------------------------------------ karras ----
 // Length of the common prefix of (key, position) at sorted positions i and j;
 // -1 outside [0, n).
 __device__ __forceinline__ int prefix_len(const uint32_t* __restrict__ keys, int64_t n, int64_t i,
@@ -126,10 +121,19 @@ __device__ __forceinline__ int prefix_len(const uint32_t* __restrict__ keys, int
   return __clz(a ^ b);
 }
 
+// Karras 2012 over the sorted (key, position) pairs: node i's direction, its
+// range's far end j by an exponential then a binary search, and its split
+// gamma by a binary search. The refit writes each node's record whole (one
+// 64-byte line from one thread: partial-line writes from two kernels cost
+// more than the line): this kernel stores j in the compact node_end array,
+// from which (with the arriving child) the refit rebuilds the record's meta,
+// and writes the children's parent codes. Only a node whose range spans
+// refit chunks -- finished by refit levels 2 and 3, which read meta from the
+// record -- gets its meta here.
 __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict__ keys, uint32_t n,
                                                      NodeRec* __restrict__ nodes,
                                                      uint32_t* __restrict__ parent,
-                                                     uint32_t* __restrict__ flags,
+                                                     uint32_t* __restrict__ node_end,
                                                      uint32_t chunk_log2) {
   const int64_t N = n;
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < N - 1;
@@ -149,19 +153,15 @@ __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict_
       if (prefix_len(keys, N, i, i + (s + t) * d) > dnode) s += t;
     } while (t > 1);
     const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
-    const int64_t first = i < j ? i : j, last = i < j ? j : i;
-    const uint32_t left = (first == gamma) ? ((uint32_t)gamma | kLeafFlag) : (uint32_t)gamma;
-    const uint32_t right =
-        (last == gamma + 1) ? ((uint32_t)(gamma + 1) | kLeafFlag) : (uint32_t)(gamma + 1);
-    nodes[i].meta = make_uint4(left, right, (uint32_t)first, (uint32_t)last);
-    const uint32_t lslot = (left & kLeafFlag) ? (uint32_t)(N - 1) + (left & ~kLeafFlag) : left;
-    const uint32_t rslot = (right & kLeafFlag) ? (uint32_t)(N - 1) + (right & ~kLeafFlag) : right;
+    const uint4 m = node_meta((uint32_t)i, (uint32_t)j, (uint32_t)gamma);
+    node_end[i] = (uint32_t)j;
+    const uint32_t lslot = (m.x & kLeafFlag) ? (uint32_t)(N - 1) + (m.x & ~kLeafFlag) : m.x;
+    const uint32_t rslot = (m.y & kLeafFlag) ? (uint32_t)(N - 1) + (m.y & ~kLeafFlag) : m.y;
     // kSpanBit: node i's leaf range crosses a refit chunk boundary.
-    const uint32_t span =
-        ((uint64_t)first >> chunk_log2) != ((uint64_t)last >> chunk_log2) ? kSpanBit : 0u;
+    const uint32_t span = (m.z >> chunk_log2) != (m.w >> chunk_log2) ? kSpanBit : 0u;
+    if (span) nodes[i].meta = m;
     parent[lslot] = ((uint32_t)i << 1) | 0u | span;
     parent[rslot] = ((uint32_t)i << 1) | 1u | span;
-    flags[i] = 0;
   }
 }
 
@@ -208,12 +208,24 @@ __device__ __forceinline__ void push_boundary(uint32_t* slots, uint32_t code, ui
 }
 
 // LDS of a level-1 workgroup, per node of the chunk (index - c0): the union
-// of the bounds of the children that arrived, (live leaves << 1) | arrivals,
-// and the node's own parent code.
+// of the bounds of the children that arrived, the node's arrival word and
+// its own parent code. The arrival word holds the node's far end (node_end,
+// as an offset in the chunk) in bits 20..30 and adds (live leaves << 1) | 1
+// per arrival. Each arrival stores its bounds and live count in its slot of
+// the node's record; the second also stores the record's meta, rebuilt from
+// the far end and the arriving child, so that the hierarchy kernel need not
+// write a 16-byte part of every record line. (Writing the whole record from
+// the second arrival, with both children's bounds in LDS, was slower: 80 KB
+// of LDS per workgroup and four 16-byte stores per record, 806 vs 672 us on
+// 10M boxes.)
+constexpr uint32_t kEndShift = 20;
+template <bool GATHER>
 __global__ void __launch_bounds__(kRefitThreads) k_refit_chunk(
-    const float4* __restrict__ leaf_boxes, uint32_t n, uint32_t chunk_log2,
+    float4* __restrict__ leaf_boxes, uint32_t n, uint32_t chunk_log2,
     const uint32_t* __restrict__ parent, NodeRec* nodes, float4* __restrict__ root_box,
-    uint32_t* __restrict__ root_live, uint32_t* error, uint32_t* boundary) {
+    uint32_t* __restrict__ root_live, uint32_t* error, uint32_t* boundary,
+    const uint32_t* __restrict__ node_end, const float4* __restrict__ boxes,
+    uint32_t* __restrict__ ids, uint32_t off, uint32_t* __restrict__ leaf_of_id) {
   extern __shared__ float4 s_box[];
   const uint32_t C = 1u << chunk_log2;
   uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_box + C);
@@ -222,13 +234,30 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_chunk(
   const uint32_t c1 = (c0 + C < n ? c0 + C : n) - 1;  // inclusive
   uint32_t* slots = boundary + (uint64_t)blockIdx.x * kRefitChunkStride;
   for (uint32_t j = threadIdx.x; j < C; j += kRefitThreads) {
+    const bool internal = c0 + j + 1 < n;  // internal nodes are 0 .. n - 2
     s_box[j] = empty_box();
-    s_cnt[j] = 0;
-    s_par[j] = c0 + j + 1 < n ? parent[c0 + j] : 0u;  // internal nodes are 0 .. n - 2
+    // A node spanning chunks has its far end outside the chunk; level 1
+    // never completes it, so the masked offset is not used.
+    s_cnt[j] = internal ? ((node_end[c0 + j] - c0) & (C - 1)) << kEndShift : 0u;
+    s_par[j] = internal ? parent[c0 + j] : 0u;
   }
   __syncthreads();
   for (uint32_t k = c0 + threadIdx.x; k <= c1; k += kRefitThreads) {
-    float4 b = leaf_boxes[k];
+    float4 b;
+    if (GATHER) {
+      const uint32_t local = ids[k];
+      if (local >= n) {  // unreachable for a correct sort; keeps reads in bounds
+        atomicOr(error, 2u);
+        continue;
+      }
+      const uint32_t id = local + off;
+      if (off) ids[k] = id;
+      b = boxes[id];
+      leaf_boxes[k] = b;
+      leaf_of_id[id] = off + k;
+    } else {
+      b = leaf_boxes[k];
+    }
     uint32_t live = 1;
     if (box_dead(b)) {
       b = empty_box();
@@ -244,6 +273,7 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_chunk(
       continue;
     }
     uint32_t code = parent[(n - 1) + k];
+    uint32_t self = k;  // position (leaf) or index (internal) of the arriving child
     for (int level = 0;; level++) {
       const uint32_t p = (code & ~kSpanBit) >> 1, side = code & 1u;
       if (level >= kMaxLevels || p >= n - 1) {
@@ -271,7 +301,11 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_chunk(
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       put_child(nodes + p, side, b, live);
       if (!(old & 1u)) break;  // first arrival: the sibling carries on
-      live += old >> 1;
+      // Second arrival: the union is whole (a wave's LDS operations complete
+      // in order, and the sibling's count shows here). The split is the
+      // arriving child's position (left side) or the one before it (right).
+      nodes[p].meta = node_meta(p, c0 + (old >> kEndShift), side ? self - 1 : self);
+      live += (old & ((1u << kEndShift) - 1)) >> 1;
       b = s_box[j];
       if (p == 0) {
         root_box[0] = b;
@@ -279,6 +313,7 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_chunk(
         break;
       }
       code = s_par[j];
+      self = p;
     }
   }
 }
@@ -452,13 +487,6 @@ hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* sce
   return hipGetLastError();
 }
 
-hipError_t launch_gather_leaves(const float4* boxes, uint32_t* sorted_ids, uint32_t n, uint32_t off,
-                                float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kThreads), 0, stream, boxes, sorted_ids, n,
-                     off, leaf_boxes, leaf_of_id);
-  return hipGetLastError();
-}
 
 // Every stride-th element of `src` into dst[0, m): the autotuner's query
 // sample. Elements are 8-byte float points, 16-byte float envelopes or double
@@ -489,10 +517,10 @@ hipError_t launch_gather_strided(const void* src, uint32_t elem_bytes, uint64_t 
 }
 
 hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes, uint32_t* parent,
-                         uint32_t* flags, hipStream_t stream) {
+                         uint32_t* node_end, hipStream_t stream) {
   if (n < 2) return hipSuccess;
   hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, sorted_keys, n,
-                     nodes, parent, flags, refit_chunk_log2(n));
+                     nodes, parent, node_end, refit_chunk_log2(n));
   return hipGetLastError();
 }
 
@@ -508,9 +536,15 @@ hipError_t launch_refit(const RefitArgs& a, hipStream_t stream) {
     hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(n - 1)), dim3(kThreads), 0, stream, a.flags,
                        n - 1);
   }
-  const size_t lds = (size_t)(1u << lg) * (sizeof(float4) + 2 * sizeof(uint32_t));
-  hipLaunchKernelGGL(k_refit_chunk, dim3(chunks), dim3(kRefitThreads), lds, stream, a.leaf_boxes,
-                     n, lg, a.parent, a.nodes, a.root_box, a.root_live, a.error, a.boundary);
+  const size_t lds = refit_chunk_lds_bytes(lg);
+  if (a.gather)
+    hipLaunchKernelGGL(k_refit_chunk<true>, dim3(chunks), dim3(kRefitThreads), lds, stream,
+                       a.leaf_boxes, n, lg, a.parent, a.nodes, a.root_box, a.root_live, a.error,
+                       a.boundary, a.node_end, a.boxes, a.ids, a.off, a.leaf_of_id);
+  else
+    hipLaunchKernelGGL(k_refit_chunk<false>, dim3(chunks), dim3(kRefitThreads), lds, stream,
+                       a.leaf_boxes, n, lg, a.parent, a.nodes, a.root_box, a.root_live, a.error,
+                       a.boundary, a.node_end, nullptr, nullptr, 0u, nullptr);
   if (chunks > 1) {
     const uint32_t groups = (chunks + kRefitGroup - 1) / kRefitGroup;
     hipLaunchKernelGGL(k_refit_span<false>, dim3(groups), dim3(kRefitSpanThreads), 0, stream,

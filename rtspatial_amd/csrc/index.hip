@@ -109,6 +109,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   dev_free(ix->vals_alt);
   dev_free(ix->nodes);
   dev_free(ix->parent);
+  dev_free(ix->node_end);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);
   dev_free(ix->partials);
@@ -119,6 +120,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   RTS_HIP(dev_alloc(&ix->vals_alt, cap));
   RTS_HIP(dev_alloc(&ix->nodes, cap));
   RTS_HIP(dev_alloc(&ix->parent, 2 * cap));
+  RTS_HIP(dev_alloc(&ix->node_end, cap));
   RTS_HIP(dev_alloc(&ix->flags, cap + refit_boundary_words(cap)));
   RTS_HIP(dev_alloc(&ix->sort_scratch, sort_scratch_bytes(cap)));
   sort_scratch_carve(ix->sort_scratch, cap, &ix->ss);
@@ -175,6 +177,7 @@ RefitArgs refit_args(rts_index* ix, bool delta) {
   a.leaf_boxes = ix->leaf_boxes + off;
   a.n = (uint32_t)(delta ? ix->n_delta : ix->n_main);
   a.parent = ix->parent + 2 * off;
+  a.node_end = ix->node_end + off;
   a.nodes = ix->nodes + off;
   a.root_box = delta ? ix->delta_root_box : ix->root_box;
   a.root_live = delta ? ix->delta_root_live : ix->root_live;
@@ -209,12 +212,16 @@ int build_tree(rts_index* ix, hipStream_t stream) {
     std::swap(ix->leaf_ids, ix->vals_alt);
   }
   ph = phase_begin(RTS_PHASE_BUILD_TREE, stream);
-  RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids, n, 0, ix->leaf_boxes, ix->leaf_of_id,
-                               stream));
-  RTS_HIP(launch_karras(ix->keys, n, ix->nodes, ix->parent, ix->flags, stream));
+  RTS_HIP(launch_karras(ix->keys, n, ix->nodes, ix->parent, ix->node_end, stream));
   ix->n_main = N;
   ix->n_delta = 0;
-  RTS_HIP(launch_refit(refit_args(ix, false), stream));
+  RefitArgs ra = refit_args(ix, false);  // the refit also gathers the leaves
+  ra.gather = true;
+  ra.boxes = ix->boxes;
+  ra.ids = ix->leaf_ids;
+  ra.off = 0;
+  ra.leaf_of_id = ix->leaf_of_id;
+  RTS_HIP(launch_refit(ra, stream));
   phase_end(ph, stream);
   ix->n_tree = N;
   ix->stale = false;
@@ -249,12 +256,16 @@ int build_delta(rts_index* ix, hipStream_t stream) {
                            hipMemcpyDeviceToDevice, stream));
   }
   ph = phase_begin(RTS_PHASE_BUILD_TREE, stream);
-  RTS_HIP(launch_gather_leaves(ix->boxes, ix->leaf_ids + off, nd, (uint32_t)off,
-                               ix->leaf_boxes + off, ix->leaf_of_id, stream));
-  RTS_HIP(launch_karras(ix->keys + off, nd, ix->nodes + off, ix->parent + 2 * off, ix->flags,
-                        stream));
+  RTS_HIP(launch_karras(ix->keys + off, nd, ix->nodes + off, ix->parent + 2 * off,
+                        ix->node_end + off, stream));
   ix->n_delta = nd;
-  RTS_HIP(launch_refit(refit_args(ix, true), stream));
+  RefitArgs ra = refit_args(ix, true);  // the refit also gathers the leaves
+  ra.gather = true;
+  ra.boxes = ix->boxes;
+  ra.ids = ix->leaf_ids + off;
+  ra.off = (uint32_t)off;
+  ra.leaf_of_id = ix->leaf_of_id;
+  RTS_HIP(launch_refit(ra, stream));
   phase_end(ph, stream);
   ix->n_tree = ix->n;
   ix->delta_builds++;
@@ -464,6 +475,7 @@ int rts_index_destroy(rts_index* ix) {
   dev_free(ix->vals_alt);
   dev_free(ix->nodes);
   dev_free(ix->parent);
+  dev_free(ix->node_end);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);
   dev_free(ix->partials);
@@ -1105,7 +1117,7 @@ int rts_index_memory(rts_index* ix, rts_memory_usage* out) {
   out->geometries = cap * (sizeof(float4) + sizeof(uint32_t)) + (ix->boxes64 ? cap * sizeof(Box64) : 0);
   out->aabbs = t * sizeof(float4);
   out->bvh = t ? (t * sizeof(uint32_t) * 4                      // leaf ids, keys, keys/vals alt
-                  + t * sizeof(NodeRec) + 2 * t * sizeof(uint32_t) // nodes, parents
+                  + t * sizeof(NodeRec) + 3 * t * sizeof(uint32_t) // nodes, parents, far ends
                   + (t + refit_boundary_words(t)) * sizeof(uint32_t) // flags + boundary list
                   + sort_scratch_bytes(t) + ix->n_partials_cap * sizeof(float4))
                : 0;

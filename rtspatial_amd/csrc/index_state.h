@@ -40,6 +40,7 @@ struct rts_index {
   uint32_t* vals_alt = nullptr;
   rts::NodeRec* nodes = nullptr;
   uint32_t* parent = nullptr;
+  uint32_t* node_end = nullptr;  // far end of each internal node's leaf range (refit meta)
   uint32_t* flags = nullptr;
   char* sort_scratch = nullptr;
   rts::SortScratch ss{};

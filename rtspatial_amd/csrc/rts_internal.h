@@ -52,20 +52,17 @@ hipError_t launch_morton_boxes(const float4* boxes, uint32_t n, const float4* sc
                                uint32_t* keys, hipStream_t stream);
 hipError_t launch_morton_points(const float2* pts, uint32_t n, const float4* scene,
                                 uint32_t* keys, hipStream_t stream);
-// Leaves in Morton order: for k < n, with id = sorted_ids[k] + off (sorted_ids
-// holds ids counted from `off` and is rewritten to the ids themselves):
-// leaf_boxes[k] = boxes[id], leaf_of_id[id] = off + k. off > 0 places a delta
-// tree over ids [off, off + n) at positions [off, off + n); the caller passes
-// sorted_ids and leaf_boxes already advanced by off.
-hipError_t launch_gather_leaves(const float4* boxes, uint32_t* sorted_ids, uint32_t n, uint32_t off,
-                                float4* leaf_boxes, uint32_t* leaf_of_id, hipStream_t stream);
 // Every stride-th element of src (elem_bytes 8, 16 or 32) into dst[0, m).
 hipError_t launch_gather_strided(const void* src, uint32_t elem_bytes, uint64_t stride, void* dst,
                                  uint32_t m, hipStream_t stream);
-hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes,
-                         uint32_t* parent, uint32_t* flags, hipStream_t stream);
+// Karras hierarchy over n sorted keys: the children's parent codes and each
+// internal node's far end node_end[i] (the other end of its leaf range; n - 1
+// words, kept for every refit, which writes the node records); only nodes
+// spanning refit chunks get their record's meta here.
+hipError_t launch_karras(const uint32_t* sorted_keys, uint32_t n, NodeRec* nodes, uint32_t* parent,
+                         uint32_t* node_end, hipStream_t stream);
 struct RefitArgs {
-  const float4* leaf_boxes;
+  float4* leaf_boxes;
   uint32_t n;
   const uint32_t* parent;
   NodeRec* nodes;
@@ -74,11 +71,24 @@ struct RefitArgs {
   uint32_t* flags;
   uint32_t* error;
   uint32_t* boundary;  // refit_boundary_words(n): per-chunk boundary slots, then the top list
+  const uint32_t* node_end;  // the hierarchy kernel's far ends (records' meta)
+  // Build refits also gather the leaves: leaf k's box is boxes[id] with
+  // id = ids[k] + off (ids counted from off, rewritten to the ids themselves
+  // when off > 0), stored to leaf_boxes[k], and leaf_of_id[id] = off + k. off
+  // > 0 places a delta tree over ids [off, off + n) at positions [off, off +
+  // n): leaf_boxes and ids are then passed advanced by off.
+  bool gather = false;
+  const float4* boxes = nullptr;
+  uint32_t* ids = nullptr;
+  uint32_t off = 0;
+  uint32_t* leaf_of_id = nullptr;
 };
 // Refit chunks: C = 2^refit_chunk_log2(n) leaves per level-1 workgroup, whose
-// LDS holds 24 bytes per node of the chunk (48 KB at 2048 leaves: three
-// workgroups per CU).
+// LDS holds 24 bytes per node of the chunk (48 KB at 2048 leaves).
 __host__ __device__ constexpr uint32_t refit_chunk_log2(uint64_t) { return 11u; }
+inline size_t refit_chunk_lds_bytes(uint32_t lg) {
+  return (size_t)(1u << lg) * (sizeof(float4) + 2 * sizeof(uint32_t));
+}
 // Parent codes (node << 1 | side) carry kSpanBit when the parent's leaf range
 // crosses a refit chunk boundary.
 constexpr uint32_t kSpanBit = 0x80000000u;

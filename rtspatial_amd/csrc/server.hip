@@ -110,6 +110,7 @@ struct Slot {
   unsigned long long* h_hash = nullptr;  // pinned: max
   char* d_desc = nullptr;
   uint32_t* d_counters = nullptr;
+  uint32_t* d_giants = nullptr;  // kGiantBuckets x max_batch_queries tile ids
   uint32_t* d_counts = nullptr;
   uint64_t* d_offsets = nullptr;
   uint64_t* d_seg = nullptr;
@@ -345,6 +346,8 @@ BatchArgs slot_args(const Engine& E, const Slot& s) {
   a.arena = s.d_arena;
   a.arena_cap = E.cfg.arena_pairs;
   a.order = nullptr;
+  a.giants = s.d_giants;
+  a.giant_cap = (uint32_t)E.cfg.max_batch_queries;
   return a;
 }
 
@@ -389,6 +392,7 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_hash), maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_desc, E.desc_bytes));
   SRV_HIP(hipMalloc(&s.d_counters, kTileCounterWords * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_giants, kGiantBuckets * maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_counts, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_offsets, maxq * sizeof(uint64_t)));
   SRV_HIP(hipMalloc(&s.d_seg, batch_seg_sums_words() * sizeof(uint64_t)));
@@ -407,7 +411,8 @@ void slot_free(Slot& s) {
   if (s.h_desc) (void)hipHostFree(s.h_desc);
   if (s.h_ranges) (void)hipHostFree(s.h_ranges);
   if (s.h_hash) (void)hipHostFree(s.h_hash);
-  for (void* p : {(void*)s.d_desc, (void*)s.d_counters, (void*)s.d_counts, (void*)s.d_offsets,
+  for (void* p : {(void*)s.d_desc, (void*)s.d_counters, (void*)s.d_giants, (void*)s.d_counts,
+                  (void*)s.d_offsets,
                   (void*)s.d_seg, (void*)s.d_total, (void*)s.d_ranges, (void*)s.d_hash,
                   (void*)s.d_arena, (void*)s.d_sort})
     if (p) (void)hipFree(p);

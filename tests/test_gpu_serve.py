@@ -174,6 +174,42 @@ def test_updates_interleaved_each_request_sees_its_epoch(oracle, scene, mode, po
     assert np.array_equal(got, trees[-1].pairs(int(pred), q))
 
 
+@pytest.mark.parametrize("sort_min_tiles", [1, 4096])
+def test_giant_queries_beside_small_ones(oracle, sort_min_tiles):
+    # Queries of 1.2M, ~300k and ~70k results (the write pass takes them
+    # first, largest first) in batches with thousands of small queries,
+    # with and without the Morton order of the wave tiles.
+    rng = np.random.default_rng(5)
+    n = 1_200_000
+    lo = rng.random((n, 2), dtype=np.float32)
+    boxes = np.concatenate([lo, lo + np.float32(1e-4)], axis=1)
+    log(f"serve giants sort_min_tiles={sort_min_tiles}")
+
+    def env(k, side):
+        c = rng.random((k, 2), dtype=np.float32) * np.float32(1 - side)
+        return np.concatenate([c, c + np.float32(side)], axis=1).astype(np.float32)
+
+    host = [
+        (np.array([[-1, -1, 2, 2], [-1, -1, 2, 2]], np.float32), P.kIntersects),
+        (env(3000, 2e-3), P.kIntersects),
+        (env(3, 0.5), P.kWithin),
+        (env(2000, 3e-3), P.kWithin),
+        (env(4, 0.24), P.kIntersects),
+        (rng.random((5000, 2), dtype=np.float32), P.kContains),
+        (np.array([[-1, -1, 2, 2]], np.float32), P.kWithin),
+    ]
+    reqs = [S.Request(queries=dev(q), predicate=p, arrival_s=0.0, collect=0) for q, p in host]
+    want = _collect_caps(oracle, boxes, host, reqs)
+    assert max(w.shape[0] for w in want) >= 1 << 20
+    ix = make_index(boxes)
+    res, st = S.run(ix, reqs, mode="batched", slots=3, sort_min_tiles=sort_min_tiles)
+    assert st["requests"] == len(host)
+    for i, (r, w) in enumerate(zip(res, want)):
+        assert r["count"] == w.shape[0], i
+        assert not r["truncated"]
+        assert np.array_equal(canon(r["pairs"]), w), i
+
+
 def test_empty_and_single_query_requests(oracle, scene):
     rng, means, boxes = scene
     log("serve tiny requests")
