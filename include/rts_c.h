@@ -494,7 +494,8 @@ typedef struct rts_serve_config {
   uint32_t use_graphs;         /* must be 0: hipGraph replay of a slot's launches is refused
                                   (RTS_ERR_INVALID), because a replay run did not complete
                                   on the MI355X and the cause is not established */
-  uint64_t max_batch_queries;  /* queries per batch (4M) */
+  uint64_t max_batch_queries;  /* queries per batch (4M). Each slot also holds a range log of
+                                  192 8-byte entries per query (6.4 GB per slot at 4M) */
   uint64_t chunk_queries;      /* largest chunk of one request (256K) */
   uint64_t arena_pairs;        /* result arena per slot (or of the plain path), pairs */
   uint32_t hash_results;       /* compute each request's pair hash (verification runs) */

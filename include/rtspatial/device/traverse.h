@@ -369,10 +369,36 @@ template <typename E>
 struct HasBeginStep<E, std::void_t<decltype(std::declval<E&>().begin_step(0ull, 0u))>>
     : std::true_type {};
 
-template <int PRED, bool POINTS, bool WRITE, typename Q, typename Emit>
+// A walk's leaf-range log (LOG walks). A counting walk records in `entries`
+// (room for `cap`) every leaf range that a writing walk of the same query
+// scans -- {first leaf | kAllMatch, leaves}, a subtree inside the query
+// included -- and sets `logged` to their number (more than cap: they did not
+// all fit). A writing walk given n_in <= cap entries scans those ranges
+// instead of walking the tree: the same leaves, tested the same way, hence
+// the same results (in another order). The write pass of a two-pass query
+// then skips the dependent chain of node loads of a second walk.
+constexpr uint32_t kLogOverflow = 0xFFFFFFFFu;
+// Log entries bypass the caches both ways (written once, read once), so
+// that they do not evict the nodes and leaf data the walks re-read.
+__device__ __forceinline__ void log_store(uint2* p, uint32_t first, uint32_t leaves) {
+  __builtin_nontemporal_store(((uint64_t)leaves << 32) | first, reinterpret_cast<uint64_t*>(p));
+}
+__device__ __forceinline__ uint2 log_load(const uint2* p) {
+  const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(p));
+  return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+}
+struct RangeLog {
+  uint2* entries = nullptr;
+  uint32_t cap = 0;
+  uint32_t logged = 0;           // counting walk: entries found (wave-uniform)
+  uint32_t n_in = kLogOverflow;  // writing walk: entries to scan (kLogOverflow: walk the tree)
+};
+
+template <int PRED, bool POINTS, bool WRITE, bool LOG = false, typename Q, typename Emit>
 __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qid,
-                               Emit&& emit, WaveLds& L) {
+                               Emit&& emit, WaveLds& L, RangeLog* log = nullptr) {
   constexpr bool kCounting = !WRITE;
+  constexpr bool kRecord = LOG && !WRITE;
   const uint32_t lane = lane_id();
   uint32_t* stack = L.stack;
   uint32_t* rfirst = L.rfirst;
@@ -380,30 +406,46 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
   uint32_t sp = 0, m = 0, total = 0;  // wave-uniform
   uint64_t emitted = 0;               // wave-uniform: results of scanned ranges
   uint64_t inside_live = 0;           // per lane: counting mode, subtrees inside the query
+  uint32_t nlog = 0;                  // wave-uniform: kRecord, ranges logged
+  uint32_t lpos = 0, lend = 0;        // wave-uniform: replayed log entries [lpos, lend)
 
-  const float4 rb = t.root_box[0];
-  if (node_test<PRED, POINTS>(rb, qb)) {
-    const bool inside = inside_test<PRED, POINTS>(rb, qb);
-    if (kCounting && inside) {
-      if (lane == 0) inside_live = t.root_live[0];
-    } else if (t.n <= t.leaf_size || t.n == 1 || inside) {
-      if (lane == 0) {
-        rfirst[0] = (inside && t.root_live[0] == t.n) ? kAllMatch : 0u;
-        rpre[0] = 0;
-        rpre[1] = t.n;
+  if (LOG && WRITE && log->n_in <= log->cap) {
+    lend = log->n_in;  // replay: no walk
+  } else {
+    const float4 rb = t.root_box[0];
+    if (node_test<PRED, POINTS>(rb, qb)) {
+      const bool inside = inside_test<PRED, POINTS>(rb, qb);
+      if (kCounting && inside) {
+        if (lane == 0) inside_live = t.root_live[0];
+        if (kRecord) {
+          if (lane == 0 && log->cap > 0)
+            log_store(log->entries, t.root_live[0] == t.n ? kAllMatch : 0u, t.n);
+          nlog = 1;
+        }
+      } else if (t.n <= t.leaf_size || t.n == 1 || inside) {
+        const uint32_t f = (inside && t.root_live[0] == t.n) ? kAllMatch : 0u;
+        if (lane == 0) {
+          rfirst[0] = f;
+          rpre[0] = 0;
+          rpre[1] = t.n;
+        }
+        m = 1;
+        total = t.n;
+        if (kRecord) {
+          if (lane == 0 && log->cap > 0) log_store(log->entries, f, t.n);
+          nlog = 1;
+        }
+      } else {
+        if (lane == 0) stack[0] = 0;
+        sp = 1;
       }
-      m = 1;
-      total = t.n;
-    } else {
-      if (lane == 0) stack[0] = 0;
-      sp = 1;
     }
   }
   lds_fence();
 
   const uint64_t max_iters = 4ull * t.n + 64;
   uint64_t iters = 0;
-  while (sp > 0 || m > 0) {
+  while (sp > 0 || m > 0 || lpos < lend) {
     if (++iters > max_iters) {
       if (lane == 0) atomicOr(t.error, kErrTraversal);
       break;
@@ -419,6 +461,7 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
       const bool degrade = sp + 2 * kWave > (uint32_t)kWaveStack;
       uint32_t push0 = 0, push1 = 0, np = 0;
       uint32_t rf0 = 0, rn0 = 0, rf1 = 0, rn1 = 0, nr = 0;
+      uint32_t lf0 = 0, ln0 = 0, lf1 = 0, ln1 = 0, nl = 0;  // kRecord: this lane's log entries
       if (act && e < t.n - 1) {
         const NodeRec* nd = t.nodes + e;
         const uint4 mt = nd->meta;
@@ -435,6 +478,17 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
           const bool inside = inside_test<PRED, POINTS>(cb[side], qb);
           if (kCounting && inside) {
             inside_live += live[side];
+            if (kRecord) {
+              const uint32_t f = first[side] | (live[side] == cnt[side] ? kAllMatch : 0u);
+              if (nl == 0) {
+                lf0 = f;
+                ln0 = cnt[side];
+              } else {
+                lf1 = f;
+                ln1 = cnt[side];
+              }
+              nl++;
+            }
           } else if ((ref[side] & kLeafFlag) || cnt[side] <= t.leaf_size || degrade || inside) {
             uint32_t f = first[side];
             if (inside && live[side] == cnt[side]) f |= kAllMatch;
@@ -446,6 +500,16 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
               rn1 = cnt[side];
             }
             nr++;
+            if (kRecord) {
+              if (nl == 0) {
+                lf0 = f;
+                ln0 = cnt[side];
+              } else {
+                lf1 = f;
+                ln1 = cnt[side];
+              }
+              nl++;
+            }
           } else {
             if (np == 0) push0 = ref[side]; else push1 = ref[side];
             np++;
@@ -476,6 +540,32 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
       m += rtot;
       total += stot;
       if (lane == 0) rpre[m] = total;
+      if (kRecord) {
+        const uint32_t linc = wave_inclusive_sum(nl);
+        const uint32_t at = nlog + (linc - nl);
+        if (nl > 0 && at < log->cap) log_store(log->entries + at, lf0, ln0);
+        if (nl > 1 && at + 1 < log->cap) log_store(log->entries + at + 1, lf1, ln1);
+        nlog += readlane_u32(linc, kWave - 1);
+      }
+      lds_fence();
+    } else if (LOG && WRITE && m == 0 && lpos < lend) {
+      // ---- replay: the next logged ranges into the range list
+      const uint32_t c = lend - lpos < (uint32_t)kWaveRanges ? lend - lpos : (uint32_t)kWaveRanges;
+      uint32_t carry = 0;  // wave-uniform
+      for (uint32_t b0 = 0; b0 < c; b0 += kWave) {
+        const uint32_t i = b0 + lane;
+        const uint2 en = i < c ? log_load(log->entries + lpos + i) : make_uint2(0u, 0u);
+        const uint32_t inc = wave_inclusive_sum(en.y);
+        if (i < c) {
+          rfirst[i] = en.x;
+          rpre[i] = carry + (inc - en.y);
+        }
+        carry += readlane_u32(inc, kWave - 1);
+      }
+      if (lane == 0) rpre[c] = carry;
+      m = c;
+      total = carry;
+      lpos += c;
       lds_fence();
     } else {
       // ---- scan the queued leaf ranges, kScanStep leaves per step. Lane L
@@ -562,6 +652,7 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
       lds_fence();
     }
   }
+  if (kRecord) log->logged = nlog;
   return emitted + wave_sum_u64(inside_live);
 }
 

@@ -82,6 +82,7 @@ struct BatchArgs {
   const uint32_t* order;         // wave tiles in Morton order (tile -> tile), or null
   uint32_t* giants;              // kGiantBuckets lists of giant_cap tiles each
   uint32_t giant_cap;
+  RangeLogArgs log;  // range logs of the wave tiles' count walks (range_log.h)
 };
 
 // Morton ordering of a batch's wave tiles: keys (class << 30 | Morton code of

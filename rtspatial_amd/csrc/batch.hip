@@ -19,6 +19,7 @@
 #include "batch.h"
 #include "checksum.h"
 #include "morton.h"
+#include "range_log.h"
 #include "rtspatial/device/traverse.h"
 #include "rtspatial/device/wave.h"
 
@@ -83,7 +84,7 @@ __device__ __forceinline__ uint32_t grab_tile(uint32_t* counters, const uint32_t
 template <bool WRITE>
 __device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const BatchArgs& a,
                                          const BatchHeader& h, uint32_t tile, bool skip_giants,
-                                         WaveLds& lds) {
+                                         LogCursor& lc, WaveLds& lds) {
   const uint32_t ri = find_request(a.requests, h.n_requests, tile);
   const BatchRequest r = a.requests[ri];
   const uint32_t local = tile - r.tbase;
@@ -109,10 +110,13 @@ __device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const 
     const QueryRT qb{load_query(r.queries, points, local), pred};
     const uint64_t g = r.qbase + local;
     if (WRITE) {
-      wave_query<0, false, true>(t, qb, local + r.qid_base,
-                                 QueueEmit{a.arena, a.arena_cap, a.offsets[g]}, lds);
+      RangeLog lg = log_for_write(a.log, g);
+      wave_query<0, false, true, true>(t, qb, local + r.qid_base,
+                                       QueueEmit{a.arena, a.arena_cap, a.offsets[g]}, lds, &lg);
     } else {
-      const uint64_t c = wave_query<0, false, false>(t, qb, local, NoEmit{}, lds);
+      RangeLog lg = log_for_count(lc);
+      const uint64_t c = wave_query<0, false, false, true>(t, qb, local, NoEmit{}, lds, &lg);
+      log_commit(lc, a.log, g, lg);
       if (lane_id() == 0) {
         a.counts[g] = (uint32_t)c;
         // Giants are listed as they are counted (rare: one atomic each).
@@ -146,8 +150,11 @@ __device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const 
 // workgroup: a workgroup's slot is freed as soon as its one wave runs out of
 // tiles (four-wave workgroups held theirs until the slowest wave ended). The
 // write pass first takes the giant tiles, largest bucket first.
+// Occupancy is pinned at 7 (count) and 6 (write) waves per SIMD, 72 and 80
+// VGPRs, which the range logs would otherwise push just past (76 and 81).
 template <bool WRITE>
-__global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const BatchArgs a) {
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WRITE ? 6 : 7)))
+k_batch(const TreeView* tv, const BatchArgs a) {
   __shared__ WaveLds s_lds;
   const TreeView t0 = *tv;
   TreeView t = t0;  // leaf_size may differ per request
@@ -169,6 +176,7 @@ __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const Batch
     for (int b = 0; b < kGiantBuckets; b++) giants &= gc[2 * b * kCounterStride] <= a.giant_cap;
   }
   int bucket = giants ? 0 : kGiantBuckets;  // wave-uniform: list being drained
+  LogCursor lc = WRITE ? LogCursor{} : log_cursor(a.log);
   uint32_t pos = 0;
   // Every iteration takes a tile, from a list or the queue, or finds none
   // left; listed tiles are wave tiles, so 2 * n_tiles + 1 iterations bound
@@ -194,7 +202,7 @@ __global__ void __launch_bounds__(kWave) k_batch(const TreeView* tv, const Batch
       // lead with the class).
       if (a.order && tile < wave_end) tile = a.order[tile];
     }
-    run_tile<WRITE>(t0, t, a, h, tile, giants && !listed, s_lds);
+    run_tile<WRITE>(t0, t, a, h, tile, giants && !listed, lc, s_lds);
   }
 }
 
@@ -297,8 +305,10 @@ uint32_t persistent_blocks(K kernel, int threads) {
 }
 
 template <bool WRITE>
-void launch_batch_kernel(const TreeView* t, const BatchArgs& a, hipStream_t stream) {
+void launch_batch_kernel(const TreeView* t, const BatchArgs& a_in, hipStream_t stream) {
   static const uint32_t blocks = persistent_blocks(k_batch<WRITE>, kWave);
+  BatchArgs a = a_in;
+  a.log.per_wave = (uint32_t)(a.log.n_entries / blocks);
   hipLaunchKernelGGL((k_batch<WRITE>), dim3(blocks), dim3(kWave), 0, stream, t, a);
 }
 

@@ -590,6 +590,41 @@ int rts_index_update_d(rts_index* ix, const rts_update_d* ups, uint64_t n, rts_s
 namespace {
 
 // Per-call device scratch, carved from one stream-ordered allocation.
+// Stream-ordered scratch of query calls (range logs take gigabytes) comes
+// from one memory pool per device that keeps its memory: the default pool
+// hands freed memory back at every synchronisation, so each call paid for
+// the driver's allocation again (the per-request path's p99 rose from 272
+// ms to 1.69 s with range logs). The pools live until the process ends.
+hipError_t scratch_alloc(void** p, size_t bytes, hipStream_t stream) {
+  static std::mutex mu;
+  static hipMemPool_t pools[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  hipMemPool_t pool;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!pools[dev]) {
+      hipMemPoolProps props = {};
+      props.allocType = hipMemAllocationTypePinned;
+      props.handleTypes = hipMemHandleTypeNone;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      e = hipMemPoolCreate(&pools[dev], &props);
+      if (e != hipSuccess) {
+        pools[dev] = nullptr;
+        return e;
+      }
+      uint64_t keep = UINT64_MAX;
+      e = hipMemPoolSetAttribute(pools[dev], hipMemPoolAttrReleaseThreshold, &keep);
+      if (e != hipSuccess) return e;
+    }
+    pool = pools[dev];
+  }
+  return hipMallocFromPoolAsync(p, bytes, pool, stream);
+}
+
 struct QueryScratch {
   char* block = nullptr;
   const uint32_t* order = nullptr;  // Morton permutation (optional)
@@ -598,6 +633,7 @@ struct QueryScratch {
   uint64_t* total = nullptr;        //                [0] total, [1] queue base
   void* scan_tmp = nullptr;
   uint32_t* tickets = nullptr;      // wave traversal: per-XCD query tickets
+  RangeLogArgs log;                 // wave two-pass: the count walks' range logs
 };
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
@@ -611,6 +647,10 @@ int alloc_scratch(uint64_t nq, bool sort, bool wave, bool two_pass, hipStream_t 
                   size_t ext_bytes = 0, char** base_out = nullptr) {
   size_t off = 0;
   size_t o_keys = 0, o_sort = 0, o_counts = 0, o_offsets = 0, o_total = 0, o_scan = 0, o_tk = 0;
+  size_t o_log = 0, o_log_off = 0, o_log_n = 0;
+  // Range logs: not in a caller's static scratch.
+  const bool logs = two_pass && !ext;
+  const uint64_t log_entries = std::min<uint64_t>(nq * kLogPerQuery, kLogMaxEntries);
   if (wave) {
     o_tk = off;
     off = align_up(off + kWaveTicketWords * sizeof(uint32_t));
@@ -631,13 +671,21 @@ int alloc_scratch(uint64_t nq, bool sort, bool wave, bool two_pass, hipStream_t 
     o_scan = off;
     off = align_up(off + scan_scratch_bytes(nq));
   }
+  if (logs) {
+    o_log = off;
+    off = align_up(off + log_entries * sizeof(uint2));
+    o_log_off = off;
+    off = align_up(off + nq * sizeof(uint32_t));
+    o_log_n = off;
+    off = align_up(off + nq * sizeof(uint32_t));
+  }
   char* b = nullptr;
   if (ext) {
     RTS_CHECK_ARG(off <= ext_bytes, "query: the call's scratch exceeds the static scratch array");
     b = ext;
   } else {
     if (off == 0) return RTS_OK;
-    RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&qs->block), off, stream));
+    RTS_HIP(scratch_alloc(reinterpret_cast<void**>(&qs->block), off, stream));
     b = qs->block;
   }
   if (base_out) *base_out = b;
@@ -647,6 +695,12 @@ int alloc_scratch(uint64_t nq, bool sort, bool wave, bool two_pass, hipStream_t 
     qs->offsets = reinterpret_cast<uint64_t*>(b + o_offsets);
     qs->total = reinterpret_cast<uint64_t*>(b + o_total);
     qs->scan_tmp = b + o_scan;
+  }
+  if (logs) {
+    qs->log.entries = reinterpret_cast<uint2*>(b + o_log);
+    qs->log.n_entries = log_entries;
+    qs->log.off = reinterpret_cast<uint32_t*>(b + o_log_off);
+    qs->log.n = reinterpret_cast<uint32_t*>(b + o_log_n);
   }
   *order_off = o_keys;
   *sort_off = o_sort;
@@ -783,7 +837,7 @@ int run_plan(rts_index* ix, const Plan& plan, int pred, const void* queries, uin
         RTS_HIP(launch_query_wave(ql, wa, staged ? kPhaseWriteStaged : kPhaseCountSink, stream));
         phase_end(ph, stream);
       } else {
-        WaveArgs wa{qs.counts, qs.offsets, qs.total + 1, qs.tickets};
+        WaveArgs wa{qs.counts, qs.offsets, qs.total + 1, qs.tickets, qs.log};
         PhaseToken ph = phase_begin(RTS_PHASE_QUERY_COUNT, stream);
         RTS_HIP(launch_query_wave(ql, wa, kPhaseCount, stream));
         RTS_HIP(launch_exclusive_scan(qs.counts, m, qs.offsets, qs.total, qs.scan_tmp, stream));
@@ -830,7 +884,7 @@ int kernel_queries(rts_index* ix, const void* queries, uint64_t nq, bool points,
   *f64 = ix->cfg.coord == RTS_COORD_F64;
   if (ix->cfg.coord != RTS_COORD_F64_AS_F32) return RTS_OK;
   rq->stream = stream;
-  RTS_HIP(hipMallocAsync(&rq->p, nq * query_bytes(points, false), stream));
+  RTS_HIP(scratch_alloc(&rq->p, nq * query_bytes(points, false), stream));
   RTS_HIP(launch_round_queries64(queries, points, nq, rq->p, stream));
   *out = rq->p;
   return RTS_OK;
@@ -1037,8 +1091,8 @@ int rts_index_order_queries(rts_index* ix, const void* queries, uint64_t n, int 
   RTS_HIP(hipSetDevice(ix->device));
   char* block = nullptr;
   const size_t keys_bytes = align_up(4 * n * sizeof(uint32_t));
-  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&block), keys_bytes + sort_scratch_bytes(n),
-                         stream));
+  RTS_HIP(scratch_alloc(reinterpret_cast<void**>(&block), keys_bytes + sort_scratch_bytes(n),
+                        stream));
   const uint32_t* perm = nullptr;
   // Envelopes and segments share a layout; a segment is ordered by the
   // centre of its bounding box.
@@ -1133,8 +1187,8 @@ int rts_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* k
   if (n == 0) return RTS_OK;
   const size_t scratch = sort_scratch_bytes(n);
   char* base = nullptr;
-  RTS_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), 2 * n * sizeof(uint32_t) + scratch + 256,
-                         stream));
+  RTS_HIP(scratch_alloc(reinterpret_cast<void**>(&base), 2 * n * sizeof(uint32_t) + scratch + 256,
+                        stream));
   uint32_t* kb = reinterpret_cast<uint32_t*>(base);
   uint32_t* vb = kb + n;
   char* sbase = reinterpret_cast<char*>(

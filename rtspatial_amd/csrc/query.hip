@@ -16,6 +16,7 @@
 // device-scope atomic per ~1000 pairs and fully coalesced 8-byte stores, in
 // place of the reference's one atomic per result (utils/queue.h:22-27).
 #include "rts_internal.h"
+#include "range_log.h"
 #include "rtspatial/device/traverse.h"
 #include "rtspatial/device/wave.h"
 
@@ -244,6 +245,7 @@ __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const
     se.cap = dq->capacity;
   }
   uint32_t slice = blockIdx.x % kXcds;
+  LogCursor lc = MODE == kWaveCount ? log_cursor(a.log) : LogCursor{};
   // Every iteration takes a query or finds none left: q.n + 1 iterations
   // bound the loop.
   for (uint32_t it = 0; it <= q.n; it++) {
@@ -256,14 +258,24 @@ __global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const
       wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base, se, s_lds);
       continue;
     }
+    // Two-pass calls: the count walk logs its leaf ranges (range_log.h), the
+    // write pass replays them instead of walking again.
     if (MODE == kWaveWrite) {
       const rts_dev_queue* dq = static_cast<const rts_dev_queue*>(q.sink.arg);
-      wave_query<PRED, POINTS, true>(t, qb, qi + q.qid_base,
-                                     QueueEmit{dq->data, dq->capacity, a.base[0] + a.offsets[qi]},
-                                     s_lds);
+      RangeLog lg = log_for_write(a.log, qi);
+      wave_query<PRED, POINTS, true, true>(
+          t, qb, qi + q.qid_base, QueueEmit{dq->data, dq->capacity, a.base[0] + a.offsets[qi]},
+          s_lds, &lg);
       continue;
     }
-    const uint64_t all = wave_query<PRED, POINTS, false>(t, qb, 0, NoEmit{}, s_lds);
+    uint64_t all;
+    if (MODE == kWaveCount) {
+      RangeLog lg = log_for_count(lc);
+      all = wave_query<PRED, POINTS, false, true>(t, qb, 0, NoEmit{}, s_lds, &lg);
+      log_commit(lc, a.log, qi, lg);
+    } else {
+      all = wave_query<PRED, POINTS, false>(t, qb, 0, NoEmit{}, s_lds);
+    }
     if (lane_id() == 0) {
       if (MODE == kWaveCount) {
         a.counts[qi] = (uint32_t)all;
@@ -290,12 +302,14 @@ uint32_t wave_grid_blocks() {
 }
 
 template <int PRED, bool POINTS, int MODE, bool F64>
-hipError_t launch_wave_mode(const QueryLaunch& q, const WaveArgs& a, hipStream_t stream) {
-  hipError_t e = launch_fill_u32(a.tickets, 0u, kXcds * kTicketStride, stream);
+hipError_t launch_wave_mode(const QueryLaunch& q, const WaveArgs& a_in, hipStream_t stream) {
+  hipError_t e = launch_fill_u32(a_in.tickets, 0u, kXcds * kTicketStride, stream);
   if (e != hipSuccess) return e;
   const uint32_t resident = wave_grid_blocks();
   uint32_t blocks = resident < q.n ? resident : q.n;
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;  // whole rounds over the XCDs
+  WaveArgs a = a_in;
+  if (blocks) a.log.per_wave = (uint32_t)(a.log.n_entries / blocks);
   hipLaunchKernelGGL((k_query_wave<PRED, POINTS, MODE, F64>), dim3(blocks), dim3(kWave), 0, stream,
                      q, a);
   return hipGetLastError();

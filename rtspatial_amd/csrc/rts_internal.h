@@ -150,11 +150,25 @@ hipError_t launch_query(const QueryLaunch& q, hipStream_t stream);
 // 1: counts added to the counting handler's counter; 2: pairs written at
 // queue slot a.base[0] + a.offsets[query]; 3: pairs collected in one pass
 // (staged per wavefront, reserved in the queue by atomics).
+// Range logs of two-pass wave walks (range_log.h).
+struct RangeLogArgs {
+  uint2* entries = nullptr;  // null: no logs (both passes walk the tree)
+  uint64_t n_entries = 0;    // below 2^32
+  uint32_t per_wave = 0;     // entries of one wave's region: set by the launch (n_entries / grid)
+  uint32_t* off = nullptr;   // per query: first entry
+  uint32_t* n = nullptr;     // per query: entries, or kLogOverflow
+};
+// Range-log entries (8 bytes each) of a two-pass query call: 512 per query,
+// at most 2^29 (4 GiB).
+constexpr uint64_t kLogPerQuery = 512;
+constexpr uint64_t kLogMaxEntries = 1ull << 29;
+
 struct WaveArgs {
   uint32_t* counts;
   const uint64_t* offsets;
   const uint64_t* base;
   uint32_t* tickets;  // kWaveTicketWords words, cleared by the launch
+  RangeLogArgs log;   // two-pass calls: the count walks' range logs
 };
 constexpr uint32_t kWaveTicketWords = 8 * 16;
 enum { kPhaseCount = 0, kPhaseCountSink = 1, kPhaseWrite = 2, kPhaseWriteStaged = 3 };

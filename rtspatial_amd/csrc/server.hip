@@ -111,6 +111,9 @@ struct Slot {
   char* d_desc = nullptr;
   uint32_t* d_counters = nullptr;
   uint32_t* d_giants = nullptr;  // kGiantBuckets x max_batch_queries tile ids
+  uint2* d_log = nullptr;        // range-log entries (kServeLogPerQuery per query)
+  uint32_t* d_log_off = nullptr; // per query: first range-log entry
+  uint32_t* d_log_n = nullptr;   // per query: range-log entries
   uint32_t* d_counts = nullptr;
   uint64_t* d_offsets = nullptr;
   uint64_t* d_seg = nullptr;
@@ -333,6 +336,13 @@ uint32_t tile_flags(const Engine& E, const rts_request& r) {
   return f;
 }
 
+// Range-log entries of a slot (8 bytes each): 192 per query of a full batch
+// (envelope queries of the served mix log a few hundred leaf ranges each; a
+// query past its wave's share is walked again), at most 2^30.
+uint64_t serve_log_entries(uint64_t max_batch_queries) {
+  return std::min<uint64_t>(max_batch_queries * 192, 1ull << 30);
+}
+
 BatchArgs slot_args(const Engine& E, const Slot& s) {
   BatchArgs a;
   a.header = reinterpret_cast<const BatchHeader*>(s.d_desc + kHeaderOff);
@@ -348,6 +358,10 @@ BatchArgs slot_args(const Engine& E, const Slot& s) {
   a.order = nullptr;
   a.giants = s.d_giants;
   a.giant_cap = (uint32_t)E.cfg.max_batch_queries;
+  a.log.entries = s.d_log;
+  a.log.n_entries = serve_log_entries(E.cfg.max_batch_queries);
+  a.log.off = s.d_log_off;
+  a.log.n = s.d_log_n;
   return a;
 }
 
@@ -393,6 +407,9 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipMalloc(&s.d_desc, E.desc_bytes));
   SRV_HIP(hipMalloc(&s.d_counters, kTileCounterWords * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_giants, kGiantBuckets * maxq * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_log, serve_log_entries(maxq) * sizeof(uint2)));
+  SRV_HIP(hipMalloc(&s.d_log_off, maxq * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_log_n, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_counts, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_offsets, maxq * sizeof(uint64_t)));
   SRV_HIP(hipMalloc(&s.d_seg, batch_seg_sums_words() * sizeof(uint64_t)));
@@ -411,7 +428,8 @@ void slot_free(Slot& s) {
   if (s.h_desc) (void)hipHostFree(s.h_desc);
   if (s.h_ranges) (void)hipHostFree(s.h_ranges);
   if (s.h_hash) (void)hipHostFree(s.h_hash);
-  for (void* p : {(void*)s.d_desc, (void*)s.d_counters, (void*)s.d_giants, (void*)s.d_counts,
+  for (void* p : {(void*)s.d_desc, (void*)s.d_counters, (void*)s.d_giants, (void*)s.d_log,
+                  (void*)s.d_log_off, (void*)s.d_log_n, (void*)s.d_counts,
                   (void*)s.d_offsets,
                   (void*)s.d_seg, (void*)s.d_total, (void*)s.d_ranges, (void*)s.d_hash,
                   (void*)s.d_arena, (void*)s.d_sort})
