@@ -445,11 +445,15 @@ int rts_pairs_checksum(const rts_pair* pairs, uint64_t n, uint64_t n_queries, ui
  *     bytes fit the slot's arena, and the batch runs as one persistent
  *     work-stealing launch sequence (count, scan, write). Chunks whose results
  *     overflow the arena are re-launched.
+ *     Segment requests, and every request on a double index, are walked by
+ *     one count and one write launch each inside their batch (the plain
+ *     path's wave kernels), into the batch's counts, scan and arena.
  *   RTS_SERVE_PLAIN: the per-request path: each request (split the same way)
  *     is one rts_index_query_* call on one stream, in arrival order.
  *
- * The index must hold float coordinates (RTS_COORD_F32); a double index is
- * RTS_ERR_INVALID here.
+ * The index holds float coordinates (RTS_COORD_F32) or exact doubles
+ * (RTS_COORD_F64; requests then hold rts_point_d / rts_envelope_d /
+ * rts_line_d arrays); RTS_COORD_F64_AS_F32 is RTS_ERR_INVALID here.
  * ------------------------------------------------------------------------ */
 typedef enum rts_serve_mode {
   RTS_SERVE_BATCHED = 0,
@@ -460,7 +464,8 @@ typedef struct rts_request {
   double arrival_s;      /* eligible from this time (seconds after the run starts) */
   const void* queries;   /* device (or pinned) rts_point or rts_envelope array */
   uint64_t n;            /* queries */
-  int32_t points;        /* 1 = rts_point queries (predicate must be RTS_CONTAINS) */
+  int32_t points;        /* query kind: 0 = envelopes, 1 = points (predicate RTS_CONTAINS),
+                            2 = line segments (RTS_INTERSECTS, or RTS_CONTAINS: none) */
   int32_t predicate;     /* rts_predicate */
   rts_pair* out;         /* optional device buffer for the request's pairs, (geom_id,
                             query index within the request); NULL = results are
@@ -470,7 +475,7 @@ typedef struct rts_request {
 
 typedef struct rts_update_batch {
   double arrival_s;
-  const rts_update* updates; /* device (or pinned) */
+  const rts_update* updates; /* device (or pinned); rts_update_d records on a double index */
   uint64_t n;
 } rts_update_batch;
 

@@ -11,6 +11,7 @@ constexpr uint32_t kReqPoints = 1u;       // queries are points (float2)
 constexpr uint32_t kReqPredShift = 1;     // bits 1-2: rts_predicate
 constexpr uint32_t kReqWaveTiles = 8u;    // one query per wavefront (else 64 per tile)
 constexpr uint32_t kReqLeafShift = 4;     // bits 4-6: log2(leaf_size) + 1, 0 = the tree's
+constexpr uint32_t kReqExt = 128u;        // walked by per-request launches (BatchExt), no tiles
 
 struct BatchRequest {
   const void* queries;  // device (or pinned host) queries of the request
@@ -34,7 +35,12 @@ constexpr int kClassWavePoints = 3;
 constexpr int kClassLane = 4;  // 64 queries per tile, one per lane, any predicate
 constexpr int kClasses = 5;
 
+// Requests the batch kernels do not walk (segments, double queries) come
+// after every tile class; they own no tiles.
+constexpr int kClassExt = kClasses;
+
 __host__ __device__ inline int tile_class(uint32_t flags) {
+  if (flags & kReqExt) return kClassExt;
   if (!(flags & kReqWaveTiles)) return kClassLane;
   if (flags & kReqPoints) return kClassWavePoints;
   switch ((flags >> kReqPredShift) & 3) {
@@ -103,7 +109,22 @@ struct BatchSort {
 // graph and replayable for any batch whose sizes fit the buffers); otherwise
 // the wave tiles are Morton-ordered first. `t` is a device copy of the tree
 // view.
+// A request of the batch that the batch kernels do not walk: segment
+// queries, or any query on a double index. Its queries [qbase, qbase + n) of
+// the batch are counted and written by one wave launch each (query.hip, the
+// plain path's kernels) into the batch's counts, offsets and arena.
+struct BatchExt {
+  QueryLaunch ql;  // tree, queries, n, qid_base, pred, point_queries, f64, leaf size
+  uint64_t qbase;
+  bool none;       // a predicate that matches nothing (segments under RTS_CONTAINS)
+};
+struct BatchExtArgs {
+  uint32_t* tickets;     // kWaveTicketWords words
+  const uint64_t* zero;  // a device 0 (queue base of the arena)
+  const void* arena_queue;  // device rts_dev_queue over the slot's arena
+};
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a, const BatchSort& sort,
+                        const BatchExt* ext, size_t n_ext, const BatchExtArgs& ea,
                         hipStream_t stream);
 // hashes[c] = sum of pair_hash(query_id, geom_id) (checksum.h) over range c of
 // the arena (ranges as written by the batch: {first slot, count}, in order).

@@ -330,6 +330,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_keys(const BatchArgs a, cons
 }
 
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSort& sort,
+                        const BatchExt* ext, size_t n_ext, const BatchExtArgs& ea,
                         hipStream_t stream) {
   BatchArgs a = a_in;
   a.order = nullptr;
@@ -353,6 +354,19 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, a.tile_counters,
                      kTileCounterWords);
   launch_batch_kernel<false>(t, a, stream);
+  // Requests the batch kernel does not walk: one count launch each, into the
+  // same counts (so the scan below covers them).
+  for (size_t i = 0; i < n_ext; i++) {
+    const BatchExt& x = ext[i];
+    hipError_t e;
+    if (x.none) {
+      e = launch_fill_u32(a.counts + x.qbase, 0u, x.ql.n, stream);
+    } else {
+      const WaveArgs wa{a.counts + x.qbase, a.offsets + x.qbase, ea.zero, ea.tickets, {}};
+      e = launch_query_wave(x.ql, wa, kPhaseCount, stream);
+    }
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_seg_sums, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, stream, a);
   hipLaunchKernelGGL(k_seg_apply, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
@@ -360,6 +374,16 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
   phase_end(ph, stream);
   ph = timed ? phase_begin(RTS_PHASE_BATCH_WRITE, stream) : PhaseToken{-1, nullptr};
   launch_batch_kernel<true>(t, a, stream);
+  for (size_t i = 0; i < n_ext; i++) {
+    const BatchExt& x = ext[i];
+    if (x.none) continue;
+    QueryLaunch ql = x.ql;
+    ql.sink.mode = RTS_HANDLER_COLLECT;
+    ql.sink.arg = ea.arena_queue;
+    const WaveArgs wa{a.counts + x.qbase, a.offsets + x.qbase, ea.zero, ea.tickets, {}};
+    const hipError_t e = launch_query_wave(ql, wa, kPhaseWrite, stream);
+    if (e != hipSuccess) return e;
+  }
   phase_end(ph, stream);
   return hipGetLastError();
 }

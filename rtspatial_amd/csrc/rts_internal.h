@@ -146,10 +146,6 @@ struct QueryLaunch {
   int traversal;         // rts_traversal (resolved: THREAD or WAVE)
 };
 hipError_t launch_query(const QueryLaunch& q, hipStream_t stream);
-// Wave-per-query traversal. phase 0: per-query counts -> a.counts;
-// 1: counts added to the counting handler's counter; 2: pairs written at
-// queue slot a.base[0] + a.offsets[query]; 3: pairs collected in one pass
-// (staged per wavefront, reserved in the queue by atomics).
 // Range logs of two-pass wave walks (range_log.h).
 struct RangeLogArgs {
   uint2* entries = nullptr;  // null: no logs (both passes walk the tree)
@@ -171,6 +167,10 @@ struct WaveArgs {
   RangeLogArgs log;   // two-pass calls: the count walks' range logs
 };
 constexpr uint32_t kWaveTicketWords = 8 * 16;
+// Wave-per-query traversal. phase 0: per-query counts -> a.counts;
+// 1: counts added to the counting handler's counter; 2: pairs written at
+// queue slot a.base[0] + a.offsets[query]; 3: pairs collected in one pass
+// (staged per wavefront, reserved in the queue by atomics).
 enum { kPhaseCount = 0, kPhaseCountSink = 1, kPhaseWrite = 2, kPhaseWriteStaged = 3 };
 hipError_t launch_query_wave(const QueryLaunch& q, const WaveArgs& a, int phase,
                              hipStream_t stream);

@@ -100,7 +100,45 @@ struct Estimate {
   }
 };
 
-int kind_of(const rts_request& r) { return r.points ? 3 : (r.predicate & 3); }
+// rts_request.points: the query kind.
+constexpr int kEnvelopes = 0, kPoints = 1, kSegments = 2;
+
+// Estimate classes: envelope predicates 0..2, points 3, segments 4.
+constexpr int kKinds = 5;
+int kind_of(const rts_request& r) {
+  return r.points == kPoints ? 3 : r.points == kSegments ? 4 : (r.predicate & 3);
+}
+
+// Bytes of one query of r: float or double (a double index takes doubles).
+size_t query_bytes_of(const rts_index* ix, const rts_request& r) {
+  return (r.points == kPoints ? 8 : 16) * (ix->f64() ? 2 : 1);
+}
+
+// Requests the batch kernels do not walk: segments, and every query on a
+// double index (BatchExt).
+bool is_ext(const rts_index* ix, const rts_request& r) {
+  return r.points == kSegments || ix->f64();
+}
+
+// One query call of the plain path for n queries of r starting at qp.
+int query_request(rts_index* ix, const rts_request& r, const void* qp, uint64_t n,
+                  rts_handler handler, void* arg, hipStream_t stream) {
+  const rts_predicate pred = (rts_predicate)r.predicate;
+  if (r.points == kPoints)
+    return ix->f64() ? rts_index_query_points_d(ix, pred, static_cast<const rts_point_d*>(qp), n,
+                                                handler, arg, stream)
+                     : rts_index_query_points(ix, pred, static_cast<const rts_point*>(qp), n,
+                                              handler, arg, stream);
+  if (r.points == kSegments)
+    return ix->f64() ? rts_index_query_lines_d(ix, pred, static_cast<const rts_line_d*>(qp), n,
+                                               handler, arg, stream)
+                     : rts_index_query_lines(ix, pred, static_cast<const rts_line*>(qp), n, handler,
+                                             arg, stream);
+  return ix->f64() ? rts_index_query_envelopes_d(ix, pred, static_cast<const rts_envelope_d*>(qp),
+                                                 n, handler, arg, stream)
+                   : rts_index_query_envelopes(ix, pred, static_cast<const rts_envelope*>(qp), n,
+                                               handler, arg, stream);
+}
 
 struct Slot {
   hipStream_t stream = nullptr;
@@ -122,6 +160,10 @@ struct Slot {
   unsigned long long* d_hash = nullptr;
   rts_pair* d_arena = nullptr;
   uint32_t* d_sort = nullptr;  // Morton ordering of the wave tiles: 4 key/value arrays + scratch
+  // Extension requests: [0, 24) rts_dev_queue over the arena, [32, 40) a 0,
+  // [64, 64 + 4 kWaveTicketWords) wave tickets.
+  char* d_ext = nullptr;
+  std::vector<BatchExt> ext;
   std::vector<Chunk> chunks;
   bool busy = false;
 };
@@ -138,7 +180,7 @@ struct Engine {
   uint64_t n_ups = 0;
   rts_request_result* res = nullptr;
   std::vector<ReqState> st;
-  Estimate est[4];
+  Estimate est[kKinds];
   Clock::time_point t0;
   size_t desc_bytes = 0;
   uint32_t epoch = 0;
@@ -317,8 +359,10 @@ int copy_out(Engine& E, const Chunk& c, const rts_pair* src, uint64_t cnt, hipSt
 // (predicate, query kind), else one query per wavefront for envelope
 // intersects / within and 64 per tile otherwise.
 uint32_t tile_flags(const Engine& E, const rts_request& r) {
-  uint32_t f = (r.points ? kReqPoints : 0u) | ((uint32_t)(r.predicate & 3) << kReqPredShift);
-  const rts_index::TunedPlan& tp = E.ix->tuned[r.predicate & 3][r.points ? 1 : 0];
+  if (is_ext(E.ix, r)) return kReqExt;
+  const bool pts = r.points == kPoints;
+  uint32_t f = (pts ? kReqPoints : 0u) | ((uint32_t)(r.predicate & 3) << kReqPredShift);
+  const rts_index::TunedPlan& tp = E.ix->tuned[r.predicate & 3][pts ? 1 : 0];
   const uint32_t trav = E.cfg.traversal;
   bool wave;
   if (trav == RTS_TRAVERSAL_WAVE || trav == RTS_TRAVERSAL_THREAD)
@@ -326,7 +370,7 @@ uint32_t tile_flags(const Engine& E, const rts_request& r) {
   else if (tp.valid)
     wave = tp.traversal == RTS_TRAVERSAL_WAVE;
   else
-    wave = !r.points && r.predicate != RTS_CONTAINS;
+    wave = !pts && r.predicate != RTS_CONTAINS;
   if (wave) f |= kReqWaveTiles;
   if (tp.valid && tp.leaf_size >= 1 && tp.leaf_size <= 64) {
     uint32_t lg = 0;
@@ -382,7 +426,10 @@ int record_slot_work(Engine& E, Slot& s, size_t up_bytes, size_t n_req) {
   sort.ss.error = E.ix->error;  // a look-back that gives up is reported after the run
   sort.scene = E.ix->scene;
   sort.n = wave_tiles >= E.cfg.sort_min_tiles ? wave_tiles : 0;
-  SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, sort, s.stream));
+  const BatchExtArgs ea{reinterpret_cast<uint32_t*>(s.d_ext + 64),
+                        reinterpret_cast<const uint64_t*>(s.d_ext + 32), s.d_ext};
+  SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, sort,
+                       s.ext.data(), s.ext.size(), ea, s.stream));
   SRV_HIP(hipMemcpyAsync(s.h_ranges, s.d_ranges, 2 * n_req * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, s.stream));
   if (E.cfg.hash_results) {
@@ -418,6 +465,15 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipMalloc(&s.d_hash, maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_arena, E.cfg.arena_pairs * sizeof(rts_pair)));
   SRV_HIP(hipMalloc(&s.d_sort, 4 * maxq * sizeof(uint32_t) + sort_scratch_bytes(maxq)));
+  SRV_HIP(hipMalloc(&s.d_ext, 64 + kWaveTicketWords * sizeof(uint32_t)));
+  {
+    // The arena as a result queue for the extension launches' write pass
+    // (which stores at exact offsets and never touches the counter).
+    const rts_dev_queue q{s.d_arena, E.cfg.arena_pairs,
+                          reinterpret_cast<unsigned long long*>(s.d_total + 1)};
+    SRV_HIP(hipMemset(s.d_ext, 0, 64));
+    SRV_HIP(hipMemcpy(s.d_ext, &q, sizeof(q), hipMemcpyHostToDevice));
+  }
   std::memset(s.h_desc, 0, E.desc_bytes);
   SRV_TRACE("slot buffers ready (arena %llu pairs)", (unsigned long long)E.cfg.arena_pairs);
   return RTS_OK;
@@ -432,7 +488,7 @@ void slot_free(Slot& s) {
                   (void*)s.d_log_off, (void*)s.d_log_n, (void*)s.d_counts,
                   (void*)s.d_offsets,
                   (void*)s.d_seg, (void*)s.d_total, (void*)s.d_ranges, (void*)s.d_hash,
-                  (void*)s.d_arena, (void*)s.d_sort})
+                  (void*)s.d_arena, (void*)s.d_sort, (void*)s.d_ext})
     if (p) (void)hipFree(p);
   if (s.done) (void)hipEventDestroy(s.done);
   if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -446,7 +502,7 @@ int probe_estimates(Engine& E, hipStream_t stream) {
   unsigned long long* h_cnt = nullptr;
   SRV_HIP(hipMalloc(&d_cnt, sizeof(unsigned long long)));
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_cnt), sizeof(unsigned long long)));
-  bool seen[4] = {false, false, false, false};
+  bool seen[kKinds] = {false, false, false, false, false};
   int rc = RTS_OK;
   for (uint64_t i = 0; i < E.n_reqs && !rc; i++) {
     const rts_request& r = E.reqs[i];
@@ -458,12 +514,7 @@ int probe_estimates(Engine& E, hipStream_t stream) {
       rc = RTS_ERR_HIP;
       break;
     }
-    rc = r.points ? rts_index_query_points(E.ix, RTS_CONTAINS,
-                                           static_cast<const rts_point*>(r.queries), m,
-                                           RTS_HANDLER_COUNT, d_cnt, stream)
-                  : rts_index_query_envelopes(E.ix, (rts_predicate)r.predicate,
-                                              static_cast<const rts_envelope*>(r.queries), m,
-                                              RTS_HANDLER_COUNT, d_cnt, stream);
+    rc = query_request(E.ix, r, r.queries, m, RTS_HANDLER_COUNT, d_cnt, stream);
     if (rc) break;
     if (hipMemcpyAsync(h_cnt, d_cnt, sizeof(*h_cnt), hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess) {
@@ -491,10 +542,19 @@ bool update_ready(Engine& E, uint64_t next_up, double t, bool idle) {
   return idle && E.partial == 0 && E.retry.empty();
 }
 
+// An update batch on the index: rts_update records, or rts_update_d ones on
+// a double index.
+int update_index(Engine& E, const rts_update_batch& u, hipStream_t stream) {
+  return E.ix->f64()
+             ? rts_index_update_d(E.ix, reinterpret_cast<const rts_update_d*>(u.updates), u.n,
+                                  stream)
+             : rts_index_update(E.ix, u.updates, u.n, stream);
+}
+
 int apply_update(Engine& E, const rts_update_batch& u, const std::vector<Slot>& slots) {
   for (const Slot& s : slots)
     if (s.busy) SRV_HIP(hipStreamWaitEvent(E.upd_stream, s.done, 0));
-  int rc = rts_index_update(E.ix, u.updates, u.n, E.upd_stream);
+  int rc = update_index(E, u, E.upd_stream);
   if (rc) return rc;
   {
     std::lock_guard<std::mutex> lk(E.ix->mu);
@@ -550,22 +610,46 @@ int launch_slot(Engine& E, Slot& s, std::vector<Chunk> chunks) {
   std::stable_sort(chunks.begin(), chunks.end(), [&](const Chunk& x, const Chunk& y) {
     return tile_class(tile_flags(E, E.reqs[x.req])) < tile_class(tile_flags(E, E.reqs[y.req]));
   });
-  *reinterpret_cast<TreeView*>(s.h_desc + kViewOff) = tree_view(E.ix);
+  const TreeView tv = tree_view(E.ix);
+  *reinterpret_cast<TreeView*>(s.h_desc + kViewOff) = tv;
   BatchHeader* h = reinterpret_cast<BatchHeader*>(s.h_desc + kHeaderOff);
   BatchRequest* br = reinterpret_cast<BatchRequest*>(s.h_desc + kReqsOff);
   uint64_t qb = 0;
   uint32_t tb = 0;
   uint32_t class_end[kClasses] = {0, 0, 0, 0, 0};
+  s.ext.clear();
   for (size_t i = 0; i < chunks.size(); i++) {
     const Chunk& c = chunks[i];
     const rts_request& r = E.reqs[c.req];
-    const size_t qs = r.points ? sizeof(rts_point) : sizeof(rts_envelope);
-    br[i].queries = static_cast<const char*>(r.queries) + c.q0 * qs;
+    br[i].queries = static_cast<const char*>(r.queries) + c.q0 * query_bytes_of(E.ix, r);
     br[i].n = (uint32_t)c.n;
     br[i].flags = tile_flags(E, r);
     br[i].qbase = qb;
     br[i].tbase = tb;
     br[i].qid_base = (uint32_t)c.q0;  // pairs report query indices within the request
+    if (br[i].flags & kReqExt) {
+      // Walked by its own launches (after every tile class: no tiles).
+      BatchExt x{};
+      x.ql.tree = tv;
+      x.ql.queries = br[i].queries;
+      x.ql.n = (uint32_t)c.n;
+      x.ql.qid_base = (uint32_t)c.q0;
+      x.ql.order = nullptr;
+      const bool seg = r.points == kSegments;
+      x.ql.pred = seg ? kPredLine : r.predicate;
+      x.ql.point_queries = r.points == kPoints ? 1 : 0;
+      x.ql.f64 = E.ix->f64() ? 1 : 0;
+      x.ql.sink = SinkArgs{RTS_HANDLER_COUNT, nullptr};
+      x.ql.traversal = RTS_TRAVERSAL_WAVE;
+      const rts_index::TunedPlan& tp = E.ix->tuned[r.predicate & 3][r.points == kPoints ? 1 : 0];
+      x.ql.tree.leaf_size =
+          (!seg && tp.valid && tp.leaf_size >= 1) ? tp.leaf_size : E.ix->cfg.leaf_size;
+      x.qbase = qb;
+      x.none = seg && r.predicate != RTS_INTERSECTS;  // no box lies inside a segment
+      s.ext.push_back(x);
+      qb += c.n;
+      continue;
+    }
     qb += c.n;
     tb += (br[i].flags & kReqWaveTiles) ? (uint32_t)c.n : (uint32_t)((c.n + kWave - 1) / kWave);
     class_end[tile_class(br[i].flags)] = tb;
@@ -688,7 +772,7 @@ int run_plain(Engine& E) {
     }
     const double t = E.now();
     while (!rc && update_ready(E, next_up, t, inflight.empty())) {
-      rc = rts_index_update(E.ix, E.ups[next_up].updates, E.ups[next_up].n, stream);
+      rc = update_index(E, E.ups[next_up], stream);
       next_up++;
       E.epoch++;
       E.applied_updates++;
@@ -713,13 +797,8 @@ int run_plain(Engine& E) {
     const uint32_t li = (uint32_t)(ring++ % kRing);
     SRV_HIP(hipMemcpyAsync(d_spans + 2 * li, d_counter, sizeof(uint64_t), hipMemcpyDeviceToDevice,
                            stream));
-    const size_t qs = r.points ? sizeof(rts_point) : sizeof(rts_envelope);
-    const void* qp = static_cast<const char*>(r.queries) + c.q0 * qs;
-    rc = r.points ? rts_index_query_points(E.ix, RTS_CONTAINS, static_cast<const rts_point*>(qp),
-                                           c.n, RTS_HANDLER_COLLECT, d_queue, stream)
-                  : rts_index_query_envelopes(E.ix, (rts_predicate)r.predicate,
-                                              static_cast<const rts_envelope*>(qp), c.n,
-                                              RTS_HANDLER_COLLECT, d_queue, stream);
+    const void* qp = static_cast<const char*>(r.queries) + c.q0 * query_bytes_of(E.ix, r);
+    rc = query_request(E.ix, r, qp, c.n, RTS_HANDLER_COLLECT, d_queue, stream);
     if (rc) break;
     SRV_HIP(hipMemcpyAsync(d_spans + 2 * li + 1, d_counter, sizeof(uint64_t),
                            hipMemcpyDeviceToDevice, stream));
@@ -784,8 +863,9 @@ int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request*
     set_error("rts_serve_run: null argument");
     return RTS_ERR_INVALID;
   }
-  if (ix->cfg.coord != RTS_COORD_F32) {
-    set_error("rts_serve_run: the served path takes float indexes (RTS_COORD_F32)");
+  if (ix->cfg.coord == RTS_COORD_F64_AS_F32) {
+    set_error("rts_serve_run: the served path takes float or exact double indexes "
+              "(RTS_COORD_F32, RTS_COORD_F64), not RTS_COORD_F64_AS_F32");
     return RTS_ERR_INVALID;
   }
   Engine E;
@@ -808,8 +888,9 @@ int rts_serve_run(rts_index* ix, const rts_serve_config* cfg, const rts_request*
   }
   for (uint64_t i = 0; i < n_reqs; i++) {
     const rts_request& r = reqs[i];
-    if ((r.n && !r.queries) || r.predicate < 0 || r.predicate > 2 ||
-        (r.points && r.predicate != RTS_CONTAINS) || r.n >= (1ull << 31) ||
+    if ((r.n && !r.queries) || r.predicate < 0 || r.predicate > 2 || r.points < kEnvelopes ||
+        r.points > kSegments || (r.points == kPoints && r.predicate != RTS_CONTAINS) ||
+        (r.points == kSegments && r.predicate == RTS_WITHIN) || r.n >= (1ull << 31) ||
         (i && r.arrival_s < reqs[i - 1].arrival_s)) {
       set_error("rts_serve_run: bad request " + std::to_string(i));
       return RTS_ERR_INVALID;

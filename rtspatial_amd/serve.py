@@ -1,7 +1,8 @@
 """Served / batched execution of a request trace (rts_serve_run).
 
 A request is (arrival time, predicate, queries tensor [n, 2] points or [n, 4]
-envelopes, optional output capacity). `run()` returns one dict per request
+envelopes -- or segments with `segments=True` -- float32, or float64 on an
+exact double index, optional output capacity). `run()` returns one dict per request
 (admission and completion times, result count, epoch, re-launches, and the
 pairs as an int32 [k, 2] (geom_id, query index) tensor when collected) plus
 run statistics.
@@ -21,6 +22,7 @@ class Request:
     predicate: int = Predicate.kIntersects
     arrival_s: float = 0.0
     collect: int = 0  # capacity of the output buffer (0: results are dropped)
+    segments: bool = False  # [n, 4] rows are line segments (x1, y1, x2, y2), not envelopes
 
 
 @dataclasses.dataclass
@@ -49,14 +51,17 @@ def run(index, requests, updates=(), mode="batched", **cfg):
     outs = []
     for i, r in enumerate(requests):
         q = r.queries
-        if q.dtype != torch.float32 or q.dim() != 2 or q.shape[1] not in (2, 4) or not q.is_contiguous():
-            raise ValueError("request queries must be contiguous float32 [n, 2] or [n, 4]")
+        want = getattr(index, "coord_t", torch.float32)
+        if (q.dtype != want or q.dim() != 2 or q.shape[1] not in (2, 4) or not q.is_contiguous() or
+                (r.segments and q.shape[1] != 4)):
+            raise ValueError(f"request queries must be contiguous {want} [n, 2] or [n, 4] "
+                             "(the index's coordinate type)")
         out = torch.empty((max(r.collect, 1), 2), dtype=torch.int32, device=q.device) if r.collect else None
         outs.append(out)
         reqs[i].arrival_s = float(r.arrival_s)
         reqs[i].queries = q.data_ptr() if q.numel() else None
         reqs[i].n = q.shape[0]
-        reqs[i].points = 1 if q.shape[1] == 2 else 0
+        reqs[i].points = 2 if r.segments else (1 if q.shape[1] == 2 else 0)
         reqs[i].predicate = int(r.predicate)
         reqs[i].out = out.data_ptr() if out is not None else None
         reqs[i].out_capacity = r.collect
@@ -64,7 +69,7 @@ def run(index, requests, updates=(), mode="batched", **cfg):
     ups = (N.rts_update_batch * max(nu, 1))()
     keep = []
     for i, u in enumerate(updates):
-        packed = pack_updates(u.ids, u.boxes)
+        packed = pack_updates(u.ids, u.boxes, getattr(index, "coord_t", torch.float32))
         keep.append(packed)
         ups[i].arrival_s = float(u.arrival_s)
         ups[i].updates = packed.data_ptr()

@@ -252,3 +252,111 @@ def test_graph_replay_is_refused(scene):
     reqs = [S.Request(queries=dev(np.float32([[0.5, 0.5, 0.6, 0.6]])), collect=8)]
     with pytest.raises(RuntimeError, match="use_graphs"):
         S.run(ix, reqs, mode="batched", use_graphs=1)
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_segment_requests_beside_envelopes_and_points(oracle, scene, mode):
+    # Segment requests (one count and one write launch each inside their
+    # batch) mixed with tiled envelope and point requests; a segment request
+    # under RTS_CONTAINS matches nothing, as the plain call.
+    rng, means, boxes = scene
+    log(f"serve segments mode={mode}")
+    r = np.random.default_rng(11)
+    host = []
+    for i in range(24):
+        n = int(r.integers(1, 400))
+        kind = i % 4
+        if kind == 0:
+            q, pred, seg = W.mixture_boxes(r, means, n, np.log(4e-3)), P.kIntersects, True
+        elif kind == 1:
+            q, pred, seg = W.mixture_boxes(r, means, n, np.log(4e-3)), P.kWithin, False
+        elif kind == 2:
+            q, pred, seg = W.mixture_boxes(r, means, n, np.log(4e-3))[:, :2].copy(), P.kContains, False
+        else:
+            q, pred, seg = W.mixture_boxes(r, means, n, np.log(4e-3)), P.kIntersects, False
+        host.append((q, pred, seg))
+    host.append((W.mixture_boxes(r, means, 50, np.log(4e-3)), P.kContains, True))
+    want = [oracle.bf_pairs(int(p), boxes, q, lines=s) for q, p, s in host]
+    reqs = [S.Request(queries=dev(q), predicate=p, arrival_s=0.001 * i, collect=w.shape[0] + 8,
+                      segments=s) for i, ((q, p, s), w) in enumerate(zip(host, want))]
+    ix = make_index(boxes)
+    res, st = S.run(ix, reqs, mode=mode, slots=3, hash_results=1)
+    assert st["requests"] == len(host)
+    for i, (rr, w) in enumerate(zip(res, want)):
+        assert rr["count"] == w.shape[0], i
+        assert np.array_equal(canon(rr["pairs"]), w), i
+    assert want[-1].shape[0] == 0 and any(w.shape[0] for w in want[:-1])
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_double_index_requests(oracle, mode):
+    # An exact double index: every request (envelopes under each predicate,
+    # points, segments) is walked in double, with coordinates within 1e-13 of
+    # data coordinates where the float bounds cannot decide.
+    log(f"serve double index mode={mode}")
+    rng = np.random.default_rng(641)
+    lo = rng.random((40_000, 2))
+    boxes = np.concatenate([lo, lo + rng.random((40_000, 2)) * 0.01], axis=1)
+
+    def near(n):
+        src = boxes[rng.integers(0, boxes.shape[0], n)]
+        q = src + rng.choice([-1e-13, 0.0, 1e-13], size=(n, 4))
+        q[:, 2:] = np.maximum(q[:, 2:], q[:, :2]) + rng.random((n, 2)) * 0.02
+        return q
+
+    host = [(near(300), P.kIntersects, False), (near(200), P.kWithin, False),
+            (near(100), P.kContains, False),
+            (boxes[rng.integers(0, boxes.shape[0], 300), :2] + rng.choice([-1e-13, 0.0, 1e-13], size=(300, 2)),
+             P.kContains, False),
+            (near(150), P.kIntersects, True)]
+    want = [oracle.bf_pairs64(int(p), boxes, q, lines=s) for q, p, s in host]
+    reqs = [S.Request(queries=dev(np.ascontiguousarray(q)), predicate=p, arrival_s=0.0,
+                      collect=w.shape[0] + 8, segments=s) for (q, p, s), w in zip(host, want)]
+    ix = make_index(boxes)
+    res, st = S.run(ix, reqs, mode=mode, slots=2)
+    for i, (rr, w) in enumerate(zip(res, want)):
+        assert rr["count"] == w.shape[0], i
+        assert np.array_equal(canon(rr["pairs"]), w), i
+    assert all(w.shape[0] for w in want)
+
+
+def test_double_as_float_index_is_refused():
+    ix = make_index(np.array([[0.0, 0.0, 1.0, 1.0]]), coord=N.RTS_COORD_F64_AS_F32)
+    with pytest.raises(N.RtsError, match="RTS_COORD_F64_AS_F32"):
+        S.run(ix, [S.Request(queries=dev(np.array([[0.5, 0.5]])), predicate=P.kContains)])
+
+
+def test_double_index_updates_each_request_sees_its_epoch(oracle):
+    # Update batches on a double index (rts_update_d records) between
+    # requests: each request equals the exact double join on its epoch.
+    log("serve double index with updates")
+    rng = np.random.default_rng(642)
+    lo = rng.random((20_000, 2))
+    b0 = np.concatenate([lo, lo + rng.random((20_000, 2)) * 0.01], axis=1)
+    states, ups = [b0], []
+    cur = b0
+    for k in range(2):
+        ids = rng.choice(cur.shape[0], 2000, replace=False).astype(np.int64)
+        nb = cur[ids] + rng.uniform(-0.005, 0.005, size=(2000, 1)) * np.array([1, 0, 1, 0]) \
+            + rng.uniform(-0.005, 0.005, size=(2000, 1)) * np.array([0, 1, 0, 1])
+        cur = cur.copy()
+        cur[ids] = nb
+        states.append(cur)
+        ups.append(S.UpdateBatch(ids=dev(ids), boxes=dev(nb), arrival_s=0.02 * (k + 1)))
+    host = []
+    for i in range(12):
+        q = rng.random((200, 4))
+        q[:, 2:] = q[:, :2] + rng.random((200, 2)) * 0.02
+        host.append((q, (P.kIntersects, P.kWithin)[i % 2]))
+    reqs = [S.Request(queries=dev(q), predicate=p, arrival_s=0.006 * i,
+                      collect=max(oracle.bf_count64(int(p), s, q) for s in states) + 8)
+            for i, (q, p) in enumerate(host)]
+    ix = make_index(b0)
+    res, st = S.run(ix, reqs, ups, mode="batched", slots=2)
+    assert st["updates"] == 2
+    epochs = set()
+    for i, ((q, p), rr) in enumerate(zip(host, res)):
+        e = rr["epoch"]
+        epochs.add(e)
+        assert np.array_equal(canon(rr["pairs"]), oracle.bf_pairs64(int(p), states[e], q)), (i, e)
+    assert len(epochs) >= 2
