@@ -331,6 +331,9 @@ __device__ __forceinline__ void load_q(const void* queries, bool, uint32_t i, Qu
 // leaves per step, and writes the matches of this query contiguously from
 // slot `obase`.
 constexpr int kWaveStack = 256;     // LDS stack entries per wave
+// Wave stack entries are internal node indices (below 2^30); kTopRef + i
+// stands for record i of the tree's top.
+constexpr uint32_t kTopRef = 1u << 30;
 constexpr int kWaveRanges = 256;    // leaf ranges per wave before a scan
 // Scan once this many leaves are queued. Deferring the scan past one step's
 // 512 leaves leaves fewer part-filled final steps: on the 10M x 1M
@@ -399,6 +402,9 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
                                Emit&& emit, WaveLds& L, RangeLog* log = nullptr) {
   constexpr bool kCounting = !WRITE;
   constexpr bool kRecord = LOG && !WRITE;
+  // Segment walks start at the root: the top's first step would cost their
+  // larger walk registers an occupancy step.
+  constexpr bool kUseTop = PRED != kPredLine;
   const uint32_t lane = lane_id();
   uint32_t* stack = L.stack;
   uint32_t* rfirst = L.rfirst;
@@ -435,6 +441,10 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
           if (lane == 0 && log->cap > 0) log_store(log->entries, f, t.n);
           nlog = 1;
         }
+      } else if (kUseTop && t.top) {
+        // The first step expands the tree's top: one top record per lane.
+        stack[lane] = kTopRef + lane;
+        sp = kTopPairs;
       } else {
         if (lane == 0) stack[0] = 0;
         sp = 1;
@@ -462,13 +472,13 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
       uint32_t push0 = 0, push1 = 0, np = 0;
       uint32_t rf0 = 0, rn0 = 0, rf1 = 0, rn1 = 0, nr = 0;
       uint32_t lf0 = 0, ln0 = 0, lf1 = 0, ln1 = 0, nl = 0;  // kRecord: this lane's log entries
-      if (act && e < t.n - 1) {
-        const NodeRec* nd = t.nodes + e;
+      if (act && (e < t.n - 1 || (kUseTop && e >= kTopRef))) {
+        const NodeRec* nd = (kUseTop && e >= kTopRef) ? t.top + (e - kTopRef) : t.nodes + e;
         const uint4 mt = nd->meta;
         const uint4 lv = nd->live;
         const float4 cb[2] = {nd->box[0], nd->box[1]};
         const uint32_t live[2] = {lv.x, lv.y};
-        const uint32_t gamma = mt.x & ~kLeafFlag;
+        const uint32_t gamma = lv.z;  // the left range's last leaf
         const uint32_t ref[2] = {mt.x, mt.y};
         const uint32_t first[2] = {mt.z, gamma + 1};
         const uint32_t cnt[2] = {gamma - mt.z + 1, mt.w - gamma};

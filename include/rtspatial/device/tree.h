@@ -25,7 +25,8 @@ constexpr int kStackDepth = 72;           // >= max Karras depth (64) + 2
 //   box[0], box[1]  bounds of the left / right child
 //   meta            {left ref, right ref, first leaf, last leaf}
 //                   ref = internal index, or sorted leaf position | kLeafFlag
-//   live            {live leaves below the left child, ... the right child}
+//   live            {live leaves below the left child, ... the right child,
+//                    split (the left child's last leaf), 0}
 // parent[j] = (parent << 1) | side, for internal j in [0, n-1) and leaf k at
 // [n-1+k]. Leaves are the boxes in Morton order: leaf_boxes[k], id leaf_ids[k].
 struct __align__(64) NodeRec {
@@ -41,6 +42,15 @@ struct Box64 {
 };
 static_assert(sizeof(Box64) == sizeof(rts_envelope_d), "Box64 is rts_envelope_d");
 
+// The tree's top: a cut of the tree into 2 * kTopPairs disjoint subtrees
+// (or leaves), in leaf order, two to a record in the node record's format
+// (two adjacent leaf ranges: meta {ref 0, ref 1, first leaf of 0, last leaf
+// of 1}, live {live 0, live 1, last leaf of 0, 0}), so that a wavefront's
+// first expansion step tests all of them at once instead of descending the
+// top levels one dependent step at a time. Unused entries hold the empty box
+// (no test passes).
+constexpr int kTopPairs = 64;
+
 struct TreeView {
   const float4* leaf_boxes;
   const uint32_t* leaf_ids;
@@ -51,6 +61,7 @@ struct TreeView {
   uint32_t leaf_size;         // ranges with <= leaf_size leaves are scanned directly
   uint32_t* error;            // device word: bit 1 refit walk, bit 2 traversal bound
   const Box64* boxes64;       // double index: the boxes by id (else null)
+  const NodeRec* top = nullptr;  // kTopPairs records of the tree's top, or null
 };
 
 // Internal predicate id of line-segment queries (closed segment meets closed

@@ -159,7 +159,10 @@ __global__ void __launch_bounds__(kThreads) k_karras(const uint32_t* __restrict_
     const uint32_t rslot = (m.y & kLeafFlag) ? (uint32_t)(N - 1) + (m.y & ~kLeafFlag) : m.y;
     // kSpanBit: node i's leaf range crosses a refit chunk boundary.
     const uint32_t span = (m.z >> chunk_log2) != (m.w >> chunk_log2) ? kSpanBit : 0u;
-    if (span) nodes[i].meta = m;
+    if (span) {
+      nodes[i].meta = m;
+      nodes[i].live.z = (uint32_t)gamma;  // the split, read by the wave walks
+    }
     parent[lslot] = ((uint32_t)i << 1) | 0u | span;
     parent[rslot] = ((uint32_t)i << 1) | 1u | span;
   }
@@ -304,7 +307,9 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_chunk(
       // Second arrival: the union is whole (a wave's LDS operations complete
       // in order, and the sibling's count shows here). The split is the
       // arriving child's position (left side) or the one before it (right).
-      nodes[p].meta = node_meta(p, c0 + (old >> kEndShift), side ? self - 1 : self);
+      const uint32_t gamma = side ? self - 1 : self;
+      nodes[p].meta = node_meta(p, c0 + (old >> kEndShift), gamma);
+      nodes[p].live.z = gamma;  // the split, read by the wave walks
       live += (old & ((1u << kEndShift) - 1)) >> 1;
       b = s_box[j];
       if (p == 0) {
@@ -383,6 +388,102 @@ __global__ void __launch_bounds__(TOP ? kRefitTopThreads : kRefitSpanThreads) k_
       put_child(nodes + ((code & ~kSpanBit) >> 1), code & 1u, b, live);
     }
   }
+}
+
+// ------------------------------------------------------------- top ----
+// The tree's top (TreeView::top): starting from the root's two children,
+// every round splits internal entries into their two children, in place (so
+// leaf order is kept), the largest first when the 2 * kTopPairs entries
+// would overflow, until no entry can be split. One wavefront; each round is
+// one dependent load of node records.
+constexpr uint32_t kTopEntries = 2 * kTopPairs;
+__global__ void __launch_bounds__(kWave) k_tree_top(const NodeRec* __restrict__ nodes,
+                                                     NodeRec* __restrict__ top) {
+  __shared__ float4 s_box[2][kTopEntries];
+  __shared__ uint32_t s_ref[2][kTopEntries], s_first[2][kTopEntries], s_cnt[2][kTopEntries],
+      s_live[2][kTopEntries];
+  const uint32_t lane = threadIdx.x;
+  // The two children of internal node `node` as entries at..at+1 of buffer b.
+  auto children = [&](uint32_t node, int b, uint32_t at) {
+    const NodeRec nd = nodes[node];
+    const uint32_t gamma = nd.meta.x & ~kLeafFlag;
+    s_box[b][at] = nd.box[0];
+    s_box[b][at + 1] = nd.box[1];
+    s_ref[b][at] = nd.meta.x;
+    s_ref[b][at + 1] = nd.meta.y;
+    s_first[b][at] = nd.meta.z;
+    s_first[b][at + 1] = gamma + 1;
+    s_cnt[b][at] = gamma - nd.meta.z + 1;
+    s_cnt[b][at + 1] = nd.meta.w - gamma;
+    s_live[b][at] = nd.live.x;
+    s_live[b][at + 1] = nd.live.y;
+  };
+  if (lane == 0) children(0, 0, 0);
+  __syncthreads();
+  uint32_t m = 2, cur = 0;  // wave-uniform
+  // Each round splits at least one entry or stops, and m <= kTopEntries.
+  for (uint32_t round = 0; round < kTopEntries; round++) {
+    bool split[2];
+    for (int k = 0; k < 2; k++) {
+      const uint32_t i = lane + k * kWave;
+      split[k] = i < m && !(s_ref[cur][i] & kLeafFlag);
+    }
+    const uint32_t splittable =
+        (uint32_t)__popcll(ballot(split[0])) + (uint32_t)__popcll(ballot(split[1]));
+    const uint32_t room = kTopEntries - m;
+    if (splittable == 0 || room == 0) break;
+    if (splittable > room) {
+      // The `room` largest (ties: the earlier entry).
+      for (int k = 0; k < 2; k++) {
+        if (!split[k]) continue;
+        const uint32_t i = lane + k * kWave, c = s_cnt[cur][i];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < m; j++) {
+          if (s_ref[cur][j] & kLeafFlag) continue;
+          const uint32_t cj = s_cnt[cur][j];
+          rank += (cj > c || (cj == c && j < i)) ? 1u : 0u;
+        }
+        split[k] = rank < room;
+      }
+    }
+    // Entry i moves to i + (splits before i).
+    const uint32_t a0 = split[0] ? 1u : 0u, a1 = split[1] ? 1u : 0u;
+    const uint32_t inc0 = wave_inclusive_sum(a0);
+    const uint32_t tot0 = readlane_u32(inc0, kWave - 1);
+    const uint32_t inc1 = wave_inclusive_sum(a1);
+    const uint32_t before[2] = {inc0 - a0, tot0 + inc1 - a1};
+    const int nxt = cur ^ 1;
+    for (int k = 0; k < 2; k++) {
+      const uint32_t i = lane + k * kWave;
+      if (i >= m) continue;
+      const uint32_t at = i + before[k];
+      if (split[k]) {
+        children(s_ref[cur][i], nxt, at);
+      } else {
+        s_box[nxt][at] = s_box[cur][i];
+        s_ref[nxt][at] = s_ref[cur][i];
+        s_first[nxt][at] = s_first[cur][i];
+        s_cnt[nxt][at] = s_cnt[cur][i];
+        s_live[nxt][at] = s_live[cur][i];
+      }
+    }
+    m += tot0 + readlane_u32(inc1, kWave - 1);
+    cur = nxt;
+    __syncthreads();
+  }
+  // Record `lane`: entries 2 lane (left) and 2 lane + 1 (right); entries
+  // past m are empty (the right one then covers no leaves).
+  const uint32_t i0 = 2 * lane, i1 = i0 + 1;
+  const bool ok0 = i0 < m, ok1 = i1 < m;
+  const uint32_t last0 = ok0 ? s_first[cur][i0] + s_cnt[cur][i0] - 1 : 0u;
+  NodeRec r;
+  r.box[0] = ok0 ? s_box[cur][i0] : empty_box();
+  r.box[1] = ok1 ? s_box[cur][i1] : empty_box();
+  r.meta = make_uint4(ok0 ? s_ref[cur][i0] : kLeafFlag, ok1 ? s_ref[cur][i1] : kLeafFlag,
+                      ok0 ? s_first[cur][i0] : 0u,
+                      ok1 ? s_first[cur][i1] + s_cnt[cur][i1] - 1 : last0);
+  r.live = make_uint4(ok0 ? s_live[cur][i0] : 0u, ok1 ? s_live[cur][i1] : 0u, last0, 0u);
+  top[lane] = r;
 }
 
 __global__ void __launch_bounds__(kThreads) k_clear_flags(uint32_t* flags, uint32_t m) {
@@ -554,6 +655,8 @@ hipError_t launch_refit(const RefitArgs& a, hipStream_t stream) {
                        a.nodes, n, lg, chunks, a.root_box, a.root_live, a.flags, a.error,
                        a.boundary);
   }
+  if (a.top && n >= kTopMinLeaves)
+    hipLaunchKernelGGL(k_tree_top, dim3(1), dim3(kWave), 0, stream, a.nodes, a.top);
   return hipGetLastError();
 }
 

@@ -110,6 +110,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   dev_free(ix->nodes);
   dev_free(ix->parent);
   dev_free(ix->node_end);
+  dev_free(ix->top);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);
   dev_free(ix->partials);
@@ -121,6 +122,7 @@ int ensure_tree_capacity(rts_index* ix, uint64_t n) {
   RTS_HIP(dev_alloc(&ix->nodes, cap));
   RTS_HIP(dev_alloc(&ix->parent, 2 * cap));
   RTS_HIP(dev_alloc(&ix->node_end, cap));
+  RTS_HIP(dev_alloc(&ix->top, kTopPairs));
   RTS_HIP(dev_alloc(&ix->flags, cap + refit_boundary_words(cap)));
   RTS_HIP(dev_alloc(&ix->sort_scratch, sort_scratch_bytes(cap)));
   sort_scratch_carve(ix->sort_scratch, cap, &ix->ss);
@@ -178,6 +180,7 @@ RefitArgs refit_args(rts_index* ix, bool delta) {
   a.n = (uint32_t)(delta ? ix->n_delta : ix->n_main);
   a.parent = ix->parent + 2 * off;
   a.node_end = ix->node_end + off;
+  a.top = delta ? nullptr : ix->top;
   a.nodes = ix->nodes + off;
   a.root_box = delta ? ix->delta_root_box : ix->root_box;
   a.root_live = delta ? ix->delta_root_live : ix->root_live;
@@ -357,6 +360,8 @@ TreeView tree_view(const rts_index* ix) {
   t.leaf_size = ix->cfg.leaf_size;
   t.error = ix->error;
   t.boxes64 = ix->cfg.coord == RTS_COORD_F64 ? ix->boxes64 : nullptr;
+  // The top is rebuilt by every refit of the main tree (builds included).
+  t.top = ix->n_main >= kTopMinLeaves ? ix->top : nullptr;
   return t;
 }
 
@@ -369,6 +374,7 @@ TreeView delta_view(const rts_index* ix) {
   t.root_box = ix->delta_root_box;
   t.root_live = ix->delta_root_live;
   t.n = (uint32_t)ix->n_delta;
+  t.top = nullptr;  // the delta tree is walked from its root
   return t;
 }
 
@@ -476,6 +482,7 @@ int rts_index_destroy(rts_index* ix) {
   dev_free(ix->nodes);
   dev_free(ix->parent);
   dev_free(ix->node_end);
+  dev_free(ix->top);
   dev_free(ix->flags);
   dev_free(ix->sort_scratch);
   dev_free(ix->partials);
@@ -1172,6 +1179,7 @@ int rts_index_memory(rts_index* ix, rts_memory_usage* out) {
   out->aabbs = t * sizeof(float4);
   out->bvh = t ? (t * sizeof(uint32_t) * 4                      // leaf ids, keys, keys/vals alt
                   + t * sizeof(NodeRec) + 3 * t * sizeof(uint32_t) // nodes, parents, far ends
+                  + kTopPairs * sizeof(NodeRec)                     // the tree's top
                   + (t + refit_boundary_words(t)) * sizeof(uint32_t) // flags + boundary list
                   + sort_scratch_bytes(t) + ix->n_partials_cap * sizeof(float4))
                : 0;

@@ -41,6 +41,7 @@ struct rts_index {
   rts::NodeRec* nodes = nullptr;
   uint32_t* parent = nullptr;
   uint32_t* node_end = nullptr;  // far end of each internal node's leaf range (refit meta)
+  rts::NodeRec* top = nullptr;   // the main tree's top (kTopPairs records)
   uint32_t* flags = nullptr;
   char* sort_scratch = nullptr;
   rts::SortScratch ss{};

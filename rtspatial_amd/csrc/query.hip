@@ -231,8 +231,13 @@ struct StagedEmit {
 };
 
 // One wavefront per workgroup, so that a wave's LDS is released by itself.
+// Float box walks are held to 72 VGPRs (7 waves per SIMD), which the tree
+// top's first step would otherwise push just past; segment and double walks
+// would spill under that bound.
 template <int PRED, bool POINTS, int MODE, bool F64>
-__global__ void __launch_bounds__(kWave) k_query_wave(const QueryLaunch q, const WaveArgs a) {
+__global__ void __launch_bounds__(kWave)
+__attribute__((amdgpu_waves_per_eu((PRED == kPredLine || F64) ? 1 : 7)))
+k_query_wave(const QueryLaunch q, const WaveArgs a) {
   __shared__ WaveLds s_lds;
   __shared__ rts_pair s_stage[MODE == kWaveStaged ? kStageWave : 1];
   const TreeView t = q.tree;

@@ -72,6 +72,7 @@ struct RefitArgs {
   uint32_t* error;
   uint32_t* boundary;  // refit_boundary_words(n): per-chunk boundary slots, then the top list
   const uint32_t* node_end;  // the hierarchy kernel's far ends (records' meta)
+  NodeRec* top = nullptr;    // main tree of kTopMinLeaves or more: its top, rebuilt after the refit
   // Build refits also gather the leaves: leaf k's box is boxes[id] with
   // id = ids[k] + off (ids counted from off, rewritten to the ids themselves
   // when off > 0), stored to leaf_boxes[k], and leaf_of_id[id] = off + k. off
@@ -106,6 +107,8 @@ __host__ __device__ inline uint64_t refit_boundary_words(uint64_t n) {
   return (n / 2048 + 1) * kRefitChunkStride + refit_top_words(n);
 }
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
+// Trees of at least this many leaves get a top (TreeView::top) for the wave walks.
+constexpr uint32_t kTopMinLeaves = 4096;
 hipError_t launch_tree_cost(const NodeRec* nodes, const float4* root_box, uint32_t n,
                             double* out, hipStream_t stream);
 hipError_t launch_delete(const uint64_t* ids, uint64_t n_ids, float4* boxes, uint64_t n_boxes,
