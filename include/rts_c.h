@@ -442,8 +442,8 @@ int rts_pairs_checksum(const rts_pair* pairs, uint64_t n, uint64_t n_queries, ui
  *   RTS_SERVE_BATCHED: continuous batching. Up to `slots` batches are in flight
  *     on separate streams; whenever a slot is free, every arrived request
  *     (split into chunks when large) is admitted while its predicted result
- *     bytes fit the slot's arena, and the batch runs as one persistent
- *     work-stealing launch sequence (count, scan, write). Chunks whose results
+ *     slots fit the slot's arena, and the batch runs as one persistent
+ *     work-stealing launch sequence (count, scan, write). Chunks whose slots
  *     overflow the arena are re-launched.
  *     Segment requests, and every request on a double index, are walked by
  *     one count and one write launch each inside their batch (the plain
@@ -502,7 +502,10 @@ typedef struct rts_serve_config {
   uint64_t max_batch_queries;  /* queries per batch (4M). Each slot also holds a range log of
                                   192 8-byte entries per query (6.4 GB per slot at 4M) */
   uint64_t chunk_queries;      /* largest chunk of one request (256K) */
-  uint64_t arena_pairs;        /* result arena per slot (or of the plain path), pairs */
+  uint64_t arena_pairs;        /* result arena per slot (or of the plain path), pairs. A
+                                  batched envelope query takes slots for a bound on its
+                                  results (its leaf ranges that need tests counted whole,
+                                  about 1.2x its results on the served mix) */
   uint32_t hash_results;       /* compute each request's pair hash (verification runs) */
   uint32_t traversal;          /* rts_traversal for batched tiles: AUTO picks by predicate */
   uint32_t head_share_pct;     /* while later requests wait, the largest share of a batch's

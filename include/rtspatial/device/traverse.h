@@ -397,11 +397,18 @@ struct RangeLog {
   uint32_t n_in = kLogOverflow;  // writing walk: entries to scan (kLogOverflow: walk the tree)
 };
 
-template <int PRED, bool POINTS, bool WRITE, bool LOG = false, typename Q, typename Emit>
+// A counting walk with BOUND counts every leaf range that needs tests as a
+// whole instead of testing its leaves: the result is an upper bound on the
+// query's results, found without loading a leaf box (about half the
+// counting walk's time on range queries). It logs the same ranges, so a
+// writing walk that replays them emits the exact results.
+template <int PRED, bool POINTS, bool WRITE, bool LOG = false, bool BOUND = false, typename Q,
+          typename Emit>
 __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qid,
                                Emit&& emit, WaveLds& L, RangeLog* log = nullptr) {
   constexpr bool kCounting = !WRITE;
   constexpr bool kRecord = LOG && !WRITE;
+  constexpr bool kBound = BOUND && !WRITE;
   // Segment walks start at the root: the top's first step would cost their
   // larger walk registers an occupancy step.
   constexpr bool kUseTop = PRED != kPredLine;
@@ -430,13 +437,17 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
         }
       } else if (t.n <= t.leaf_size || t.n == 1 || inside) {
         const uint32_t f = (inside && t.root_live[0] == t.n) ? kAllMatch : 0u;
-        if (lane == 0) {
-          rfirst[0] = f;
-          rpre[0] = 0;
-          rpre[1] = t.n;
+        if (kBound) {
+          if (lane == 0) inside_live = t.n;
+        } else {
+          if (lane == 0) {
+            rfirst[0] = f;
+            rpre[0] = 0;
+            rpre[1] = t.n;
+          }
+          m = 1;
+          total = t.n;
         }
-        m = 1;
-        total = t.n;
         if (kRecord) {
           if (lane == 0 && log->cap > 0) log_store(log->entries, f, t.n);
           nlog = 1;
@@ -502,14 +513,18 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
           } else if ((ref[side] & kLeafFlag) || cnt[side] <= t.leaf_size || degrade || inside) {
             uint32_t f = first[side];
             if (inside && live[side] == cnt[side]) f |= kAllMatch;
-            if (nr == 0) {
-              rf0 = f;
-              rn0 = cnt[side];
+            if (kBound) {
+              inside_live += cnt[side];
             } else {
-              rf1 = f;
-              rn1 = cnt[side];
+              if (nr == 0) {
+                rf0 = f;
+                rn0 = cnt[side];
+              } else {
+                rf1 = f;
+                rn1 = cnt[side];
+              }
+              nr++;
             }
-            nr++;
             if (kRecord) {
               if (nl == 0) {
                 lf0 = f;
@@ -577,6 +592,8 @@ __device__ uint64_t wave_query(const TreeView& t, const Q& qb, const uint32_t qi
       total = carry;
       lpos += c;
       lds_fence();
+    } else if constexpr (kBound) {
+      break;  // no ranges are queued: the loop ends with the stack
     } else {
       // ---- scan the queued leaf ranges, kScanStep leaves per step. Lane L
       // takes the leaves at step positions k*64 + L (k < kPerLane), so every

@@ -74,15 +74,23 @@ struct BatchHeader {
   uint32_t pad;
 };
 
+// Counts of a batch. The count pass gives each query a bound on its results:
+// exact for lane tiles and extension requests, an upper bound for wave tiles
+// (their count walks do not test the leaves of partly covered ranges). The
+// scan of the bounds places each query's pairs in the arena, and the write
+// pass, which tests those leaves, records each query's exact count. Query q's
+// pairs are then arena[offsets[q], offsets[q] + exact[q]).
 struct BatchArgs {
   const BatchHeader* header;     // device
   const BatchRequest* requests;  // device, sorted by tbase
   uint32_t* tile_counters;       // kTileCounterWords: per-pass, per-class, per-XCD counters
-  uint32_t* counts;              // per global query
+  uint32_t* counts;              // per global query: result bound
+  uint32_t* exact;               // per global query: results
   uint64_t* offsets;             // per global query: exclusive scan of counts
+  uint64_t* exact_offsets;       // per global query: exclusive scan of exact (with collect)
   uint64_t* seg_sums;            // scan scratch (batch_seg_sums_words())
-  uint64_t* total;               // [0] results in the batch
-  uint64_t* ranges;              // per request: {first arena slot, results}
+  uint64_t* total;               // [0] sum of the bounds, [2] of the exact counts
+  uint64_t* ranges;              // per request: {first arena slot, slots (bounds), results}
   rts_pair* arena;               // results
   uint64_t arena_cap;            // pairs the arena holds
   const uint32_t* order;         // wave tiles in Morton order (tile -> tile), or null
@@ -123,14 +131,20 @@ struct BatchExtArgs {
   const uint64_t* zero;  // a device 0 (queue base of the arena)
   const void* arena_queue;  // device rts_dev_queue over the slot's arena
 };
+// With `collect`, exact_offsets is filled too (the pairs of requests that
+// copy them out are gathered by launch_copy_queries).
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a, const BatchSort& sort,
-                        const BatchExt* ext, size_t n_ext, const BatchExtArgs& ea,
+                        const BatchExt* ext, size_t n_ext, const BatchExtArgs& ea, bool collect,
                         hipStream_t stream);
-// hashes[c] = sum of pair_hash(query_id, geom_id) (checksum.h) over range c of
-// the arena (ranges as written by the batch: {first slot, count}, in order).
-hipError_t launch_hash_ranges(const rts_pair* arena, uint64_t cap, const uint64_t* ranges,
-                              const uint32_t* n_ranges, const uint64_t* total,
-                              unsigned long long* hashes, hipStream_t stream);
+// hashes[r] = sum of pair_hash(query_id, geom_id) (checksum.h) over the pairs
+// of request r of the batch (each query's arena[offsets[q], + exact[q])).
+// hashes must be zero.
+hipError_t launch_hash_queries(const BatchArgs& a, unsigned long long* hashes,
+                               hipStream_t stream);
+// Copies the pairs of the batch's queries [q0, q0 + n), in query order, to
+// out[0, take): query q's pairs go to out + exact_offsets[q] - exact_offsets[q0].
+hipError_t launch_copy_queries(const BatchArgs& a, uint64_t q0, uint64_t n, rts_pair* out,
+                               uint64_t take, hipStream_t stream);
 // Adds `add` to the query ids of the pairs in arena[span[0], span[1]) and, if
 // do_hash, adds their hash to *hash.
 hipError_t launch_rebase_hash_span(rts_pair* arena, uint64_t cap, const uint64_t* span,

@@ -12,10 +12,16 @@
 // contiguous eighth of the class's tiles, which keeps neighbouring queries on
 // one L2, and then steals from the other XCDs.
 //
-// Two passes: the count pass writes each query's result count, a scan turns
-// the counts into exclusive offsets into the slot's result arena, and the
-// write pass writes every pair at its offset. Every launch reads the batch's
-// sizes from device memory.
+// Two passes: the count pass writes a bound on each query's results (exact
+// for lane tiles; a wave tile's count walk counts the leaf ranges it would
+// have to test as a whole and logs them), a scan turns the bounds into
+// exclusive offsets into the slot's result arena, and the write pass tests
+// the logged ranges, writes each query's pairs at the start of its slots
+// and records its exact count. Per request, the batch reports its first
+// slot, its slots and its results. Every launch reads the batch's sizes from
+// device memory.
+#include <algorithm>
+
 #include "batch.h"
 #include "checksum.h"
 #include "morton.h"
@@ -92,12 +98,24 @@ __device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const 
   if (WRITE && skip_giants && wave && local < r.n &&
       a.counts[r.qbase + local] >= giant_min(kGiantBuckets - 1))
     return;
-  if (t.n == 0) {
+  // A request whose slots run past the arena is launched again: its pairs
+  // would be dropped, so its walks are skipped (a one-query request still
+  // runs, for its exact count).
+  if (WRITE && r.n > 1 && a.ranges[3 * ri] + a.ranges[3 * ri + 1] > a.arena_cap) {
     if (wave) {
-      if (local < r.n && lane_id() == 0) a.counts[r.qbase + local] = 0;
+      if (local < r.n && lane_id() == 0) a.exact[r.qbase + local] = 0;
     } else {
       const uint32_t q = local * kWave + lane_id();
-      if (q < r.n) a.counts[r.qbase + q] = 0;
+      if (q < r.n) a.exact[r.qbase + q] = 0;
+    }
+    return;
+  }
+  if (t.n == 0) {
+    if (wave) {
+      if (local < r.n && lane_id() == 0) a.counts[r.qbase + local] = a.exact[r.qbase + local] = 0;
+    } else {
+      const uint32_t q = local * kWave + lane_id();
+      if (q < r.n) a.counts[r.qbase + q] = a.exact[r.qbase + q] = 0;
     }
     return;
   }
@@ -111,11 +129,14 @@ __device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const 
     const uint64_t g = r.qbase + local;
     if (WRITE) {
       RangeLog lg = log_for_write(a.log, g);
-      wave_query<0, false, true, true>(t, qb, local + r.qid_base,
-                                       QueueEmit{a.arena, a.arena_cap, a.offsets[g]}, lds, &lg);
+      const uint64_t e = wave_query<0, false, true, true>(
+          t, qb, local + r.qid_base, QueueEmit{a.arena, a.arena_cap, a.offsets[g]}, lds, &lg);
+      if (lane_id() == 0) a.exact[g] = (uint32_t)e;
     } else {
+      // The bound: the write pass tests the logged ranges' leaves.
       RangeLog lg = log_for_count(lc);
-      const uint64_t c = wave_query<0, false, false, true>(t, qb, local, NoEmit{}, lds, &lg);
+      const uint64_t c =
+          wave_query<0, false, false, true, true>(t, qb, local, NoEmit{}, lds, &lg);
       log_commit(lc, a.log, g, lg);
       if (lane_id() == 0) {
         a.counts[g] = (uint32_t)c;
@@ -138,7 +159,7 @@ __device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const 
       lane_query<0, false, true>(t, qb, q + r.qid_base,
                                  QueueEmit{a.arena, a.arena_cap, a.offsets[g]});
     } else {
-      a.counts[g] = (uint32_t)lane_query<0, false, false>(t, qb, q, NoEmit{});
+      a.counts[g] = a.exact[g] = (uint32_t)lane_query<0, false, false>(t, qb, q, NoEmit{});
     }
   }
 }
@@ -150,8 +171,9 @@ __device__ __forceinline__ void run_tile(const TreeView& t0, TreeView& t, const 
 // workgroup: a workgroup's slot is freed as soon as its one wave runs out of
 // tiles (four-wave workgroups held theirs until the slowest wave ended). The
 // write pass first takes the giant tiles, largest bucket first.
-// Occupancy is pinned at 7 (count) and 6 (write) waves per SIMD, 72 and 80
-// VGPRs, which the range logs would otherwise push just past (76 and 81).
+// The write pass is pinned at 6 waves per SIMD (80 VGPRs), which the range
+// logs would otherwise push just past (81); the count pass, which tests no
+// leaves of wave tiles, needs 58 VGPRs and runs 8 (at least 7).
 template <bool WRITE>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WRITE ? 6 : 7)))
 k_batch(const TreeView* tv, const BatchArgs a) {
@@ -272,7 +294,7 @@ __global__ void __launch_bounds__(kThreads) k_seg_apply(const BatchArgs a) {
   }
 }
 
-// Per request: first arena slot and number of results.
+// Per request: first arena slot and slots (the sum of its bounds).
 __global__ void __launch_bounds__(kThreads) k_request_ranges(const BatchArgs a) {
   const BatchHeader h = *a.header;
   for (uint32_t r = blockIdx.x * kThreads + threadIdx.x; r < h.n_requests;
@@ -280,9 +302,37 @@ __global__ void __launch_bounds__(kThreads) k_request_ranges(const BatchArgs a) 
     const BatchRequest q = a.requests[r];
     const uint64_t start = q.n ? a.offsets[q.qbase] : 0;
     const uint64_t end = (q.qbase + q.n < h.n_queries) ? a.offsets[q.qbase + q.n] : a.total[0];
-    a.ranges[2 * r] = start;
-    a.ranges[2 * r + 1] = q.n ? end - start : 0;
+    a.ranges[3 * r] = start;
+    a.ranges[3 * r + 1] = q.n ? end - start : 0;
   }
+}
+
+// Per request, after the write pass: results (the sum of its exact counts).
+// One workgroup per request.
+__global__ void __launch_bounds__(kThreads) k_request_exact(const BatchArgs a) {
+  __shared__ uint64_t sh[kWaves];
+  const uint32_t nr = a.header->n_requests;
+  for (uint32_t r = blockIdx.x; r < nr; r += gridDim.x) {
+    const BatchRequest q = a.requests[r];
+    uint64_t s = 0;
+    for (uint32_t i = threadIdx.x; i < q.n; i += kThreads) s += a.exact[q.qbase + i];
+    s = wave_sum_u64(s);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint64_t all = 0;
+      for (int w = 0; w < kWaves; w++) all += sh[w];
+      a.ranges[3 * r + 2] = all;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_copy_u32(const uint32_t* __restrict__ src,
+                                                       uint32_t* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads)
+    dst[i] = src[i];
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -329,8 +379,15 @@ __global__ void __launch_bounds__(kThreads) k_batch_keys(const BatchArgs a, cons
   }
 }
 
+// Scan of `counts` into `offsets`, total in total[0] (fixed grid).
+void launch_seg_scan(const BatchArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_seg_sums, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_seg_apply, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
+}
+
 hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSort& sort,
-                        const BatchExt* ext, size_t n_ext, const BatchExtArgs& ea,
+                        const BatchExt* ext, size_t n_ext, const BatchExtArgs& ea, bool collect,
                         hipStream_t stream) {
   BatchArgs a = a_in;
   a.order = nullptr;
@@ -355,7 +412,7 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
                      kTileCounterWords);
   launch_batch_kernel<false>(t, a, stream);
   // Requests the batch kernel does not walk: one count launch each, into the
-  // same counts (so the scan below covers them).
+  // same counts (so the scan below covers them). Their counts are exact.
   for (size_t i = 0; i < n_ext; i++) {
     const BatchExt& x = ext[i];
     hipError_t e;
@@ -366,10 +423,13 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
       e = launch_query_wave(x.ql, wa, kPhaseCount, stream);
     }
     if (e != hipSuccess) return e;
+    if (x.ql.n) {
+      const uint32_t blocks = (uint32_t)std::min<uint64_t>((x.ql.n + kThreads - 1) / kThreads, 1024);
+      hipLaunchKernelGGL(k_copy_u32, dim3(blocks), dim3(kThreads), 0, stream, a.counts + x.qbase,
+                         a.exact + x.qbase, (uint64_t)x.ql.n);
+    }
   }
-  hipLaunchKernelGGL(k_seg_sums, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
-  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, stream, a);
-  hipLaunchKernelGGL(k_seg_apply, dim3(kScanBlocks), dim3(kThreads), 0, stream, a);
+  launch_seg_scan(a, stream);
   hipLaunchKernelGGL(k_request_ranges, dim3(16), dim3(kThreads), 0, stream, a);
   phase_end(ph, stream);
   ph = timed ? phase_begin(RTS_PHASE_BATCH_WRITE, stream) : PhaseToken{-1, nullptr};
@@ -384,6 +444,14 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
     const hipError_t e = launch_query_wave(ql, wa, kPhaseWrite, stream);
     if (e != hipSuccess) return e;
   }
+  hipLaunchKernelGGL(k_request_exact, dim3(1024), dim3(kThreads), 0, stream, a);
+  if (collect) {
+    BatchArgs x = a;
+    x.counts = a.exact;
+    x.offsets = a.exact_offsets;
+    x.total = a.total + 2;
+    launch_seg_scan(x, stream);
+  }
   phase_end(ph, stream);
   return hipGetLastError();
 }
@@ -392,60 +460,57 @@ size_t batch_seg_sums_words() { return kScanBlocks; }
 
 namespace {
 
-// hashes[c] = sum over the pairs of range c (ranges[2c] = first slot,
-// ranges[2c+1] = count; ranges tile [0, *total) in order) of
-// pair_hash(query_id, geom_id) (checksum.h). hashes must be zero.
-// A wavefront takes blocks of kHashBlock consecutive slots (lanes interleaved,
-// so loads coalesce). Each lane keeps a running sum for its current range and
-// adds it to hashes[] only when its range changes, so a request's pairs cost
-// about one atomic per lane per block rather than one per pair; where the
-// whole wave ends a block in one range, one wave-reduced atomic suffices.
-constexpr uint32_t kHashBlock = 64 * 256;
-
-__device__ __forceinline__ uint32_t range_of(const uint64_t* ranges, uint32_t nr, uint64_t i) {
-  uint32_t a = 0, b = nr - 1;
-  while (a < b) {
-    const uint32_t mid = (a + b + 1) >> 1;
-    if (ranges[2 * mid] <= i) a = mid; else b = mid - 1;
+// Request of global query q: the last request whose qbase <= q (requests are
+// laid out in query order).
+__device__ __forceinline__ uint32_t request_of_query(const BatchRequest* reqs, uint32_t nreq,
+                                                     uint64_t q) {
+  uint32_t lo = 0, hi = nreq - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (reqs[mid].qbase <= q) lo = mid; else hi = mid - 1;
   }
-  return a;
+  return lo;
 }
 
-__global__ void __launch_bounds__(kThreads) k_hash_ranges(const rts_pair* __restrict__ arena,
-                                                          uint64_t cap,
-                                                          const uint64_t* __restrict__ ranges,
-                                                          const uint32_t* __restrict__ n_ranges,
-                                                          const uint64_t* __restrict__ total,
-                                                          unsigned long long* __restrict__ hashes) {
-  const uint32_t nr = *n_ranges;
-  const uint64_t hi = *total < cap ? *total : cap;
-  if (nr == 0 || hi == 0) return;
+// hashes[r] += pair_hash over request r's pairs: one wavefront per query
+// (grid-stride over the batch's queries), its lanes over the query's pairs.
+__global__ void __launch_bounds__(kThreads) k_hash_queries(const BatchArgs a,
+                                                           unsigned long long* __restrict__ hashes) {
+  const BatchHeader h = *a.header;
+  if (h.n_requests == 0) return;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * kWaves;
-  const uint64_t w0 = (uint64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
-  for (uint64_t b0 = w0 * kHashBlock; b0 < hi; b0 += waves * kHashBlock) {
-    const uint64_t b1 = b0 + kHashBlock < hi ? b0 + kHashBlock : hi;
-    uint32_t a = range_of(ranges, nr, b0 + lane < b1 ? b0 + lane : b0);
-    uint64_t end = ranges[2 * a] + ranges[2 * a + 1];
-    uint64_t h = 0;
-    for (uint64_t i = b0 + lane; i < b1; i += kWave) {
-      if (i >= end) {
-        if (h) atomicAdd(&hashes[a], (unsigned long long)h);
-        h = 0;
-        a = range_of(ranges, nr, i);
-        end = ranges[2 * a] + ranges[2 * a + 1];
-      }
-      if (i >= end) continue;  // a slot in no range (ranges tile [0, total): not expected)
-      const rts_pair p = arena[i];
-      h += pair_hash(p.query_id, p.geom_id);
+  for (uint64_t q = (uint64_t)blockIdx.x * kWaves + threadIdx.x / kWave; q < h.n_queries;
+       q += waves) {
+    const uint64_t o = a.offsets[q];
+    const uint64_t end = o + a.exact[q] < a.arena_cap ? o + a.exact[q] : a.arena_cap;
+    uint64_t hs = 0;
+    for (uint64_t i = o + lane; i < end; i += kWave) {
+      const rts_pair p = a.arena[i];
+      hs += pair_hash(p.query_id, p.geom_id);
     }
-    const uint32_t a0 = readlane_u32(a, 0);
-    if (__all(a == a0)) {
-      h = wave_sum_u64(h);
-      if (lane == 0 && h) atomicAdd(&hashes[a0], (unsigned long long)h);
-    } else if (h) {
-      atomicAdd(&hashes[a], (unsigned long long)h);
-    }
+    hs = wave_sum_u64(hs);
+    if (lane == 0 && hs)
+      atomicAdd(&hashes[request_of_query(a.requests, h.n_requests, q)], (unsigned long long)hs);
+  }
+}
+
+// Pairs of queries [q0, q0 + n) to out[0, take), in query order: one
+// wavefront per query.
+__global__ void __launch_bounds__(kThreads) k_copy_queries(const BatchArgs a, uint64_t q0,
+                                                           uint64_t n, rts_pair* __restrict__ out,
+                                                           uint64_t take) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * kWaves;
+  const uint64_t base = a.exact_offsets[q0];
+  for (uint64_t j = (uint64_t)blockIdx.x * kWaves + threadIdx.x / kWave; j < n; j += waves) {
+    const uint64_t q = q0 + j;
+    const uint64_t d = a.exact_offsets[q] - base;
+    if (d >= take) continue;
+    const uint64_t e = a.exact[q];
+    const uint64_t m = d + e < take ? e : take - d;
+    const uint64_t o = a.offsets[q];
+    for (uint64_t i = lane; i < m && o + i < a.arena_cap; i += kWave) out[d + i] = a.arena[o + i];
   }
 }
 
@@ -472,11 +537,18 @@ __global__ void __launch_bounds__(kThreads) k_rebase_hash_span(rts_pair* __restr
 
 }  // namespace
 
-hipError_t launch_hash_ranges(const rts_pair* arena, uint64_t cap, const uint64_t* ranges,
-                              const uint32_t* n_ranges, const uint64_t* total,
-                              unsigned long long* hashes, hipStream_t stream) {
-  hipLaunchKernelGGL(k_hash_ranges, dim3(4096), dim3(kThreads), 0, stream, arena, cap, ranges,
-                     n_ranges, total, hashes);
+hipError_t launch_hash_queries(const BatchArgs& a, unsigned long long* hashes,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(k_hash_queries, dim3(4096), dim3(kThreads), 0, stream, a, hashes);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_queries(const BatchArgs& a, uint64_t q0, uint64_t n, rts_pair* out,
+                               uint64_t take, hipStream_t stream) {
+  if (n == 0 || take == 0) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((n + kWaves - 1) / kWaves, 4096);
+  hipLaunchKernelGGL(k_copy_queries, dim3((uint32_t)blocks), dim3(kThreads), 0, stream, a, q0, n,
+                     out, take);
   return hipGetLastError();
 }
 

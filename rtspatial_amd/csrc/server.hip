@@ -77,7 +77,8 @@ constexpr uint64_t kUnknown = ~0ull;
 struct Chunk {
   uint32_t req;
   uint64_t q0, n;
-  uint64_t known;  // exact result count when known (a re-launch), else kUnknown
+  uint64_t known;        // arena slots it takes, when known (a re-launch), else kUnknown
+  uint64_t known_exact;  // its results, when known
 };
 
 struct ReqState {
@@ -144,7 +145,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   char* h_desc = nullptr;                // pinned: [TreeView | BatchHeader | BatchRequest x max]
-  uint64_t* h_ranges = nullptr;          // pinned: 2 x max
+  uint64_t* h_ranges = nullptr;          // pinned: 3 x max
   unsigned long long* h_hash = nullptr;  // pinned: max
   char* d_desc = nullptr;
   uint32_t* d_counters = nullptr;
@@ -152,8 +153,10 @@ struct Slot {
   uint2* d_log = nullptr;        // range-log entries (kServeLogPerQuery per query)
   uint32_t* d_log_off = nullptr; // per query: first range-log entry
   uint32_t* d_log_n = nullptr;   // per query: range-log entries
-  uint32_t* d_counts = nullptr;
+  uint32_t* d_counts = nullptr;  // per query: result bound (BatchArgs)
+  uint32_t* d_exact = nullptr;   // per query: results
   uint64_t* d_offsets = nullptr;
+  uint64_t* d_exact_off = nullptr;
   uint64_t* d_seg = nullptr;
   uint64_t* d_total = nullptr;
   uint64_t* d_ranges = nullptr;
@@ -203,13 +206,16 @@ double predicted(const Engine& E, const Chunk& c) {
 }
 
 // A chunk's results are final.
-void finish_chunk(Engine& E, const Chunk& c, uint64_t count, uint64_t hash, double t) {
+// `slots`: the arena slots its results took (the batched path sizes each
+// query's slots by a bound on its results); the estimates predict slots.
+void finish_chunk(Engine& E, const Chunk& c, uint64_t count, uint64_t slots, uint64_t hash,
+                  double t) {
   ReqState& s = E.st[c.req];
   rts_request_result& R = E.res[c.req];
   R.count += count;
   R.hash += hash;
   E.last_progress = t;
-  E.est[kind_of(E.reqs[c.req])].observe(count, c.n);
+  E.est[kind_of(E.reqs[c.req])].observe(slots, c.n);
   if (--s.chunks_out == 0 && s.next_q >= E.reqs[c.req].n) {
     s.done = true;
     R.done_s = t;
@@ -229,19 +235,19 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
   while (!E.retry.empty() && out.size() < max_req) {
     Chunk c = E.retry.front();
     if (c.known != kUnknown && c.known > E.cfg.arena_pairs && c.n == 1) {
-      // One query with more results than an arena holds: report its exact
-      // count, mark the request truncated and drop its pairs.
+      // One query that needs more slots than an arena holds: report its
+      // exact count, mark the request truncated and drop its pairs.
       E.retry.pop_front();
       E.res[c.req].truncated = 1;
-      finish_chunk(E, c, c.known, 0, t);
+      finish_chunk(E, c, c.known_exact, c.known, 0, t);
       continue;
     }
     if (c.known != kUnknown && (double)c.known > (double)E.cfg.arena_pairs && c.n > 1) {
       // It can never fit one arena: split it; the halves' counts are unknown.
       E.retry.pop_front();
       const uint64_t h = c.n / 2;
-      E.retry.push_front(Chunk{c.req, c.q0 + h, c.n - h, kUnknown});
-      E.retry.push_front(Chunk{c.req, c.q0, h, kUnknown});
+      E.retry.push_front(Chunk{c.req, c.q0 + h, c.n - h, kUnknown, kUnknown});
+      E.retry.push_front(Chunk{c.req, c.q0, h, kUnknown, kUnknown});
       E.st[c.req].chunks_out++;
       continue;
     }
@@ -307,7 +313,7 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
         continue;
       }
     }
-    const Chunk c{ri, s.next_q, k, kUnknown};
+    const Chunk c{ri, s.next_q, k, kUnknown, kUnknown};
     out.push_back(c);
     if (!s.admitted) {
       s.admitted = true;
@@ -328,15 +334,18 @@ void admit(Engine& E, double t, uint32_t max_req, uint64_t max_q, double budget,
   }
 }
 
-void overflowed(Engine& E, const Chunk& c, uint64_t count) {
-  E.retry.push_back(Chunk{c.req, c.q0, c.n, count});
+void overflowed(Engine& E, const Chunk& c, uint64_t slots, uint64_t count) {
+  E.retry.push_back(Chunk{c.req, c.q0, c.n, slots, count});
   E.res[c.req].relaunches++;
   E.relaunches++;
-  E.est[kind_of(E.reqs[c.req])].observe(count, c.n);
+  E.est[kind_of(E.reqs[c.req])].observe(slots, c.n);
 }
 
-// Copies a finished chunk's pairs from the arena to the request's `out`.
-int copy_out(Engine& E, const Chunk& c, const rts_pair* src, uint64_t cnt, hipStream_t stream) {
+// Copies a finished chunk's cnt pairs to the request's `out`: with `batch`,
+// those of the batch's queries [qbase, qbase + c.n) (each query's pairs lie
+// at the start of its slots), else the contiguous src[0, cnt).
+int copy_out(Engine& E, const Chunk& c, const rts_pair* src, const BatchArgs* batch,
+             uint64_t qbase, uint64_t cnt, hipStream_t stream) {
   const rts_request& r = E.reqs[c.req];
   if (!r.out || !cnt) return RTS_OK;
   ReqState& st = E.st[c.req];
@@ -346,8 +355,11 @@ int copy_out(Engine& E, const Chunk& c, const rts_pair* src, uint64_t cnt, hipSt
     E.res[c.req].truncated = 1;
   }
   if (take) {
-    SRV_HIP(hipMemcpyAsync(r.out + st.filled, src, take * sizeof(rts_pair), hipMemcpyDeviceToDevice,
-                           stream));
+    if (batch)
+      SRV_HIP(launch_copy_queries(*batch, qbase, c.n, r.out + st.filled, take, stream));
+    else
+      SRV_HIP(hipMemcpyAsync(r.out + st.filled, src, take * sizeof(rts_pair),
+                             hipMemcpyDeviceToDevice, stream));
     SRV_HIP(hipStreamSynchronize(stream));
   }
   st.filled += take;
@@ -393,7 +405,9 @@ BatchArgs slot_args(const Engine& E, const Slot& s) {
   a.requests = reinterpret_cast<const BatchRequest*>(s.d_desc + kReqsOff);
   a.tile_counters = s.d_counters;
   a.counts = s.d_counts;
+  a.exact = s.d_exact;
   a.offsets = s.d_offsets;
+  a.exact_offsets = s.d_exact_off;
   a.seg_sums = s.d_seg;
   a.total = s.d_total;
   a.ranges = s.d_ranges;
@@ -428,15 +442,16 @@ int record_slot_work(Engine& E, Slot& s, size_t up_bytes, size_t n_req) {
   sort.n = wave_tiles >= E.cfg.sort_min_tiles ? wave_tiles : 0;
   const BatchExtArgs ea{reinterpret_cast<uint32_t*>(s.d_ext + 64),
                         reinterpret_cast<const uint64_t*>(s.d_ext + 32), s.d_ext};
+  // Requests that copy their pairs out need the compacted offsets.
+  bool collect = false;
+  for (const Chunk& c : s.chunks) collect = collect || E.reqs[c.req].out != nullptr;
   SRV_HIP(launch_batch(reinterpret_cast<const TreeView*>(s.d_desc + kViewOff), a, sort,
-                       s.ext.data(), s.ext.size(), ea, s.stream));
-  SRV_HIP(hipMemcpyAsync(s.h_ranges, s.d_ranges, 2 * n_req * sizeof(uint64_t),
+                       s.ext.data(), s.ext.size(), ea, collect, s.stream));
+  SRV_HIP(hipMemcpyAsync(s.h_ranges, s.d_ranges, 3 * n_req * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, s.stream));
   if (E.cfg.hash_results) {
     SRV_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(s.d_hash), 0u, 2 * n_req, s.stream));
-    SRV_HIP(launch_hash_ranges(s.d_arena, E.cfg.arena_pairs, s.d_ranges,
-                               reinterpret_cast<const uint32_t*>(s.d_desc + kHeaderOff),
-                               s.d_total, s.d_hash, s.stream));
+    SRV_HIP(launch_hash_queries(a, s.d_hash, s.stream));
     SRV_HIP(hipMemcpyAsync(s.h_hash, s.d_hash, n_req * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s.stream));
   }
@@ -449,7 +464,7 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
   SRV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_desc), E.desc_bytes));
-  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_ranges), 2 * maxr * sizeof(uint64_t)));
+  SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_ranges), 3 * maxr * sizeof(uint64_t)));
   SRV_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_hash), maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_desc, E.desc_bytes));
   SRV_HIP(hipMalloc(&s.d_counters, kTileCounterWords * sizeof(uint32_t)));
@@ -458,10 +473,13 @@ int slot_init(Engine& E, Slot& s) {
   SRV_HIP(hipMalloc(&s.d_log_off, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_log_n, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_counts, maxq * sizeof(uint32_t)));
+  SRV_HIP(hipMalloc(&s.d_exact, maxq * sizeof(uint32_t)));
   SRV_HIP(hipMalloc(&s.d_offsets, maxq * sizeof(uint64_t)));
+  SRV_HIP(hipMalloc(&s.d_exact_off, maxq * sizeof(uint64_t)));
   SRV_HIP(hipMalloc(&s.d_seg, batch_seg_sums_words() * sizeof(uint64_t)));
-  SRV_HIP(hipMalloc(&s.d_total, 2 * sizeof(uint64_t)));
-  SRV_HIP(hipMalloc(&s.d_ranges, 2 * maxr * sizeof(uint64_t)));
+  // [0] bound total, [1] the extension launches' queue counter, [2] exact total
+  SRV_HIP(hipMalloc(&s.d_total, 4 * sizeof(uint64_t)));
+  SRV_HIP(hipMalloc(&s.d_ranges, 3 * maxr * sizeof(uint64_t)));
   SRV_HIP(hipMalloc(&s.d_hash, maxr * sizeof(unsigned long long)));
   SRV_HIP(hipMalloc(&s.d_arena, E.cfg.arena_pairs * sizeof(rts_pair)));
   SRV_HIP(hipMalloc(&s.d_sort, 4 * maxq * sizeof(uint32_t) + sort_scratch_bytes(maxq)));
@@ -486,7 +504,7 @@ void slot_free(Slot& s) {
   if (s.h_hash) (void)hipHostFree(s.h_hash);
   for (void* p : {(void*)s.d_desc, (void*)s.d_counters, (void*)s.d_giants, (void*)s.d_log,
                   (void*)s.d_log_off, (void*)s.d_log_n, (void*)s.d_counts,
-                  (void*)s.d_offsets,
+                  (void*)s.d_exact, (void*)s.d_offsets, (void*)s.d_exact_off,
                   (void*)s.d_seg, (void*)s.d_total, (void*)s.d_ranges, (void*)s.d_hash,
                   (void*)s.d_arena, (void*)s.d_sort, (void*)s.d_ext})
     if (p) (void)hipFree(p);
@@ -591,14 +609,18 @@ int complete_slot(Engine& E, Slot& s) {
   SRV_TRACE("batch of %zu chunks complete at %.6f s", s.chunks.size(), tc);
   for (size_t i = 0; i < s.chunks.size(); i++) {
     const Chunk& c = s.chunks[i];
-    const uint64_t start = s.h_ranges[2 * i], cnt = s.h_ranges[2 * i + 1];
-    if (start + cnt > E.cfg.arena_pairs) {
-      overflowed(E, c, cnt);
+    // {first slot, slots (the queries' result bounds), results}
+    const uint64_t start = s.h_ranges[3 * i], slots = s.h_ranges[3 * i + 1];
+    const uint64_t cnt = s.h_ranges[3 * i + 2];
+    if (start + slots > E.cfg.arena_pairs) {
+      overflowed(E, c, slots, cnt);
       continue;
     }
-    const int rc = copy_out(E, c, s.d_arena + start, cnt, s.stream);
+    const BatchArgs a = slot_args(E, s);
+    const uint64_t qbase = reinterpret_cast<const BatchRequest*>(s.h_desc + kReqsOff)[i].qbase;
+    const int rc = copy_out(E, c, nullptr, &a, qbase, cnt, s.stream);
     if (rc) return rc;
-    finish_chunk(E, c, cnt, E.cfg.hash_results ? s.h_hash[i] : 0, tc);
+    finish_chunk(E, c, cnt, slots, E.cfg.hash_results ? s.h_hash[i] : 0, tc);
   }
   s.busy = false;
   return RTS_OK;
@@ -662,13 +684,13 @@ int launch_slot(Engine& E, Slot& s, std::vector<Chunk> chunks) {
   h->pad = 0;
   h->n_queries = qb;
   if (E.have_upd_event) SRV_HIP(hipStreamWaitEvent(s.stream, E.upd_event, 0));
+  s.chunks = chunks;
   {
     const int rc = record_slot_work(E, s, kReqsOff + chunks.size() * sizeof(BatchRequest),
                                     chunks.size());
     if (rc) return rc;
   }
   SRV_HIP(hipEventRecord(s.done, s.stream));
-  s.chunks = chunks;
   s.busy = true;
   E.batches++;
   SRV_TRACE("batch %llu launched: %zu chunks, %u tiles, %llu queries", (unsigned long long)E.batches,
@@ -759,11 +781,12 @@ int run_plain(Engine& E) {
       inflight.pop_front();
       const uint64_t before = h_spans[2 * pc.log], after = h_spans[2 * pc.log + 1];
       if (after > E.cfg.arena_pairs) {
-        overflowed(E, pc.c, after - before);
+        overflowed(E, pc.c, after - before, after - before);
         continue;
       }
-      rc = copy_out(E, pc.c, arena + before, after - before, stream);
-      finish_chunk(E, pc.c, after - before, E.cfg.hash_results ? h_hash[pc.log] : 0, E.now());
+      rc = copy_out(E, pc.c, arena + before, nullptr, 0, after - before, stream);
+      finish_chunk(E, pc.c, after - before, after - before,
+                   E.cfg.hash_results ? h_hash[pc.log] : 0, E.now());
     }
     if (rc) break;
     if (stalled(E, !inflight.empty())) {
