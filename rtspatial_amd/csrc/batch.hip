@@ -304,27 +304,67 @@ __global__ void __launch_bounds__(kThreads) k_request_ranges(const BatchArgs a) 
     const uint64_t end = (q.qbase + q.n < h.n_queries) ? a.offsets[q.qbase + q.n] : a.total[0];
     a.ranges[3 * r] = start;
     a.ranges[3 * r + 1] = q.n ? end - start : 0;
+    a.ranges[3 * r + 2] = 0;  // results: summed by k_request_exact
   }
 }
 
-// Per request, after the write pass: results (the sum of its exact counts).
-// One workgroup per request.
+// Request of global query q: the last request whose qbase <= q (requests are
+// laid out in query order).
+__device__ __forceinline__ uint32_t request_of_query(const BatchRequest* reqs, uint32_t nreq,
+                                                     uint64_t q) {
+  uint32_t lo = 0, hi = nreq - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (reqs[mid].qbase <= q) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// After the write pass: each request's results, the sum of its queries'
+// exact counts, added into ranges[3r + 2]. A wavefront sums a block of
+// kExactBlock counts: with one atomic when the block lies in one request
+// (requests are contiguous in query space), else lane by lane, each lane
+// over its own run of the block, one atomic per request it meets. (One
+// atomic per query from the write pass, on one address per request, made
+// that pass twice as slow.)
+constexpr uint32_t kExactPerLane = 16;
+constexpr uint32_t kExactBlock = kWave * kExactPerLane;
+
 __global__ void __launch_bounds__(kThreads) k_request_exact(const BatchArgs a) {
-  __shared__ uint64_t sh[kWaves];
-  const uint32_t nr = a.header->n_requests;
-  for (uint32_t r = blockIdx.x; r < nr; r += gridDim.x) {
-    const BatchRequest q = a.requests[r];
-    uint64_t s = 0;
-    for (uint32_t i = threadIdx.x; i < q.n; i += kThreads) s += a.exact[q.qbase + i];
-    s = wave_sum_u64(s);
-    if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint64_t all = 0;
-      for (int w = 0; w < kWaves; w++) all += sh[w];
-      a.ranges[3 * r + 2] = all;
+  const BatchHeader h = *a.header;
+  if (h.n_requests == 0) return;
+  const uint32_t lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * kWaves;
+  for (uint64_t w = (uint64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+       w * kExactBlock < h.n_queries; w += waves) {
+    const uint64_t b0 = w * kExactBlock;
+    const uint64_t b1 = b0 + kExactBlock < h.n_queries ? b0 + kExactBlock : h.n_queries;
+    const uint32_t r0 = request_of_query(a.requests, h.n_requests, b0);
+    const uint32_t r1 = request_of_query(a.requests, h.n_requests, b1 - 1);
+    if (r0 == r1) {
+      uint64_t s = 0;
+      for (uint64_t i = b0 + lane; i < b1; i += kWave) s += a.exact[i];
+      s = wave_sum_u64(s);
+      if (lane == 0 && s) atomicAdd(reinterpret_cast<unsigned long long*>(&a.ranges[3 * r0 + 2]), s);
+    } else {
+      uint64_t i = b0 + (uint64_t)lane * kExactPerLane;
+      const uint64_t e = i + kExactPerLane < b1 ? i + kExactPerLane : b1;
+      if (i < e) {
+        uint32_t r = request_of_query(a.requests, h.n_requests, i);
+        uint64_t r_end = r + 1 < h.n_requests ? a.requests[r + 1].qbase : h.n_queries;
+        uint64_t s = 0;
+        for (; i < e; i++) {
+          while (i >= r_end) {
+            if (s) atomicAdd(reinterpret_cast<unsigned long long*>(&a.ranges[3 * r + 2]), s);
+            s = 0;
+            r++;
+            r_end = r + 1 < h.n_requests ? a.requests[r + 1].qbase : h.n_queries;
+          }
+          s += a.exact[i];
+        }
+        if (s) atomicAdd(reinterpret_cast<unsigned long long*>(&a.ranges[3 * r + 2]), s);
+      }
     }
-    __syncthreads();
   }
 }
 
@@ -444,6 +484,9 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
     const hipError_t e = launch_query_wave(ql, wa, kPhaseWrite, stream);
     if (e != hipSuccess) return e;
   }
+  phase_end(ph, stream);
+  // Per request results, and the compacted offsets for requests that copy
+  // their pairs out.
   hipLaunchKernelGGL(k_request_exact, dim3(1024), dim3(kThreads), 0, stream, a);
   if (collect) {
     BatchArgs x = a;
@@ -452,25 +495,12 @@ hipError_t launch_batch(const TreeView* t, const BatchArgs& a_in, const BatchSor
     x.total = a.total + 2;
     launch_seg_scan(x, stream);
   }
-  phase_end(ph, stream);
   return hipGetLastError();
 }
 
 size_t batch_seg_sums_words() { return kScanBlocks; }
 
 namespace {
-
-// Request of global query q: the last request whose qbase <= q (requests are
-// laid out in query order).
-__device__ __forceinline__ uint32_t request_of_query(const BatchRequest* reqs, uint32_t nreq,
-                                                     uint64_t q) {
-  uint32_t lo = 0, hi = nreq - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (reqs[mid].qbase <= q) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
 
 // hashes[r] += pair_hash over request r's pairs: one wavefront per query
 // (grid-stride over the batch's queries), its lanes over the query's pairs.
