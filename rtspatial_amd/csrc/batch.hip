@@ -502,26 +502,68 @@ size_t batch_seg_sums_words() { return kScanBlocks; }
 
 namespace {
 
-// hashes[r] += pair_hash over request r's pairs: one wavefront per query
-// (grid-stride over the batch's queries), its lanes over the query's pairs.
+// hashes[r] += pair_hash over request r's pairs. The queries' slot ranges
+// [offsets[q], + counts[q]) tile the arena's [0, total), and query q's pairs
+// are the first exact[q] slots of its range. A wavefront takes blocks of
+// kHashBlock consecutive slots (lanes interleaved, so loads coalesce); each
+// lane finds the query of its first slot, then follows the ranges forward,
+// keeping a running sum that it adds to hashes[] when its request changes.
+// Where the whole wave ends a block in one request, one wave-reduced atomic
+// suffices.
+constexpr uint32_t kHashBlock = kWave * 256;
+
+__device__ __forceinline__ uint64_t query_of_slot(const uint64_t* offsets, uint64_t nq,
+                                                  uint64_t i) {
+  uint64_t lo = 0, hi = nq - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (offsets[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 __global__ void __launch_bounds__(kThreads) k_hash_queries(const BatchArgs a,
                                                            unsigned long long* __restrict__ hashes) {
   const BatchHeader h = *a.header;
-  if (h.n_requests == 0) return;
+  const uint64_t nq = h.n_queries;
+  const uint64_t hi = a.total[0] < a.arena_cap ? a.total[0] : a.arena_cap;
+  if (h.n_requests == 0 || nq == 0 || hi == 0) return;
   const uint32_t lane = lane_id();
   const uint64_t waves = (uint64_t)gridDim.x * kWaves;
-  for (uint64_t q = (uint64_t)blockIdx.x * kWaves + threadIdx.x / kWave; q < h.n_queries;
-       q += waves) {
-    const uint64_t o = a.offsets[q];
-    const uint64_t end = o + a.exact[q] < a.arena_cap ? o + a.exact[q] : a.arena_cap;
+  const uint64_t w0 = (uint64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  for (uint64_t b0 = w0 * kHashBlock; b0 < hi; b0 += waves * kHashBlock) {
+    const uint64_t b1 = b0 + kHashBlock < hi ? b0 + kHashBlock : hi;
+    uint64_t i = b0 + lane;
+    uint64_t q = query_of_slot(a.offsets, nq, i < b1 ? i : b0);
+    uint64_t q_end = a.offsets[q] + a.counts[q];
+    uint64_t v_end = a.offsets[q] + a.exact[q];
+    uint32_t r = request_of_query(a.requests, h.n_requests, q);
+    uint64_t r_end = r + 1 < h.n_requests ? a.requests[r + 1].qbase : nq;
     uint64_t hs = 0;
-    for (uint64_t i = o + lane; i < end; i += kWave) {
-      const rts_pair p = a.arena[i];
-      hs += pair_hash(p.query_id, p.geom_id);
+    for (; i < b1; i += kWave) {
+      while (i >= q_end && q + 1 < nq) {
+        q++;
+        q_end = a.offsets[q] + a.counts[q];
+        v_end = a.offsets[q] + a.exact[q];
+        while (q >= r_end) {
+          if (hs) atomicAdd(&hashes[r], (unsigned long long)hs);
+          hs = 0;
+          r++;
+          r_end = r + 1 < h.n_requests ? a.requests[r + 1].qbase : nq;
+        }
+      }
+      if (i < v_end && i >= a.offsets[q]) {
+        const rts_pair p = a.arena[i];
+        hs += pair_hash(p.query_id, p.geom_id);
+      }
     }
-    hs = wave_sum_u64(hs);
-    if (lane == 0 && hs)
-      atomicAdd(&hashes[request_of_query(a.requests, h.n_requests, q)], (unsigned long long)hs);
+    const uint32_t r0 = readlane_u32(r, 0);
+    if (__all(r == r0)) {
+      hs = wave_sum_u64(hs);
+      if (lane == 0 && hs) atomicAdd(&hashes[r0], (unsigned long long)hs);
+    } else if (hs) {
+      atomicAdd(&hashes[r], (unsigned long long)hs);
+    }
   }
 }
 
