@@ -226,6 +226,34 @@ def test_empty_and_single_query_requests(oracle, scene):
 
 
 @pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_short_output_keeps_the_first_pairs_in_query_order(oracle, scene, mode):
+    # `out` holds half of a one-chunk request's results: the count is still
+    # the whole count, `truncated` is set, and `out` holds the pairs of the
+    # request's first queries in query order, the last of them possibly in
+    # part. (The batched path gathers each query's pairs from the start of
+    # its slots; the plain path copies one contiguous range.)
+    rng, means, boxes = scene
+    log(f"serve truncated output mode={mode}")
+    q = W.mixture_boxes(np.random.default_rng(11), means, 3000, np.log(5e-3))
+    want = oracle.RTree(boxes).pairs(oracle.INTERSECTS, q)  # (query, geom), sorted
+    cap = want.shape[0] // 2
+    assert cap > 0
+    ix = make_index(boxes)
+    res, _ = S.run(ix, [S.Request(queries=dev(q), predicate=P.kIntersects, collect=cap)],
+                   mode=mode)
+    r = res[0]
+    assert r["count"] == want.shape[0] and r["truncated"]
+    assert r["pairs"].shape[0] == cap
+    qids = r["pairs"].cpu().numpy()[:, 1].astype(np.int64)
+    assert np.all(np.diff(qids) >= 0)
+    got = canon(r["pairs"])
+    last = got[-1, 0]
+    assert np.array_equal(got[got[:, 0] < last], want[want[:, 0] < last])
+    part = {tuple(x) for x in got[got[:, 0] == last]}
+    assert part <= {tuple(x) for x in want[want[:, 0] == last]}
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
 def test_empty_index_serves_empty_results(mode):
     # No box was ever inserted: every request completes with no results and
     # no pairs, in both modes (the batched count pass writes zero counts).
