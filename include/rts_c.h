@@ -376,11 +376,16 @@ int rts_index_memory(rts_index* index, rts_memory_usage* out);
 /* Autotuner. No reference counterpart: it stands where the reference's
  * CalculateBestParallelism sampling (spatial_index.cuh:41-57,690-760) picks a
  * ray parallelism. It times the query over an evenly strided sample of
- * `queries` (shrunk until its results are at most 2^27 pairs) under every
+ * `queries` (shrunk until its results are at most 2^26 pairs) under every
  * candidate plan -- leaf size, one query per lane or per wavefront (per lane
- * only when the sample averages at most 256 results per query), and, when the
- * sample is the whole set, Morton-ordered or input-ordered queries -- with
- * the given handler. With `apply` set, it stores the fastest plan in the index for this
+ * only when the sample averages at most 256 results per query) -- with the
+ * given handler, and then Morton-ordered against input-ordered queries on the
+ * whole set when that is cheap (at most 4M queries and, going by the sample,
+ * 2^26 results; a strided sample has none of the set's locality). The
+ * ordering measured at n queries moves the size rule's threshold (65536
+ * queries): calls of at least min(n, 65536) queries are ordered when it won,
+ * calls of at most max(n, 65535) are not when it lost. With `apply` set, it
+ * stores the fastest plan in the index for this
  * (predicate, query kind); later queries of that kind, and the batches of
  * rts_serve_run, use it for every setting their config leaves on auto. The
  * plan survives rts_index_clear. Results are identical under every plan;
@@ -397,8 +402,8 @@ typedef struct rts_tune_config {
 typedef struct rts_tune_result {
   uint32_t leaf_size;      /* the winner */
   uint32_t traversal;      /* RTS_TRAVERSAL_THREAD or RTS_TRAVERSAL_WAVE */
-  uint32_t sort_queries;   /* 1 = Morton-ordered, 2 = input order; 0 = not tuned (the
-                              sample was smaller than the query set), the call's size decides */
+  uint32_t sort_queries;   /* 1 = Morton-ordered, 2 = input order; 0 = not tuned (the whole
+                              set was too large to time), the call's size decides */
   uint32_t plans;          /* plans timed */
   uint64_t sample;         /* queries in the sample */
   uint64_t sample_results; /* results of the sample */

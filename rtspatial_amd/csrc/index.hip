@@ -757,6 +757,12 @@ struct Plan {
 // ordering of batches of 64K queries or more (neighbouring lanes share their
 // descent; neighbouring waves, which take consecutive queries on one XCD,
 // share that XCD's L2).
+// Calls of at least this many queries are Morton-ordered unless a tuning
+// moved the threshold (TunedPlan::sort_min_n).
+constexpr uint64_t kSortMinQueries = 65536;
+// The autotuner times query ordering on the whole query set up to this size.
+constexpr uint64_t kOrderTuneQueries = 1ull << 22;
+
 Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
   // Segment queries (kPredLine) are not tuned: they take the config or the defaults.
   static const rts_index::TunedPlan kUntuned{};
@@ -772,9 +778,9 @@ Plan plan_query(const rts_index* ix, int pred, bool points, uint64_t nq) {
   if (ix->cfg.sort_queries == 1 || ix->cfg.sort_queries == 2)
     p.sort = ix->cfg.sort_queries == 1;
   else if (t.valid && t.sort_tuned)
-    p.sort = t.sort;
+    p.sort = nq >= t.sort_min_n;
   else
-    p.sort = nq >= 65536;
+    p.sort = nq >= kSortMinQueries;
   p.leaf_size = t.valid ? t.leaf_size : ix->cfg.leaf_size;
   return p;
 }
@@ -941,11 +947,13 @@ int run_lines(rts_index* ix, int pred, const void* queries, uint64_t nq, bool do
 struct TuneBuffers {
   void* sample = nullptr;
   rts_pair* pairs = nullptr;
-  char* small = nullptr;  // [0, 24): rts_dev_queue, [32, 40): counter
+  rts_pair* all_pairs = nullptr;  // results of the whole query set (ordering runs)
+  char* small = nullptr;  // [0, 24): rts_dev_queue, [32, 40): counter, [40, 64): a second queue
   hipEvent_t ev[2] = {nullptr, nullptr};
   ~TuneBuffers() {
     if (sample) (void)hipFree(sample);
     if (pairs) (void)hipFree(pairs);
+    if (all_pairs) (void)hipFree(all_pairs);
     if (small) (void)hipFree(small);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -1306,7 +1314,8 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
   // far is not repeated.
   double best = 1e300;
   bool warm[3] = {false, false, false};
-  auto time_plan = [&](const Plan& p, double* ms_out) -> int {
+  auto time_on = [&](const Plan& p, const void* qs, uint64_t qn, void* qarg,
+                     double* ms_out) -> int {
     float ms = 0.0f;
     double fastest = 1e300;
     const uint32_t r0 = warm[p.traversal] ? 1u : 0u;
@@ -1314,7 +1323,7 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     for (uint32_t r = r0; r <= cfg.reps; ++r) {
       RTS_HIP(hipMemsetAsync(counter, 0, sizeof(*counter), stream));
       RTS_HIP(hipEventRecord(tb.ev[0], stream));
-      int e = run_plan(ix, p, pred, tb.sample, m, pts, f64, handler, arg, stream);
+      int e = run_plan(ix, p, pred, qs, qn, pts, f64, handler, qarg, stream);
       if (e) return e;
       RTS_HIP(hipEventRecord(tb.ev[1], stream));
       RTS_HIP(hipEventSynchronize(tb.ev[1]));
@@ -1327,6 +1336,9 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     }
     *ms_out = fastest;
     return RTS_OK;
+  };
+  auto time_plan = [&](const Plan& p, double* ms_out) -> int {
+    return time_on(p, tb.sample, m, arg, ms_out);
   };
 
   double default_ms = 0;
@@ -1374,19 +1386,63 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     }
   next_mode:;
   }
+  // Query ordering pays through the walks' cache locality, which a strided
+  // sample does not have. When the whole set is cheap to run -- at most
+  // kOrderTuneQueries queries and, going by the sample, at most kMaxPairs
+  // results -- the winning plan is timed on the whole set in both orders.
+  // (On the 100k x 100k point query, ordering cost 0.088 ms and saved
+  // 0.022 ms of traversal: 0.164 against 0.100 ms per call.)
+  bool order_tuned = whole, order = winner.sort;
+  const double est_all = (double)total * (double)n / (double)std::max<uint64_t>(m, 1);
+  if (!whole && n <= kOrderTuneQueries && ix->cfg.coord != RTS_COORD_F64_AS_F32 &&
+      est_all <= (double)kMaxPairs) {
+    void* all_arg = counter;
+    uint64_t cap = 0;
+    if (handler == RTS_HANDLER_COLLECT) {
+      cap = (uint64_t)(est_all * 1.5) + 4096;
+      RTS_HIP(hipMalloc(reinterpret_cast<void**>(&tb.all_pairs), cap * sizeof(rts_pair)));
+      rts_dev_queue q{tb.all_pairs, cap, counter};
+      RTS_HIP(hipMemcpyAsync(tb.small + 40, &q, sizeof(q), hipMemcpyHostToDevice, stream));
+      all_arg = tb.small + 40;
+    }
+    double ms_in[2] = {0, 0};
+    bool fits = true;
+    for (int o = 0; o < 2 && fits; ++o) {
+      Plan p = winner;
+      p.sort = o == 1;
+      rc = time_on(p, queries, n, all_arg, &ms_in[o]);
+      if (rc) return rc;
+      if (handler == RTS_HANDLER_COLLECT) {
+        unsigned long long got = 0;
+        RTS_HIP(hipMemcpyAsync(&got, counter, sizeof(got), hipMemcpyDeviceToHost, stream));
+        RTS_HIP(hipStreamSynchronize(stream));
+        fits = got <= cap;
+      }
+    }
+    if (fits) {
+      order_tuned = true;
+      order = ms_in[1] < ms_in[0];
+      plans += 2;
+    }
+  }
   rc = check_device_error(ix);
   if (rc) return rc;
   if (cfg.apply) {
     rts_index::TunedPlan& t = ix->tuned[pred][pts ? 1 : 0];
     t.valid = true;
     t.traversal = (uint32_t)winner.traversal;
-    t.sort = winner.sort;
-    t.sort_tuned = whole;
+    t.sort = order;
+    t.sort_tuned = order_tuned;
+    // The ordering measured at n queries moves the size rule's threshold:
+    // calls of at least min(n, the rule's size) are ordered when ordering
+    // won, calls of at most max(n, ...) are not when it lost.
+    t.sort_min_n = order ? std::min<uint64_t>(n, kSortMinQueries)
+                         : std::max<uint64_t>(n + 1, kSortMinQueries);
     t.leaf_size = winner.leaf_size;
   }
   out->leaf_size = winner.leaf_size;
   out->traversal = (uint32_t)winner.traversal;
-  out->sort_queries = whole ? (winner.sort ? 1 : 2) : 0;
+  out->sort_queries = order_tuned ? (order ? 1 : 2) : 0;
   out->plans = plans;
   out->sample = m;
   out->sample_results = total;

@@ -82,7 +82,8 @@ struct rts_index {
     bool valid = false;
     uint32_t traversal = 0;
     bool sort = false;
-    bool sort_tuned = false;  // the sample was the whole query set
+    bool sort_tuned = false;  // ordering was timed on the whole query set
+    uint64_t sort_min_n = 0;  // with sort_tuned: calls of at least this many queries are ordered
     uint32_t leaf_size = 0;
   };
   TunedPlan tuned[3][2];

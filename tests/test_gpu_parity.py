@@ -434,15 +434,17 @@ def test_autotune_keeps_results_exact(oracle, kind):
     want = oracle.RTree(b).pairs(int(pred), q)
     dq = dev(q)
     r = ix.Autotune(pred, dq, sample=4096, reps=2)
-    # A strided sample: 8 plans (4 leaf sizes x 2 traversals, the query order
-    # left to the call's size), 5 when a traversal mode is dropped after its
-    # first plan, 4 when the sample's queries average more than 256 results
-    # (one query per lane is then not timed at all).
+    # A strided sample: 8 plans (4 leaf sizes x 2 traversals), 5 when a
+    # traversal mode is dropped after its first plan, 4 when the sample's
+    # queries average more than 256 results (one query per lane is then not
+    # timed at all); then the winner in both query orders on the whole set
+    # (30k queries: cheap to run), 2 more.
     lane_hopeless = r["sample_results"] > 256 * r["sample"]
     lo_plans = 4 if lane_hopeless else 5
-    assert r["sample"] == 4096 and lo_plans <= r["plans"] <= 8 and r["sort_queries"] == 0
+    assert r["sample"] == 4096 and lo_plans + 2 <= r["plans"] <= 10
+    assert r["sort_queries"] in (1, 2)
     if lane_hopeless:
-        assert r["plans"] == 4 and r["traversal"] == WAVE
+        assert r["plans"] == 6 and r["traversal"] == WAVE
     assert r["leaf_size"] in (4, 8, 16, 32) and r["traversal"] in (THREAD, WAVE)
     assert 0 < r["best_ms"] <= r["default_ms"]
     sample_want = oracle.RTree(b).count(int(pred), q[:: q.shape[0] // 4096][:4096])
