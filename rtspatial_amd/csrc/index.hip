@@ -1407,6 +1407,10 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
     }
     double ms_in[2] = {0, 0};
     bool fits = true;
+    // Both orders get their full repetitions (the early exit compares
+    // against the sample's best time).
+    const double sample_best = best;
+    best = 1e300;
     for (int o = 0; o < 2 && fits; ++o) {
       Plan p = winner;
       p.sort = o == 1;
@@ -1419,6 +1423,7 @@ int rts_index_autotune(rts_index* ix, rts_predicate pred, const void* queries, u
         fits = got <= cap;
       }
     }
+    best = sample_best;
     if (fits) {
       order_tuned = true;
       order = ms_in[1] < ms_in[0];
