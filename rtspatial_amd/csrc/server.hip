@@ -3,22 +3,23 @@
 // Query call at a time (spatial_index.cuh:352-515).
 //
 // Batched mode is continuous batching over `slots` streams. Each slot owns
-// device buffers for one batch (descriptors, per-query counts and offsets,
-// result arena) and a pinned staging area; its launch sequence (descriptor
-// upload, count pass, scan, request ranges, write pass, range download) is
+// device buffers for one batch (descriptors, per-query result bounds, exact
+// counts and offsets, result arena) and a pinned staging area; its launch
+// sequence (descriptor upload, count pass, scan, request ranges, write pass,
+// per-request results, range download) is
 // launched directly for every batch. (Replaying that sequence from a hipGraph
 // did not complete on the MI355X, with the cause not established, so
 // rts_serve_run refuses use_graphs.)
 //
 // Admission: arrived requests are taken in arrival order and split into
 // chunks of at most chunk_queries queries; chunks join the batch while their
-// predicted results (an estimate of results per query for each predicate,
-// refreshed from every completed chunk) fit the slot's arena. While later
+// predicted arena slots (an estimate of slots per query for each predicate,
+// refreshed from every completed chunk; a query's slots are a bound on its
+// results) fit the slot's arena. While later
 // requests wait, one request takes at most head_share_pct percent of the
 // remaining budget, and
 // requests that do not fit are skipped for later ones that do (backfill). A
-// chunk whose actual results overflow the arena is re-launched in a later
-// batch.
+// chunk whose slots overflow the arena is re-launched in a later batch.
 //
 // Updates: once an update batch has arrived, no new request is admitted; when
 // the requests already started are fully admitted and their batches are done
